@@ -1,0 +1,213 @@
+/*
+ * occxc.h -- C ABI of the MI355X-native Kohn-Sham exchange-correlation build.
+ *
+ * The boundary replaces the body of OCC's
+ *     Mat DFT::compute_vxc(const MolecularOrbitals &mo, const Mat &Schwarz)
+ *     (include/occ/dft/dft.h:395-429, dispatching to compute_K_dft<d,kind>,
+ *      dft.h:190-388)
+ * plus the value read back by
+ *     double DFT::exchange_correlation_energy() const   (dft.h:83)
+ * and additionally returns N_elec, which OCC only logs (dft.h:384-385).
+ *
+ * Conventions kept from OCC:
+ *   - basis shells in AOBasis order (atom order, then per-element JSON shell
+ *     order, general contractions split, src/gto/shell.cpp:529-538);
+ *     coefficients normalised as Shell::incorporate_shell_norm leaves them
+ *     (shell.cpp:180-186), without the common_fac(l) factor (gto.cpp:7-16);
+ *     spherical components in CCA order m=-l..l, Cartesian xx,xy,xz,yy,yz,zz
+ *     (include/occ/gto/shell_order.h:8-23);
+ *   - lengths in Bohr (units.h:7-8);
+ *   - D is the restricted density C_occ C_occ^T, nbf x nbf column-major, with
+ *     no factor 2 (include/occ/qm/orb.h:64-66); the build doubles it
+ *     (dft.h:205) and reads only the upper triangle (src/gto/density.cpp:14,38);
+ *   - the returned matrix is what compute_vxc returns (jk.K = -Vxc at
+ *     dft.h:435);
+ *   - functionals are named by their libxc ids (src/dft/functional.cpp:78-90).
+ *
+ * No function throws; every entry point returns an occxc_status and leaves a
+ * message for occxc_last_error_message().  Host buffers are caller-owned;
+ * device buffers inside a context are library-owned.  A context is
+ * single-writer; distinct contexts may be used from different threads.
+ */
+#ifndef OCCXC_H
+#define OCCXC_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  OCCXC_OK = 0,
+  OCCXC_ERR_INVALID_ARGUMENT = 1,
+  OCCXC_ERR_UNSUPPORTED = 2,      /* General spinorbitals, d > 1, unknown id */
+  OCCXC_ERR_DEVICE = 3,           /* HIP runtime error or no device */
+  OCCXC_ERR_OUT_OF_MEMORY = 4,
+  OCCXC_ERR_ADMISSION = 5,        /* serving: footprint exceeds the budget */
+  OCCXC_ERR_CANCELLED = 6,
+  OCCXC_ERR_INTERNAL = 7
+} occxc_status;
+
+/* libxc functional ids understood by the build */
+enum {
+  OCCXC_LDA_X = 1,
+  OCCXC_LDA_C_VWN = 7,        /* VWN5 */
+  OCCXC_LDA_C_PW = 12,
+  OCCXC_LDA_C_PW_MOD = 13,
+  OCCXC_GGA_X_PBE = 101,
+  OCCXC_GGA_X_B88 = 106,
+  OCCXC_GGA_C_PBE = 130,
+  OCCXC_GGA_C_LYP = 131
+};
+
+typedef enum {
+  OCCXC_PREC_FP64 = 0,   /* FP64 AO cache and contractions */
+  OCCXC_PREC_FP32 = 1,   /* FP32 AO cache, FP32 contractions, FP32 accumulation */
+  OCCXC_PREC_BF16X3 = 2  /* FP32 AO cache, 3-term split-BF16 MFMA, FP32 accumulation */
+} occxc_precision;
+
+typedef struct {
+  int nshell;
+  const int *l;              /* [nshell] angular momentum, 0..4 */
+  const int *pure;           /* [nshell] 1 spherical, 0 Cartesian */
+  const int *nprim;          /* [nshell] */
+  const double *origin;      /* [3*nshell], Bohr */
+  const double *exponents;   /* [sum nprim] */
+  const double *coefficients;/* [sum nprim], normalised (see above) */
+} occxc_basis;
+
+typedef struct {
+  long long npts;
+  const double *points;      /* [3*npts] x,y,z interleaved, Bohr */
+  const double *weights;     /* [npts] */
+} occxc_grid;
+
+typedef struct {
+  int nfunc;
+  const int *ids;            /* libxc ids, summed like Result::operator+= */
+} occxc_functional;
+
+typedef struct {
+  int precision;             /* occxc_precision */
+  double density_threshold;  /* leaf skip, DFT::m_density_threshold = 1e-10 (dft.h:760) */
+  double screening_threshold;/* shell decay cutoff, m_xc_screening_threshold = 1e-10 (dft.h:761) */
+  int screen;                /* 1: screened leaf path (dft.h:279-372); 0: dense path (OCC_XC_NO_SCREEN) */
+  int device;                /* HIP device ordinal */
+} occxc_options;
+
+typedef struct occxc_context occxc_context;
+
+typedef struct {
+  long long npts;            /* grid points */
+  long long nleaves;         /* leaves with >= 1 significant shell */
+  int nbf;
+  int nshell;
+  double mean_local_nbf;     /* mean m_b over active leaves */
+  double contraction_flops;  /* sum_b 4 n_b m_b^2 (screened work definition) */
+  double ao_bytes;           /* bytes of the resident AO values */
+  double ao_write_bytes;     /* algorithmic bytes of one AO evaluation */
+  double device_bytes;       /* total device footprint of the context */
+} occxc_context_info;
+
+/* --- the XC build ---------------------------------------------------- */
+
+void occxc_default_options(occxc_options *opt);
+
+occxc_status occxc_context_create(const occxc_basis *basis,
+                                  const occxc_grid *grid,
+                                  const occxc_functional *func,
+                                  const occxc_options *opt,
+                                  occxc_context **ctx);
+occxc_status occxc_context_destroy(occxc_context *ctx);
+occxc_status occxc_context_get_info(const occxc_context *ctx,
+                                    occxc_context_info *info);
+
+/* One XC build from host buffers: D (nbf*nbf) in; Vxc (nbf*nbf), Exc, N_elec
+ * out.  Mirrors compute_vxc + exchange_correlation_energy. */
+occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc,
+                         double *exc, double *nelec);
+
+/* Same build on device-resident buffers, enqueued on `stream` (a hipStream_t,
+ * NULL for the context's own stream) without host synchronisation.
+ * d_result receives {Exc, N_elec}. */
+occxc_status occxc_build_device(occxc_context *ctx, const double *d_D,
+                                double *d_Vxc, double *d_result, void *stream);
+
+/* Wait for work enqueued on the context's stream. */
+occxc_status occxc_synchronize(occxc_context *ctx);
+
+/* Re-evaluate the resident AO values of the context (done once at creation;
+ * a build that must not rely on residency calls this first).  Enqueued on
+ * the context's stream. */
+occxc_status occxc_context_evaluate_ao(occxc_context *ctx);
+
+/* --- stage-level entry points (each mirrors one reference function) --- */
+
+/* gto::evaluate_basis (src/gto/gto.cpp:69-140): phi on npts points for the
+ * whole basis, times common_fac.  out is [ncomp][nbf][npts] (point fastest),
+ * ncomp = 1 (deriv 0) or 4 (deriv 1: phi, phi_x, phi_y, phi_z). */
+occxc_status occxc_evaluate_basis(const occxc_basis *basis, long long npts,
+                                  const double *points, int deriv,
+                                  double *out, int device);
+
+/* gto::evaluate_decay_cutoff (src/gto/gto.cpp:18-67). */
+occxc_status occxc_evaluate_decay_cutoff(const occxc_basis *basis,
+                                         double threshold, double *extents,
+                                         int device);
+
+/* DensityFunctional::evaluate (src/dft/functional.cpp:18-65), unpolarised:
+ * ACCUMULATES into exc/vrho/vsigma like Result::operator+= (functional.h:
+ * 436-457).  sigma and vsigma may be NULL for LDA ids. */
+occxc_status occxc_functional_evaluate(int id, long long n, const double *rho,
+                                       const double *sigma, double *exc,
+                                       double *vrho, double *vsigma,
+                                       int device);
+
+/* SpatialGridHierarchy (src/numint/spatial_grid_hierarchy.cpp:28-147):
+ * Morton sort + 128-point leaves.  Returns the number of leaves in *nleaves;
+ * leaf arrays must hold npts/32 + 2 entries. */
+occxc_status occxc_grid_hierarchy(long long npts, const double *points,
+                                  const double *weights, double *sorted_points,
+                                  double *sorted_weights, long long *perm,
+                                  long long *leaf_offset, long long *leaf_count,
+                                  double *leaf_center, double *leaf_radius,
+                                  long long *nleaves);
+
+/* --- molecular grid construction (src/numint) ------------------------- */
+
+typedef enum {
+  OCCXC_RADIAL_EULER_MACLAURIN = 0,
+  OCCXC_RADIAL_TREUTLER_AHLRICHS = 1,
+  OCCXC_RADIAL_BECKE = 2,
+  OCCXC_RADIAL_GAUSS_CHEBYSHEV = 3,
+  OCCXC_RADIAL_MURA_KNOWLES = 4
+} occxc_radial_method;
+
+/* Raw radial rules of grid_utils.cpp (generate_*_radial_grid, :337-572):
+ * n points requested, `param` = alpha (EM), rm (Becke) or charge (MK).
+ * *npoints_out receives the number written (EM returns n-1, :382). */
+occxc_status occxc_radial_grid(int method, int n, double param, double *r,
+                               double *w, int *npoints_out);
+
+/* Becke partition with Treutler-Ahlrichs radii adjustment
+ * (calculate_atomic_grid_weights, grid_utils.cpp:125-232) and the per-atom
+ * normalisation of MolecularGrid::get_partitioned_atom_grid
+ * (molecular_grid.cpp:46-82): weights[i] *= P_{atom[i]} / sum_k P_k.
+ * radii are the Bragg radii in Bohr (grid_utils.cpp:22-41). */
+occxc_status occxc_becke_partition(int natoms, const double *atom_xyz,
+                                   const double *radii, int ta_adjustment,
+                                   long long npts, const double *points,
+                                   const int *atom_of_point, double *weights,
+                                   int device);
+
+/* --- errors ------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *occxc_last_error_message(void);
+const char *occxc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCCXC_H */

@@ -1,0 +1,780 @@
+// api.cpp -- the occxc C ABI: context set-up (grid hierarchy, screening,
+// device layout), the XC build orchestration and the stage entry points.
+//
+// The context holds, per molecule, everything OCC's DFT object caches or
+// recomputes per call in compute_K_dft (include/occ/dft/dft.h:190-388): the
+// Morton-sorted grid and its leaves (spatial_grid_hierarchy.cpp:28-147), the
+// shell extents (gto.cpp:18-67) and per-leaf significant shells
+// (dft.h:307-336), and, resident on the device, the AO values of every leaf.
+// A build then runs: D2 = 2 sym(D) -> density + functional per leaf -> Vxc per
+// leaf -> K + K^T and the fixed-order reduction of Exc and N_elec.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/occxc.h"
+#include "common.hpp"
+#include "contract.hpp"
+#include "kernels.hpp"
+#include "xc_functionals.hpp"
+
+namespace occxc {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+void clear_last_error() { g_last_error.clear(); }
+
+// Split [0, n) over host threads (set-up work only).
+static void parallel_for(long long n, const std::function<void(long long, long long)> &fn) {
+  unsigned hw = std::thread::hardware_concurrency();
+  int nt = (int)std::min<unsigned>(hw ? hw : 1, 16);
+  if (n < 4096 || nt <= 1) {
+    fn(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  long long chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    long long lo = t * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    th.emplace_back(fn, lo, hi);
+  }
+  for (auto &t : th) t.join();
+}
+
+static int shell_size(int l, int pure) { return pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2; }
+
+static double common_fac(int l, int pure) {
+  // gto.cpp:7-16
+  if (l == 0) return 0.282094791773878143;
+  if (l == 1) return 0.488602511902919921;
+  return pure ? std::sqrt((2.0 * l + 1.0) / (4.0 * 3.14159265358979323846)) : 1.0;
+}
+
+static bool g_table_ready[64] = {false};
+
+static void ensure_device_tables(int device) {
+  if (device >= 0 && device < 64 && !g_table_ready[device]) {
+    upload_sph_table();
+    g_table_ready[device] = true;
+  }
+}
+
+struct HostBasis {
+  int nsh = 0, nbf = 0, maxl = 0;
+  std::vector<ShellDev> shells;
+  std::vector<double> exps, coefs;
+};
+
+static HostBasis make_basis(const occxc_basis *b) {
+  OCCXC_REQUIRE(b && b->nshell > 0 && b->l && b->pure && b->nprim && b->origin && b->exponents &&
+                    b->coefficients,
+                OCCXC_ERR_INVALID_ARGUMENT, "basis: null pointer or no shells");
+  HostBasis h;
+  h.nsh = b->nshell;
+  int poff = 0;
+  for (int s = 0; s < b->nshell; ++s) {
+    const int l = b->l[s], np = b->nprim[s];
+    OCCXC_REQUIRE(l >= 0 && l <= kMaxL, OCCXC_ERR_UNSUPPORTED,
+                  "basis: angular momentum above " + std::to_string(kMaxL) + " (shell " +
+                      std::to_string(s) + ")");
+    OCCXC_REQUIRE(np >= 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "basis: shell " + std::to_string(s) + " has no primitives");
+    ShellDev sh{};
+    sh.x = b->origin[3 * s];
+    sh.y = b->origin[3 * s + 1];
+    sh.z = b->origin[3 * s + 2];
+    sh.l = l;
+    sh.pure = b->pure[s] ? 1 : 0;
+    sh.nprim = np;
+    sh.poff = poff;
+    sh.bf0 = h.nbf;
+    sh.size = shell_size(l, sh.pure);
+    const double fac = common_fac(l, sh.pure);
+    for (int k = 0; k < np; ++k) {
+      const double a = b->exponents[poff + k], c = b->coefficients[poff + k];
+      OCCXC_REQUIRE(std::isfinite(a) && a > 0.0 && std::isfinite(c), OCCXC_ERR_INVALID_ARGUMENT,
+                    "basis: non-positive or non-finite exponent/coefficient in shell " +
+                        std::to_string(s));
+      h.exps.push_back(a);
+      h.coefs.push_back(c * fac);
+    }
+    h.shells.push_back(sh);
+    h.nbf += sh.size;
+    h.maxl = std::max(h.maxl, l);
+    poff += np;
+  }
+  return h;
+}
+
+// ---------------------------------------------------- grid hierarchy --
+
+static inline uint64_t spread_bits_3(uint64_t v) {
+  // insert two zero bits between each of the low 21 bits
+  v &= 0x1fffffULL;
+  uint64_t r = 0;
+  for (int i = 0; i < 21; ++i) r |= ((v >> i) & 1ULL) << (3 * i);
+  return r;
+}
+
+struct Hierarchy {
+  std::vector<double> xyz, w;       // sorted
+  std::vector<long long> perm;
+  std::vector<long long> off, cnt;
+  std::vector<double> center, radius;
+};
+
+// SpatialGridHierarchy (spatial_grid_hierarchy.cpp:28-147): 21-bit-per-axis
+// Morton codes over the bounding box, sort by (code, index), leaves of 128
+// (min 32, max 256) with centroid / max-distance spheres.
+static Hierarchy build_hierarchy(long long n, const double *xyz, const double *w) {
+  Hierarchy h;
+  if (n <= 0) return h;
+  double mn[3], mx[3], ext[3];
+  for (int k = 0; k < 3; ++k) mn[k] = mx[k] = xyz[k];
+  for (long long i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = std::min(mn[k], xyz[3 * i + k]);
+      mx[k] = std::max(mx[k], xyz[3 * i + k]);
+    }
+  for (int k = 0; k < 3; ++k) {
+    ext[k] = mx[k] - mn[k];
+    if (ext[k] < 1e-10) ext[k] = 1.0;
+  }
+  std::vector<std::pair<uint64_t, long long>> codes(n);
+  const uint64_t scale = (1ULL << 21) - 1;
+  parallel_for(n, [&](long long lo, long long hi) {
+    for (long long i = lo; i < hi; ++i) {
+      uint64_t q[3];
+      for (int k = 0; k < 3; ++k) {
+        double t = (xyz[3 * i + k] - mn[k]) / ext[k];
+        t = std::min(std::max(t, 0.0), 1.0);
+        q[k] = (uint64_t)(t * (double)scale);
+      }
+      codes[i] = {spread_bits_3(q[0]) | (spread_bits_3(q[1]) << 1) | (spread_bits_3(q[2]) << 2), i};
+    }
+  });
+  std::sort(codes.begin(), codes.end());
+  h.xyz.resize(3 * n);
+  h.w.resize(n);
+  h.perm.resize(n);
+  for (long long i = 0; i < n; ++i) {
+    const long long o = codes[i].second;
+    h.perm[i] = o;
+    for (int k = 0; k < 3; ++k) h.xyz[3 * i + k] = xyz[3 * o + k];
+    h.w[i] = w[o];
+  }
+  const long long target = 128, minl = 32, maxl = 256;
+  long long off = 0;
+  while (off < n) {
+    const long long rem = n - off;
+    long long cnt;
+    if (rem <= maxl) cnt = rem;
+    else if (rem < 2 * minl) cnt = rem / 2;
+    else cnt = std::min(target, rem);
+    double c[3] = {0, 0, 0};
+    for (long long i = 0; i < cnt; ++i)
+      for (int k = 0; k < 3; ++k) c[k] += h.xyz[3 * (off + i) + k];
+    for (int k = 0; k < 3; ++k) c[k] /= (double)cnt;
+    double r = 0.0;
+    for (long long i = 0; i < cnt; ++i) {
+      const double dx = h.xyz[3 * (off + i)] - c[0], dy = h.xyz[3 * (off + i) + 1] - c[1],
+                   dz = h.xyz[3 * (off + i) + 2] - c[2];
+      r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    h.off.push_back(off);
+    h.cnt.push_back(cnt);
+    h.center.insert(h.center.end(), c, c + 3);
+    h.radius.push_back(r);
+    off += cnt;
+  }
+  return h;
+}
+
+static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+} // namespace occxc
+
+using namespace occxc;
+
+struct occxc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  occxc_options opt{};
+  int nbf = 0, nsh = 0, deriv = 0, ncomp = 1, nfunc = 0;
+  int funcs[kMaxFunc] = {0};
+  long long npts = 0, ptot = 0;
+  int nleaves = 0;
+  double mean_m = 0.0, flops = 0.0, ao_write_bytes = 0.0;
+  size_t ao_elems = 0, ao_elem_size = 8;
+  std::vector<LeafDev> leaves;
+  DevBuf<ShellDev> d_shells;
+  DevBuf<double> d_exps, d_coefs;
+  DevBuf<LeafDev> d_leaves;
+  DevBuf<int> d_leaf_shells, d_ao_map, d_active;
+  DevBuf<double> d_px, d_py, d_pz, d_pw;
+  DevBuf<char> d_ao;
+  DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
+  ContractPlan plan;
+  BuildArgs args{};
+  bool ao_ready = false;
+  size_t device_bytes = 0;
+};
+
+namespace {
+
+template <typename F> occxc_status guarded(F &&f) {
+  try {
+    clear_last_error();
+    f();
+    return OCCXC_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return OCCXC_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return OCCXC_ERR_INTERNAL;
+  }
+}
+
+void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid *grid,
+                   const occxc_functional *func, const occxc_options *opt) {
+  c->opt = *opt;
+  OCCXC_REQUIRE(opt->precision >= 0 && opt->precision <= 2, OCCXC_ERR_INVALID_ARGUMENT,
+                "options: unknown precision mode");
+  OCCXC_REQUIRE(grid && grid->npts >= 0 && (grid->npts == 0 || (grid->points && grid->weights)),
+                OCCXC_ERR_INVALID_ARGUMENT, "grid: null pointer");
+  OCCXC_REQUIRE(grid->npts < (1LL << 31) - kMaxLeafPts * 4, OCCXC_ERR_UNSUPPORTED,
+                "grid: more than 2^31 points");
+  OCCXC_REQUIRE(func && func->nfunc >= 1 && func->nfunc <= kMaxFunc && func->ids,
+                OCCXC_ERR_INVALID_ARGUMENT, "functional: need 1..8 functional ids");
+  int deriv = 0;
+  for (int f = 0; f < func->nfunc; ++f) {
+    const int fam = xc_family(func->ids[f]);
+    OCCXC_REQUIRE(fam >= 0, OCCXC_ERR_UNSUPPORTED,
+                  "functional id " + std::to_string(func->ids[f]) + " not found");
+    deriv = std::max(deriv, fam);
+    c->funcs[f] = func->ids[f];
+  }
+  c->nfunc = func->nfunc;
+  c->deriv = deriv;
+  c->ncomp = deriv ? 4 : 1;
+  HostBasis hb = make_basis(basis);
+  c->nbf = hb.nbf;
+  c->nsh = hb.nsh;
+  for (long long i = 0; i < grid->npts; ++i) {
+    OCCXC_REQUIRE(std::isfinite(grid->points[3 * i]) && std::isfinite(grid->points[3 * i + 1]) &&
+                      std::isfinite(grid->points[3 * i + 2]) && std::isfinite(grid->weights[i]),
+                  OCCXC_ERR_INVALID_ARGUMENT, "grid: non-finite point or weight");
+  }
+
+  DeviceGuard guard(opt->device);
+  c->device = opt->device;
+  ensure_device_tables(c->device);
+  OCCXC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->d_shells.alloc(hb.shells.size());
+  c->d_shells.upload(hb.shells.data(), hb.shells.size(), c->stream);
+  c->d_exps.alloc(hb.exps.size());
+  c->d_exps.upload(hb.exps.data(), hb.exps.size(), c->stream);
+  c->d_coefs.alloc(hb.coefs.size());
+  c->d_coefs.upload(hb.coefs.data(), hb.coefs.size(), c->stream);
+
+  // shell extents on the device (gto.cpp:18-67)
+  std::vector<double> ext(hb.nsh, 30.0);
+  const bool screened = opt->screen != 0 && opt->screening_threshold > 0.0;
+  if (screened) {
+    DevBuf<double> d_ext(hb.nsh);
+    launch_decay_cutoff(c->d_shells.get(), hb.nsh, c->d_exps.get(), c->d_coefs.get(),
+                        opt->screening_threshold, d_ext.get(), c->stream);
+    d_ext.download(ext.data(), hb.nsh, c->stream);
+    OCCXC_HIP(hipStreamSynchronize(c->stream));
+  }
+
+  // leaves: Morton hierarchy (screened) or 128-point chunks in input order
+  Hierarchy h;
+  const long long n = grid->npts;
+  c->npts = n;
+  if (screened) {
+    h = build_hierarchy(n, grid->points, grid->weights);
+  } else {
+    h.xyz.assign(grid->points, grid->points + 3 * n);
+    h.w.assign(grid->weights, grid->weights + n);
+    for (long long o = 0; o < n; o += 128) {
+      h.off.push_back(o);
+      h.cnt.push_back(std::min<long long>(128, n - o));
+      h.center.insert(h.center.end(), {0.0, 0.0, 0.0});
+      h.radius.push_back(0.0);
+    }
+  }
+  const long long nl_all = (long long)h.off.size();
+  // significant shells per leaf (dft.h:314-320)
+  std::vector<std::vector<int>> sig(nl_all);
+  parallel_for(nl_all, [&](long long lo, long long hi) {
+    for (long long li = lo; li < hi; ++li) {
+      if (h.cnt[li] == 0) continue;
+      auto &v = sig[li];
+      for (int s = 0; s < hb.nsh; ++s) {
+        if (screened) {
+          const ShellDev &sh = hb.shells[s];
+          const double dx = sh.x - h.center[3 * li], dy = sh.y - h.center[3 * li + 1],
+                       dz = sh.z - h.center[3 * li + 2];
+          const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+          if (dist - ext[s] < h.radius[li]) v.push_back(s);
+        } else {
+          v.push_back(s);
+        }
+      }
+    }
+  });
+  std::vector<int> leaf_shells, ao_map;
+  std::vector<double> px, py, pz, pw;
+  long long pt_off = 0, ao_off = 0;
+  double msum = 0.0;
+  for (long long li = 0; li < nl_all; ++li) {
+    if (sig[li].empty()) continue;
+    LeafDev L{};
+    L.n = (int)h.cnt[li];
+    L.np = (int)round_up(L.n, kPtPad);
+    L.pt_off = (int)pt_off;
+    L.sh_off = (int)leaf_shells.size();
+    L.nsig = (int)sig[li].size();
+    L.map_off = (int)ao_map.size();
+    int m = 0;
+    for (int s : sig[li]) {
+      leaf_shells.push_back(s);
+      for (int q = 0; q < hb.shells[s].size; ++q) ao_map.push_back(hb.shells[s].bf0 + q);
+      m += hb.shells[s].size;
+    }
+    OCCXC_REQUIRE(m <= kMaxLocalAo, OCCXC_ERR_UNSUPPORTED,
+                  "a grid leaf has more than " + std::to_string(kMaxLocalAo) +
+                      " significant basis functions");
+    L.m = m;
+    L.mp = (int)round_up(m, kAoPad);
+    for (int q = m; q < L.mp; ++q) ao_map.push_back(-1);
+    L.ao_off = ao_off;
+    ao_off += (long long)c->ncomp * L.mp * L.np;
+    const long long o = h.off[li];
+    for (int p = 0; p < L.np; ++p) {
+      const long long src = o + (p < L.n ? p : 0);
+      px.push_back(h.xyz[3 * src]);
+      py.push_back(h.xyz[3 * src + 1]);
+      pz.push_back(h.xyz[3 * src + 2]);
+      pw.push_back(p < L.n ? h.w[src] : 0.0);
+    }
+    pt_off += L.np;
+    msum += m;
+    c->flops += 4.0 * L.n * (double)m * m;
+    c->ao_write_bytes += (double)L.n * m * c->ncomp * (opt->precision == 0 ? 8.0 : 4.0) + 32.0 * L.n;
+    c->leaves.push_back(L);
+  }
+  c->nleaves = (int)c->leaves.size();
+  c->ptot = pt_off;
+  c->mean_m = c->nleaves ? msum / c->nleaves : 0.0;
+  c->ao_elems = (size_t)ao_off;
+  c->ao_elem_size = opt->precision == 0 ? 8 : 4;
+
+  // device layout
+  c->d_leaves.alloc(c->leaves.size());
+  c->d_leaves.upload(c->leaves.data(), c->leaves.size(), c->stream);
+  c->d_leaf_shells.alloc(leaf_shells.size());
+  c->d_leaf_shells.upload(leaf_shells.data(), leaf_shells.size(), c->stream);
+  c->d_ao_map.alloc(ao_map.size());
+  c->d_ao_map.upload(ao_map.data(), ao_map.size(), c->stream);
+  c->d_px.alloc(px.size());
+  c->d_px.upload(px.data(), px.size(), c->stream);
+  c->d_py.alloc(py.size());
+  c->d_py.upload(py.data(), py.size(), c->stream);
+  c->d_pz.alloc(pz.size());
+  c->d_pz.upload(pz.data(), pz.size(), c->stream);
+  c->d_pw.alloc(pw.size());
+  c->d_pw.upload(pw.data(), pw.size(), c->stream);
+  size_t free_b = 0, total_b = 0;
+  OCCXC_HIP(hipMemGetInfo(&free_b, &total_b));
+  const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
+  OCCXC_REQUIRE(ao_bytes + (size_t)6 * c->nbf * c->nbf * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
+                "resident AO values need " + std::to_string(ao_bytes >> 20) + " MiB, " +
+                    std::to_string(free_b >> 20) + " MiB free");
+  c->d_ao.alloc(ao_bytes);
+  const size_t nn = (size_t)c->nbf * c->nbf;
+  c->d_D.alloc(nn);
+  c->d_D2.alloc(nn);
+  c->d_K.alloc(nn);
+  c->d_V.alloc(nn);
+  c->d_coef.alloc((size_t)4 * std::max<long long>(c->ptot, 1));
+  c->d_part.alloc((size_t)2 * std::max(c->nleaves, 1));
+  c->d_active.alloc((size_t)std::max(c->nleaves, 1));
+  c->d_result.alloc(2);
+
+  BuildArgs &a = c->args;
+  a.nbf = c->nbf;
+  a.nleaves = c->nleaves;
+  a.deriv = c->deriv;
+  a.ncomp = c->ncomp;
+  a.nfunc = c->nfunc;
+  for (int f = 0; f < kMaxFunc; ++f) a.func_ids[f] = c->funcs[f];
+  a.dthr = opt->density_threshold;
+  a.ptot = c->ptot;
+  a.leaves = c->d_leaves.get();
+  a.shells = c->d_shells.get();
+  a.leaf_shells = c->d_leaf_shells.get();
+  a.ao_map = c->d_ao_map.get();
+  a.exps = c->d_exps.get();
+  a.coefs = c->d_coefs.get();
+  a.px = c->d_px.get();
+  a.py = c->d_py.get();
+  a.pz = c->d_pz.get();
+  a.pw = c->d_pw.get();
+  a.D2 = c->d_D2.get();
+  a.K = c->d_K.get();
+  a.coef = c->d_coef.get();
+  a.leaf_part = c->d_part.get();
+  a.leaf_active = c->d_active.get();
+
+  c->plan = make_contract_plan(c->leaves, c->deriv, opt->precision, c->stream);
+  c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
+                    c->d_ao_map.bytes() + c->d_leaves.bytes() + c->plan.bytes;
+
+  launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
+  OCCXC_HIP(hipStreamSynchronize(c->stream));
+  c->ao_ready = true;
+}
+
+void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_result,
+               hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const long long nn = (long long)c->nbf * c->nbf;
+  launch_prepare_density(d_D, c->d_D2.get(), c->nbf, s);
+  launch_zero(c->d_K.get(), nn, s);
+  if (c->opt.precision == OCCXC_PREC_FP64 && !c->plan.use_mfma) {
+    launch_density_functional(a, c->d_ao.get(), false, s);
+    launch_vxc(a, c->d_ao.get(), false, s);
+  } else {
+    run_contractions(c->plan, a, c->d_ao.get(), c->opt.precision, s);
+  }
+  launch_finalize(c->d_K.get(), d_V, c->nbf, c->d_part.get(), c->nleaves, d_result, s);
+}
+
+} // namespace
+
+extern "C" {
+
+void occxc_default_options(occxc_options *opt) {
+  if (!opt) return;
+  opt->precision = OCCXC_PREC_FP64;
+  opt->density_threshold = 1e-10;
+  opt->screening_threshold = 1e-10;
+  opt->screen = 1;
+  opt->device = 0;
+}
+
+occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *grid,
+                                  const occxc_functional *func, const occxc_options *opt,
+                                  occxc_context **ctx) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "ctx: null pointer");
+    *ctx = nullptr;
+    occxc_options o;
+    occxc_default_options(&o);
+    if (opt) o = *opt;
+    auto c = std::make_unique<occxc_context>();
+    try {
+      context_setup(c.get(), basis, grid, func, &o);
+    } catch (...) {
+      if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->stream = nullptr;
+      }
+      throw;
+    }
+    *ctx = c.release();
+  });
+}
+
+occxc_status occxc_context_destroy(occxc_context *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    {
+      DeviceGuard g(ctx->device);
+      if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+      }
+      ctx->stream = nullptr;
+      delete ctx;
+    }
+  });
+}
+
+occxc_status occxc_context_get_info(const occxc_context *ctx, occxc_context_info *info) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && info, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    info->npts = ctx->npts;
+    info->nleaves = ctx->nleaves;
+    info->nbf = ctx->nbf;
+    info->nshell = ctx->nsh;
+    info->mean_local_nbf = ctx->mean_m;
+    info->contraction_flops = ctx->flops;
+    info->ao_bytes = (double)ctx->ao_elems * ctx->ao_elem_size;
+    info->ao_write_bytes = ctx->ao_write_bytes;
+    info->device_bytes = (double)ctx->device_bytes;
+  });
+}
+
+occxc_status occxc_build_device(occxc_context *ctx, const double *d_D, double *d_Vxc,
+                                double *d_result, void *stream) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    run_build(ctx, d_D, d_Vxc, d_result, s);
+  });
+}
+
+occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc, double *exc,
+                         double *nelec) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && D && Vxc && exc && nelec, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    for (size_t i = 0; i < nn; ++i)
+      OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
+    ctx->d_D.upload(D, nn, ctx->stream);
+    run_build(ctx, ctx->d_D.get(), ctx->d_V.get(), ctx->d_result.get(), ctx->stream);
+    double res[2];
+    ctx->d_V.download(Vxc, nn, ctx->stream);
+    ctx->d_result.download(res, 2, ctx->stream);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+    *exc = res[0];
+    *nelec = res[1];
+  });
+}
+
+occxc_status occxc_synchronize(occxc_context *ctx) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+occxc_status occxc_context_evaluate_ao(occxc_context *ctx) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    launch_ao_eval(ctx->args, ctx->d_ao.get(), ctx->opt.precision != 0, ctx->stream);
+  });
+}
+
+occxc_status occxc_evaluate_basis(const occxc_basis *basis, long long npts, const double *points,
+                                  int deriv, double *out, int device) {
+  return guarded([&] {
+    OCCXC_REQUIRE(npts >= 0 && (npts == 0 || (points && out)) && (deriv == 0 || deriv == 1),
+                  OCCXC_ERR_INVALID_ARGUMENT, "evaluate_basis: bad arguments");
+    HostBasis hb = make_basis(basis);
+    DeviceGuard g(device);
+    ensure_device_tables(device);
+    DevBuf<ShellDev> ds(hb.shells.size());
+    DevBuf<double> de(hb.exps.size()), dc(hb.coefs.size()), dp(3 * std::max<long long>(npts, 1));
+    const size_t nout = (size_t)(deriv ? 4 : 1) * hb.nbf * npts;
+    DevBuf<double> dout(std::max<size_t>(nout, 1));
+    ds.upload(hb.shells.data(), hb.shells.size());
+    de.upload(hb.exps.data(), hb.exps.size());
+    dc.upload(hb.coefs.data(), hb.coefs.size());
+    dp.upload(points, 3 * npts);
+    launch_eval_basis_flat(ds.get(), hb.nsh, de.get(), dc.get(), dp.get(), npts, deriv, hb.nbf,
+                           dout.get(), nullptr);
+    dout.download(out, nout);
+    OCCXC_HIP(hipDeviceSynchronize());
+  });
+}
+
+occxc_status occxc_evaluate_decay_cutoff(const occxc_basis *basis, double threshold,
+                                         double *extents, int device) {
+  return guarded([&] {
+    OCCXC_REQUIRE(extents, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    HostBasis hb = make_basis(basis);
+    DeviceGuard g(device);
+    ensure_device_tables(device);
+    DevBuf<ShellDev> ds(hb.shells.size());
+    DevBuf<double> de(hb.exps.size()), dc(hb.coefs.size()), dx(hb.nsh);
+    ds.upload(hb.shells.data(), hb.shells.size());
+    de.upload(hb.exps.data(), hb.exps.size());
+    dc.upload(hb.coefs.data(), hb.coefs.size());
+    launch_decay_cutoff(ds.get(), hb.nsh, de.get(), dc.get(), threshold, dx.get(), nullptr);
+    dx.download(extents, hb.nsh);
+    OCCXC_HIP(hipDeviceSynchronize());
+  });
+}
+
+occxc_status occxc_functional_evaluate(int id, long long n, const double *rho,
+                                       const double *sigma, double *exc, double *vrho,
+                                       double *vsigma, int device) {
+  return guarded([&] {
+    const int fam = xc_family(id);
+    OCCXC_REQUIRE(fam >= 0, OCCXC_ERR_UNSUPPORTED, "functional id " + std::to_string(id) + " not found");
+    OCCXC_REQUIRE(n >= 0 && (n == 0 || (rho && exc && vrho)), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate: null pointer");
+    OCCXC_REQUIRE(fam == 0 || n == 0 || (sigma && vsigma), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate: GGA needs sigma and vsigma");
+    if (n == 0) return;
+    DeviceGuard g(device);
+    DevBuf<double> dr(n), ds(fam ? n : 1), de(n), dv(n), dw(fam ? n : 1);
+    dr.upload(rho, n);
+    if (fam) ds.upload(sigma, n);
+    launch_functional(id, n, dr.get(), fam ? ds.get() : nullptr, de.get(), dv.get(),
+                      fam ? dw.get() : nullptr, nullptr);
+    std::vector<double> e(n), v(n), w(fam ? n : 0);
+    de.download(e.data(), n);
+    dv.download(v.data(), n);
+    if (fam) dw.download(w.data(), n);
+    OCCXC_HIP(hipDeviceSynchronize());
+    for (long long i = 0; i < n; ++i) {
+      exc[i] += e[i];
+      vrho[i] += v[i];
+      if (fam) vsigma[i] += w[i];
+    }
+  });
+}
+
+occxc_status occxc_grid_hierarchy(long long npts, const double *points, const double *weights,
+                                  double *sorted_points, double *sorted_weights, long long *perm,
+                                  long long *leaf_offset, long long *leaf_count,
+                                  double *leaf_center, double *leaf_radius, long long *nleaves) {
+  return guarded([&] {
+    OCCXC_REQUIRE(npts >= 0 && nleaves, OCCXC_ERR_INVALID_ARGUMENT, "bad arguments");
+    Hierarchy h = build_hierarchy(npts, points, weights);
+    std::copy(h.xyz.begin(), h.xyz.end(), sorted_points);
+    std::copy(h.w.begin(), h.w.end(), sorted_weights);
+    std::copy(h.perm.begin(), h.perm.end(), perm);
+    std::copy(h.off.begin(), h.off.end(), leaf_offset);
+    std::copy(h.cnt.begin(), h.cnt.end(), leaf_count);
+    std::copy(h.center.begin(), h.center.end(), leaf_center);
+    std::copy(h.radius.begin(), h.radius.end(), leaf_radius);
+    *nleaves = (long long)h.off.size();
+  });
+}
+
+occxc_status occxc_radial_grid(int method, int n, double param, double *r, double *w,
+                               int *npoints_out) {
+  return guarded([&] {
+    OCCXC_REQUIRE(n >= 1 && r && w && npoints_out, OCCXC_ERR_INVALID_ARGUMENT, "bad arguments");
+    const double pi = 3.14159265358979323846;
+    switch (method) {
+    case OCCXC_RADIAL_EULER_MACLAURIN: {
+      // grid_utils.cpp:380-395 (returns n-1 points)
+      const int m_r = 2;
+      const double f = m_r * n * std::pow(param, 3);
+      for (int i = 1; i < n; ++i) {
+        const double nmi = n - i;
+        r[i - 1] = param * std::pow(i / nmi, m_r);
+        w[i - 1] = f * std::pow(i, 3 * m_r - 1) / std::pow(nmi, 3 * m_r + 1);
+      }
+      *npoints_out = n - 1;
+      break;
+    }
+    case OCCXC_RADIAL_TREUTLER_AHLRICHS: {
+      // grid_utils.cpp:561-574 (M4 mapping, no r^2, unscaled)
+      const double step = pi / (n + 1);
+      const double ln2 = 1.0 / std::log(2.0);
+      for (int i = 1; i <= n; ++i) {
+        const double x = std::cos(i * step);
+        const double t1 = ln2 * std::pow(1.0 + x, 0.6);
+        const double t2 = std::log((1.0 - x) / 2.0);
+        r[n - i] = -t1 * t2;
+        w[n - i] = step * std::sin(i * step) * t1 * (-0.6 / (1.0 + x) * t2 + 1.0 / (1.0 - x));
+      }
+      *npoints_out = n;
+      break;
+    }
+    case OCCXC_RADIAL_GAUSS_CHEBYSHEV:
+    case OCCXC_RADIAL_BECKE: {
+      // grid_utils.cpp:337-344 and :545-556
+      for (int i = 0; i < n; ++i) {
+        const double lin = (n == 1) ? 1.0 : 1.0 + (double)i * (2.0 * n - 2.0) / (double)(n - 1);
+        r[i] = std::cos(lin * (pi / (2.0 * n)));
+        w[i] = pi / n;
+      }
+      if (method == OCCXC_RADIAL_BECKE) {
+        for (int i = 0; i < n; ++i) {
+          const double tp1 = 1.0 + r[i], tm1 = 1.0 - r[i];
+          r[i] = (tp1 / tm1) * param;
+          w[i] *= 2.0 / (tm1 * tm1) * param;
+        }
+      }
+      *npoints_out = n;
+      break;
+    }
+    case OCCXC_RADIAL_MURA_KNOWLES: {
+      // grid_utils.cpp:517-542
+      double far = 5.2;
+      const int charge = (int)param;
+      if (charge == 3 || charge == 4 || charge == 11 || charge == 12 || charge == 19 || charge == 20)
+        far = 7.0;
+      for (int i = 0; i < n; ++i) {
+        const double x = (i + 0.5) / n, x2 = x * x, x3 = x2 * x;
+        r[i] = -far * std::log(1.0 - x3);
+        w[i] = far * 3.0 * x2 / ((1.0 - x3) * n);
+      }
+      *npoints_out = n;
+      break;
+    }
+    default:
+      throw Error(OCCXC_ERR_UNSUPPORTED, "unknown radial grid method");
+    }
+  });
+}
+
+occxc_status occxc_becke_partition(int natoms, const double *atom_xyz, const double *radii,
+                                   int ta_adjustment, long long npts, const double *points,
+                                   const int *atom_of_point, double *weights, int device) {
+  return guarded([&] {
+    OCCXC_REQUIRE(natoms >= 1 && atom_xyz && radii && npts >= 0 &&
+                      (npts == 0 || (points && atom_of_point && weights)),
+                  OCCXC_ERR_INVALID_ARGUMENT, "becke_partition: bad arguments");
+    for (long long i = 0; i < npts; ++i)
+      OCCXC_REQUIRE(atom_of_point[i] >= 0 && atom_of_point[i] < natoms, OCCXC_ERR_INVALID_ARGUMENT,
+                    "becke_partition: atom index out of range");
+    if (npts == 0) return;
+    std::vector<double> dist((size_t)natoms * natoms, 0.0);
+    for (int i = 0; i < natoms; ++i)
+      for (int j = i + 1; j < natoms; ++j) {
+        const double dx = atom_xyz[3 * i] - atom_xyz[3 * j], dy = atom_xyz[3 * i + 1] - atom_xyz[3 * j + 1],
+                     dz = atom_xyz[3 * i + 2] - atom_xyz[3 * j + 2];
+        dist[i * natoms + j] = dist[j * natoms + i] = std::sqrt(dx * dx + dy * dy + dz * dz);
+        OCCXC_REQUIRE(dist[i * natoms + j] > 0.0, OCCXC_ERR_INVALID_ARGUMENT,
+                      "becke_partition: coincident atoms");
+      }
+    DeviceGuard g(device);
+    DevBuf<double> da(3 * natoms), dr(natoms), dd(dist.size()), dp(3 * npts), dw(npts);
+    DevBuf<int> di(npts);
+    da.upload(atom_xyz, 3 * natoms);
+    dr.upload(radii, natoms);
+    dd.upload(dist.data(), dist.size());
+    dp.upload(points, 3 * npts);
+    dw.upload(weights, npts);
+    di.upload(atom_of_point, npts);
+    launch_becke(natoms, da.get(), dr.get(), dd.get(), ta_adjustment, npts, dp.get(), di.get(),
+                 dw.get(), nullptr);
+    dw.download(weights, npts);
+    OCCXC_HIP(hipDeviceSynchronize());
+  });
+}
+
+const char *occxc_last_error_message(void) { return g_last_error.c_str(); }
+
+const char *occxc_version(void) { return "occxc 0.1 gfx950"; }
+
+} // extern "C"

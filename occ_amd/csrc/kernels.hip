@@ -1,0 +1,607 @@
+// kernels.hip -- gfx950 kernels of the XC build (reference kernels).
+//
+// Stages, with the OCC function each replaces:
+//   k_ao_eval            gto::evaluate_basis_subset (src/gto/gto.cpp:212-287)
+//                        + gau2grid gg_collocation[_deriv1]
+//   k_density_functional density::evaluate_density<d,R> (src/gto/density.cpp:8-43),
+//                        set_params (dft_kernels.h:16-58),
+//                        DensityFunctional::evaluate (functional.cpp:18-65),
+//                        Result::weight_by (functional.h:460-470), the N_elec sum and
+//                        the leaf skip of process_grid_block (dft_kernels.cpp:29-40)
+//   k_vxc                xc_potential_matrix<R,d> (xc_potential_matrix.cpp:9-41) and
+//                        the K scatter of compute_K_dft (dft.h:361-369)
+//   k_finalize_*         K + K^T and the fixed-order reduction of Exc, N_elec
+//                        (dft.h:375-380)
+//   k_decay_cutoff       gto::evaluate_decay_cutoff (gto.cpp:18-67)
+//   k_becke              calculate_atomic_grid_weights + normalisation
+//                        (grid_utils.cpp:125-200, molecular_grid.cpp:70-78)
+//
+// These kernels keep every product in FP64 registers; the MFMA contraction
+// kernels (contract_mfma.hip) replace k_density_functional / k_vxc for speed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.hpp"
+#include "xc_functionals.hpp"
+
+namespace occxc {
+
+// CCA Cartesian -> spherical coefficients for l = 0..kMaxL, rows m=-l..l.
+__constant__ double c_sph[512];
+__constant__ int c_sph_off[kMaxL + 1];
+
+namespace {
+
+constexpr int kNCartMax = (kMaxL + 1) * (kMaxL + 2) / 2;
+
+// Value (and gradient) of every component of one shell at one point,
+// displacement (dx,dy,dz) from the shell centre.  Coefficients carry
+// common_fac(l).  Cartesian components in CCA order; spherical via c_sph.
+template <int L>
+__device__ __forceinline__ void shell_point(const ShellDev &sh,
+                                            const double *__restrict__ exps,
+                                            const double *__restrict__ coefs,
+                                            double dx, double dy, double dz,
+                                            bool deriv,
+                                            double (&out)[4][kNCartMax]) {
+  constexpr int NC = (L + 1) * (L + 2) / 2;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double S0 = 0.0, S1 = 0.0;
+  for (int k = 0; k < sh.nprim; ++k) {
+    const double a = exps[sh.poff + k];
+    const double c = coefs[sh.poff + k] * exp(-a * r2);
+    S0 += c;
+    S1 += -2.0 * a * c;
+  }
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+  for (int i = 1; i <= L; ++i) {
+    xp[i] = xp[i - 1] * dx;
+    yp[i] = yp[i - 1] * dy;
+    zp[i] = zp[i - 1] * dz;
+  }
+  double cv[NC], cx[NC], cy[NC], cz[NC];
+  int idx = 0;
+#pragma unroll
+  for (int a = L; a >= 0; --a) {
+#pragma unroll
+    for (int b = L - a; b >= 0; --b) {
+      const int c = L - a - b;
+      const double poly = xp[a] * yp[b] * zp[c];
+      cv[idx] = poly * S0;
+      if (deriv) {
+        const double dpx = a ? a * xp[a > 0 ? a - 1 : 0] * yp[b] * zp[c] : 0.0;
+        const double dpy = b ? b * xp[a] * yp[b > 0 ? b - 1 : 0] * zp[c] : 0.0;
+        const double dpz = c ? c * xp[a] * yp[b] * zp[c > 0 ? c - 1 : 0] : 0.0;
+        cx[idx] = dpx * S0 + poly * dx * S1;
+        cy[idx] = dpy * S0 + poly * dy * S1;
+        cz[idx] = dpz * S0 + poly * dz * S1;
+      }
+      ++idx;
+    }
+  }
+  if (sh.pure && L >= 2) {
+    const double *T = c_sph + c_sph_off[L];
+#pragma unroll
+    for (int m = 0; m < 2 * L + 1; ++m) {
+      double v = 0.0, vx = 0.0, vy = 0.0, vz = 0.0;
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        const double t = T[m * NC + k];
+        v += t * cv[k];
+        if (deriv) {
+          vx += t * cx[k];
+          vy += t * cy[k];
+          vz += t * cz[k];
+        }
+      }
+      out[0][m] = v;
+      if (deriv) {
+        out[1][m] = vx;
+        out[2][m] = vy;
+        out[3][m] = vz;
+      }
+    }
+  } else if (sh.pure && L == 1) {
+    // CCA p order is (y, z, x) (gau2grid_transform.c:44-61)
+    const int perm[3] = {1, 2, 0};
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      out[0][m] = cv[perm[m]];
+      if (deriv) {
+        out[1][m] = cx[perm[m]];
+        out[2][m] = cy[perm[m]];
+        out[3][m] = cz[perm[m]];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      out[0][k] = cv[k];
+      if (deriv) {
+        out[1][k] = cx[k];
+        out[2][k] = cy[k];
+        out[3][k] = cz[k];
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void eval_shell(const ShellDev &sh,
+                                           const double *__restrict__ exps,
+                                           const double *__restrict__ coefs,
+                                           double dx, double dy, double dz,
+                                           bool deriv,
+                                           double (&out)[4][kNCartMax]) {
+  switch (sh.l) {
+  case 0: shell_point<0>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
+  case 1: shell_point<1>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
+  case 2: shell_point<2>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
+  case 3: shell_point<3>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
+  default: shell_point<4>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
+  }
+}
+
+template <typename T> __device__ __forceinline__ double ld(const T *p) {
+  return static_cast<double>(*p);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+// Block-wide reduction for blockDim = 256 (4 wavefronts); result valid in
+// every thread.  Fixed order -> deterministic.
+__device__ double block_sum_256(double v, double *scratch) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[wid] = v;
+  __syncthreads();
+  double r = scratch[0] + scratch[1] + scratch[2] + scratch[3];
+  __syncthreads();
+  return r;
+}
+__device__ double block_max_256(double v, double *scratch) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) scratch[wid] = v;
+  __syncthreads();
+  double r = fmax(fmax(scratch[0], scratch[1]), fmax(scratch[2], scratch[3]));
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ AO --
+// One workgroup per active leaf, one thread per (padded) point; for every
+// significant shell each thread writes its components.  The AO block of a
+// leaf is [ncomp][mp][np] with points fastest, so each component row is a
+// coalesced run of np values.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao) {
+  const LeafDev L = a.leaves[blockIdx.x];
+  const int p = threadIdx.x;
+  if (p >= L.np) return;
+  const double x = a.px[L.pt_off + p], y = a.py[L.pt_off + p], z = a.pz[L.pt_off + p];
+  const bool deriv = a.deriv != 0;
+  T *base = ao + L.ao_off;
+  const long long cstride = (long long)L.mp * L.np;
+  int mu = 0;
+  for (int k = 0; k < L.nsig; ++k) {
+    const ShellDev sh = a.shells[a.leaf_shells[L.sh_off + k]];
+    double out[4][kNCartMax];
+    eval_shell(sh, a.exps, a.coefs, x - sh.x, y - sh.y, z - sh.z, deriv, out);
+    for (int c = 0; c < sh.size; ++c) {
+      base[(long long)(mu + c) * L.np + p] = static_cast<T>(out[0][c]);
+      if (deriv) {
+        base[cstride + (long long)(mu + c) * L.np + p] = static_cast<T>(out[1][c]);
+        base[2 * cstride + (long long)(mu + c) * L.np + p] = static_cast<T>(out[2][c]);
+        base[3 * cstride + (long long)(mu + c) * L.np + p] = static_cast<T>(out[3][c]);
+      }
+    }
+    mu += sh.size;
+  }
+  for (int r = L.m; r < L.mp; ++r)
+    for (int c = 0; c < a.ncomp; ++c)
+      base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
+}
+
+// ------------------------------------------------- density + functional --
+template <typename T>
+__global__ __launch_bounds__(256) void k_density_functional(BuildArgs a,
+                                                            const T *__restrict__ ao) {
+  __shared__ int s_map[kMaxLocalAo];
+  __shared__ double s_red[4];
+  const int leaf = blockIdx.x;
+  const LeafDev L = a.leaves[leaf];
+  for (int i = threadIdx.x; i < L.m; i += blockDim.x) s_map[i] = a.ao_map[L.map_off + i];
+  __syncthreads();
+  const int p = threadIdx.x;
+  const bool live = p < L.np;
+  const bool real = p < L.n;
+  const T *phi = ao + L.ao_off;
+  const long long cstride = (long long)L.mp * L.np;
+  double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+  if (live) {
+    for (int nu = 0; nu < L.m; ++nu) {
+      const double *Dcol = a.D2 + (long long)s_map[nu] * a.nbf;
+      double t = 0.0;
+      for (int mu = 0; mu < L.m; ++mu) t += Dcol[s_map[mu]] * ld(phi + (long long)mu * L.np + p);
+      rho += ld(phi + (long long)nu * L.np + p) * t;
+      if (a.deriv) {
+        gx += 2.0 * ld(phi + cstride + (long long)nu * L.np + p) * t;
+        gy += 2.0 * ld(phi + 2 * cstride + (long long)nu * L.np + p) * t;
+        gz += 2.0 * ld(phi + 3 * cstride + (long long)nu * L.np + p) * t;
+      }
+    }
+  }
+  const double w = live ? a.pw[L.pt_off + p] : 0.0;
+  const double nel = block_sum_256(real ? rho * w : 0.0, s_red);
+  const double rmax = block_max_256(real ? rho : -1e300, s_red);
+  const bool active = !(rmax < a.dthr);
+  double e_sum = 0.0, vr_sum = 0.0, vs_sum = 0.0;
+  const double sigma = gx * gx + gy * gy + gz * gz;
+  if (active && live && rho >= kDensThreshold) {
+    const double sg = fmax(sigma, kSigmaMin);
+    for (int f = 0; f < a.nfunc; ++f) {
+      double e, vr, vs;
+      xc_point(a.func_ids[f], rho, sg, e, vr, vs);
+      e_sum += e;
+      vr_sum += vr;
+      vs_sum += vs;
+    }
+  }
+  // weight_by and Exc = sum w rho eps = sum w e
+  const double exc = block_sum_256(active && real ? w * e_sum : 0.0, s_red);
+  if (live) {
+    const long long q = L.pt_off + p;
+    a.coef[q] = active ? 0.5 * w * vr_sum : 0.0;
+    a.coef[a.ptot + q] = active ? 2.0 * w * vs_sum * gx : 0.0;
+    a.coef[2 * a.ptot + q] = active ? 2.0 * w * vs_sum * gy : 0.0;
+    a.coef[3 * a.ptot + q] = active ? 2.0 * w * vs_sum * gz : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    a.leaf_part[2 * leaf] = active ? exc : 0.0;
+    a.leaf_part[2 * leaf + 1] = nel;
+    a.leaf_active[leaf] = active ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------- Vxc --
+// K_loc(mu, nu) = sum_p phi(p,mu) A(p,nu),  A = c0 phi + c . grad phi,
+// scattered into K with FP64 atomics; Vxc = K + K^T in k_finalize_matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void k_vxc(BuildArgs a, const T *__restrict__ ao) {
+  __shared__ int s_map[kMaxLocalAo];
+  __shared__ double s_c[4][kMaxLeafPts];
+  const int leaf = blockIdx.x;
+  if (!a.leaf_active[leaf]) return;
+  const LeafDev L = a.leaves[leaf];
+  for (int i = threadIdx.x; i < L.m; i += blockDim.x) s_map[i] = a.ao_map[L.map_off + i];
+  for (int i = threadIdx.x; i < L.np; i += blockDim.x)
+    for (int c = 0; c < 4; ++c) s_c[c][i] = a.coef[c * a.ptot + L.pt_off + i];
+  __syncthreads();
+  const T *phi = ao + L.ao_off;
+  const long long cstride = (long long)L.mp * L.np;
+  const long long mm = (long long)L.m * L.m;
+  for (long long idx = threadIdx.x; idx < mm; idx += blockDim.x) {
+    const int mu = (int)(idx % L.m), nu = (int)(idx / L.m);
+    const T *fm = phi + (long long)mu * L.np;
+    const T *fn = phi + (long long)nu * L.np;
+    double s = 0.0;
+    if (a.deriv) {
+      const T *fx = fn + cstride, *fy = fn + 2 * cstride, *fz = fn + 3 * cstride;
+      for (int p = 0; p < L.np; ++p)
+        s += ld(fm + p) * (s_c[0][p] * ld(fn + p) + s_c[1][p] * ld(fx + p) +
+                           s_c[2][p] * ld(fy + p) + s_c[3][p] * ld(fz + p));
+    } else {
+      for (int p = 0; p < L.np; ++p) s += ld(fm + p) * s_c[0][p] * ld(fn + p);
+    }
+    atomicAdd(a.K + s_map[mu] + (long long)s_map[nu] * a.nbf, s);
+  }
+}
+
+__global__ void k_zero(double *__restrict__ p, long long n) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x)
+    p[k] = 0.0;
+}
+
+__global__ void k_prepare_density(const double *__restrict__ D, double *__restrict__ D2, int nbf) {
+  const long long n = (long long)nbf * nbf;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(k % nbf), j = (int)(k / nbf);
+    const int r = i < j ? i : j, c = i < j ? j : i;
+    D2[k] = 2.0 * D[r + (long long)c * nbf];
+  }
+}
+
+__global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V, int nbf) {
+  const long long n = (long long)nbf * nbf;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(k % nbf), j = (int)(k / nbf);
+    V[k] = K[k] + K[j + (long long)i * nbf];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_finalize_scalars(const double *__restrict__ part,
+                                                          int nleaves,
+                                                          double *__restrict__ result) {
+  __shared__ double s_red[4];
+  double e = 0.0, n = 0.0;
+  for (int i = threadIdx.x; i < nleaves; i += 256) {
+    e += part[2 * i];
+    n += part[2 * i + 1];
+  }
+  e = block_sum_256(e, s_red);
+  n = block_sum_256(n, s_red);
+  if (threadIdx.x == 0) {
+    result[0] = e;
+    result[1] = n;
+  }
+}
+
+// ------------------------------------------------- stage entry points --
+__global__ __launch_bounds__(256) void k_eval_basis_flat(const ShellDev *__restrict__ shells,
+                                                         int nshell,
+                                                         const double *__restrict__ exps,
+                                                         const double *__restrict__ coefs,
+                                                         const double *__restrict__ xyz,
+                                                         long long npts, int deriv, int nbf,
+                                                         double *__restrict__ out) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const double x = xyz[3 * p], y = xyz[3 * p + 1], z = xyz[3 * p + 2];
+  const long long cs = (long long)nbf * npts;
+  for (int s = 0; s < nshell; ++s) {
+    const ShellDev sh = shells[s];
+    double o[4][kNCartMax];
+    eval_shell(sh, exps, coefs, x - sh.x, y - sh.y, z - sh.z, deriv != 0, o);
+    for (int c = 0; c < sh.size; ++c) {
+      const long long row = (long long)(sh.bf0 + c) * npts + p;
+      out[row] = o[0][c];
+      if (deriv) {
+        out[cs + row] = o[1][c];
+        out[2 * cs + row] = o[2][c];
+        out[3 * cs + row] = o[3][c];
+      }
+    }
+  }
+}
+
+__global__ void k_decay_cutoff(const ShellDev *__restrict__ shells, int nshell,
+                               const double *__restrict__ exps,
+                               const double *__restrict__ coefs, double thr,
+                               double *__restrict__ ext) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nshell) return;
+  const ShellDev sh = shells[s];
+  const double max_radius = 30.0, increment = 0.1, min_radius = 0.5;
+  const int npts = (int)((max_radius - min_radius) / increment) + 1;
+  const double step = (max_radius - min_radius) / (double)(npts - 1);
+  double result = 0.0;
+  for (int i = 0; i < npts; ++i) {
+    const double r = (i == npts - 1) ? max_radius : min_radius + (double)i * step;
+    const double x = 1.0 * r + sh.x, y = 0.0 + sh.y, z = 0.0 + sh.z;
+    double o[4][kNCartMax];
+    eval_shell(sh, exps, coefs, x - sh.x, y - sh.y, z - sh.z, false, o);
+    result = i * increment + min_radius;
+    double v = 0.0;
+    for (int c = 0; c < sh.size; ++c) v = fmax(v, fabs(o[0][c]));
+    if (v < thr) break;
+  }
+  ext[s] = result;
+}
+
+__global__ void k_functional(int id, long long n, const double *__restrict__ rho,
+                             const double *__restrict__ sigma, double *__restrict__ exc,
+                             double *__restrict__ vrho, double *__restrict__ vsigma) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double r = rho[i];
+  double e = 0.0, vr = 0.0, vs = 0.0;
+  if (r >= kDensThreshold) {
+    const double s = sigma ? fmax(sigma[i], kSigmaMin) : 0.0;
+    xc_point(id, r, s, e, vr, vs);
+    e /= r;
+  }
+  exc[i] = e;
+  vrho[i] = vr;
+  if (vsigma) vsigma[i] = vs;
+}
+
+// Becke partition with Treutler-Ahlrichs adjustment for one point: the cell
+// product P_k = prod_{j != k} s_kj accumulated in the pair order of
+// calculate_atomic_grid_weights_impl (grid_utils.cpp:159-200), then
+// w *= P_own / sum_k P_k (molecular_grid.cpp:77-78).
+__global__ void k_becke(int natoms, const double *__restrict__ axyz,
+                        const double *__restrict__ radii,
+                        const double *__restrict__ dist, int ta, long long npts,
+                        const double *__restrict__ pts,
+                        const int *__restrict__ atom_of_point, double *__restrict__ w) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const double x = pts[3 * p], y = pts[3 * p + 1], z = pts[3 * p + 2];
+  const int own = atom_of_point[p];
+  double total = 0.0, mine = 0.0;
+  for (int k = 0; k < natoms; ++k) {
+    const double dxk = x - axyz[3 * k], dyk = y - axyz[3 * k + 1], dzk = z - axyz[3 * k + 2];
+    const double dk = sqrt(dxk * dxk + dyk * dyk + dzk * dzk);
+    double P = 1.0;
+    for (int j = 0; j < natoms; ++j) {
+      if (j == k) continue;
+      const double dxj = x - axyz[3 * j], dyj = y - axyz[3 * j + 1], dzj = z - axyz[3 * j + 2];
+      const double dj = sqrt(dxj * dxj + dyj * dyj + dzj * dzj);
+      // pair (i, j') with i > j' as in the reference loop
+      const int i = k > j ? k : j, jj = k > j ? j : k;
+      const double di = k > j ? dk : dj, djj = k > j ? dj : dk;
+      double v = (di - djj) / dist[i * natoms + jj];
+      const double ri = radii[i], rj = radii[jj];
+      if (fabs(ri - rj) > 1e-14) {
+        const double xi = ta ? sqrt(ri / rj) : ri / rj;
+        const double u = (xi - 1.0) / (xi + 1.0);
+        const double aij = u / (u * u - 1.0);
+        v += aij * (1.0 - v * v);
+      }
+      for (int it = 0; it < 3; ++it) v = (3.0 - v * v) * v * 0.5;
+      if (k == i) {
+        P = (fabs(1.0 - v) < 1e-14) ? 0.0 : P * (0.5 * (1.0 - v));
+      } else if (!(fabs(1.0 - v) < 1e-14)) {
+        P *= 0.5 * (1.0 + v);
+      }
+    }
+    total += P;
+    if (k == own) mine = P;
+  }
+  w[p] *= mine / total;
+}
+
+} // namespace
+
+// ------------------------------------------------------------ launchers --
+static int grid_for(long long n, int block) {
+  long long g = (n + block - 1) / block;
+  if (g > 65535LL * 64) g = 65535LL * 64;
+  return (int)(g < 1 ? 1 : g);
+}
+
+static void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw std::runtime_error(std::string("kernel launch: ") + hipGetErrorString(e));
+}
+
+void upload_sph_table() {
+  // Real solid harmonics as Cartesian polynomials (Helgaker, Jorgensen,
+  // Olsen, Molecular Electronic-Structure Theory, eqs. 6.4.47-6.4.50),
+  // rows m = -l..l, columns in CCA Cartesian order.
+  double table[512] = {0};
+  int off[kMaxL + 1];
+  auto fact = [](int n) { double r = 1; for (int i = 2; i <= n; ++i) r *= i; return r; };
+  auto binom = [&](int n, int k) { return (k < 0 || k > n) ? 0.0 : fact(n) / (fact(k) * fact(n - k)); };
+  int pos = 0;
+  for (int l = 0; l <= kMaxL; ++l) {
+    off[l] = pos;
+    const int nc = (l + 1) * (l + 2) / 2;
+    for (int m = -l; m <= l; ++m) {
+      const int am = m < 0 ? -m : m;
+      const double norm = 1.0 / (std::pow(2.0, am) * fact(l)) *
+                          std::sqrt(2.0 * fact(l + am) * fact(l - am) / (m == 0 ? 2.0 : 1.0));
+      const int tv = m < 0 ? 1 : 0;  // 2 v_m
+      for (int t = 0; t <= (l - am) / 2; ++t)
+        for (int u = 0; u <= t; ++u) {
+          const int vmax2 = 2 * (int)std::floor(am / 2.0 - tv / 2.0) + tv;
+          for (int twov = tv; twov <= vmax2; twov += 2) {
+            const double sgn = ((t + (twov - tv) / 2) % 2) ? -1.0 : 1.0;
+            const double c = sgn * std::pow(0.25, t) * binom(l, t) * binom(l - t, am + t) *
+                             binom(t, u) * binom(am, twov);
+            const int px = 2 * t + am - 2 * u - twov, py = 2 * u + twov;
+            int ci = 0, idx = 0;
+            for (int i = l; i >= 0; --i)
+              for (int j = l - i; j >= 0; --j) {
+                if (i == px && j == py) ci = idx;
+                ++idx;
+              }
+            table[pos + (m + l) * nc + ci] += norm * c;
+          }
+        }
+    }
+    pos += (2 * l + 1) * nc;
+  }
+  if (hipMemcpyToSymbol(HIP_SYMBOL(c_sph), table, sizeof(double) * pos) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_sph_off), off, sizeof(off)) != hipSuccess)
+    throw std::runtime_error("upload of the spherical transform table failed");
+}
+
+void launch_zero(double *p, long long n, hipStream_t s) {
+  if (n == 0) return;
+  k_zero<<<grid_for(n, 256) < 4096 ? grid_for(n, 256) : 4096, 256, 0, s>>>(p, n);
+  check_launch();
+}
+
+void launch_prepare_density(const double *D, double *D2, int nbf, hipStream_t s) {
+  k_prepare_density<<<grid_for((long long)nbf * nbf, 256), 256, 0, s>>>(D, D2, nbf);
+  check_launch();
+}
+
+void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  if (fp32)
+    k_ao_eval<float><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
+  else
+    k_ao_eval<double><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
+  check_launch();
+}
+
+void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  if (fp32)
+    k_density_functional<float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao));
+  else
+    k_density_functional<double><<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao));
+  check_launch();
+}
+
+void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  if (fp32)
+    k_vxc<float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao));
+  else
+    k_vxc<double><<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao));
+  check_launch();
+}
+
+void launch_finalize(const double *K, double *Vxc, int nbf, const double *leaf_part,
+                     int nleaves, double *result, hipStream_t s) {
+  k_finalize_matrix<<<grid_for((long long)nbf * nbf, 256), 256, 0, s>>>(K, Vxc, nbf);
+  check_launch();
+  k_finalize_scalars<<<1, 256, 0, s>>>(leaf_part, nleaves, result);
+  check_launch();
+}
+
+void launch_eval_basis_flat(const ShellDev *shells, int nshell, const double *exps,
+                            const double *coefs, const double *xyz, long long npts, int deriv,
+                            int nbf, double *out, hipStream_t s) {
+  if (npts == 0) return;
+  k_eval_basis_flat<<<grid_for(npts, 256), 256, 0, s>>>(shells, nshell, exps, coefs, xyz, npts,
+                                                        deriv, nbf, out);
+  check_launch();
+}
+
+void launch_decay_cutoff(const ShellDev *shells, int nshell, const double *exps,
+                         const double *coefs, double threshold, double *extents,
+                         hipStream_t s) {
+  if (nshell == 0) return;
+  k_decay_cutoff<<<grid_for(nshell, 64), 64, 0, s>>>(shells, nshell, exps, coefs, threshold,
+                                                     extents);
+  check_launch();
+}
+
+void launch_functional(int id, long long n, const double *rho, const double *sigma, double *exc,
+                       double *vrho, double *vsigma, hipStream_t s) {
+  if (n == 0) return;
+  k_functional<<<grid_for(n, 256), 256, 0, s>>>(id, n, rho, sigma, exc, vrho, vsigma);
+  check_launch();
+}
+
+void launch_becke(int natoms, const double *atom_xyz, const double *radii, const double *dist,
+                  int ta, long long npts, const double *pts, const int *atom_of_point, double *w,
+                  hipStream_t s) {
+  if (npts == 0) return;
+  k_becke<<<grid_for(npts, 128), 128, 0, s>>>(natoms, atom_xyz, radii, dist, ta, npts, pts,
+                                              atom_of_point, w);
+  check_launch();
+}
+
+} // namespace occxc

@@ -1,0 +1,88 @@
+// kernels.hpp -- device-side data layout shared by the host orchestration
+// (api.cpp) and the HIP kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace occxc {
+
+constexpr int kMaxL = 4;          // highest shell angular momentum supported
+constexpr int kPtPad = 64;        // points of a leaf padded to a multiple of this
+constexpr int kAoPad = 32;        // local AOs of a leaf padded to a multiple of this
+constexpr int kMaxLeafPts = 256;  // SpatialHierarchySettings::max_leaf_size
+constexpr int kMaxFunc = 8;
+constexpr int kMaxLocalAo = 8192; // largest local AO count a leaf may have
+
+// One shell, with common_fac(l) folded into its coefficients.
+struct ShellDev {
+  double x, y, z;
+  int l, pure, nprim, poff, bf0, size;
+};
+
+// One active grid leaf (>= 1 significant shell).  Points of the leaf are
+// stored at [pt_off, pt_off + np) of the padded point arrays; points
+// [n, np) are copies of the leaf's first point with weight 0.
+struct LeafDev {
+  int pt_off;   // offset into the padded point arrays
+  int n;        // real points
+  int np;       // padded points (multiple of kPtPad)
+  int m;        // significant local AOs
+  int mp;       // padded local AOs (multiple of kAoPad)
+  int sh_off;   // offset into leaf_shells
+  int nsig;     // significant shells
+  int map_off;  // offset into ao_map (mp entries, -1 for padding)
+  long long ao_off;  // element offset of this leaf's AO block [ncomp][mp][np]
+};
+
+struct BuildArgs {
+  int nbf;
+  int nleaves;
+  int deriv;
+  int ncomp;
+  int nfunc;
+  int func_ids[kMaxFunc];
+  double dthr;
+  long long ptot;  // total padded points
+  const LeafDev *leaves;
+  const ShellDev *shells;
+  const int *leaf_shells;
+  const int *ao_map;
+  const double *exps;
+  const double *coefs;
+  const double *px, *py, *pz, *pw;
+  const double *D2;     // doubled, symmetrised density (nbf x nbf)
+  double *K;            // accumulated phi^T A (nbf x nbf)
+  double *coef;         // [4][ptot] per-point Vxc coefficients
+  double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
+  int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
+};
+
+// host-callable launchers (kernels.hip)
+void upload_sph_table();
+void launch_zero(double *p, long long n, hipStream_t s);
+void launch_prepare_density(const double *D, double *D2, int nbf, hipStream_t s);
+void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s);
+void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32,
+                               hipStream_t s);
+void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s);
+void launch_finalize(const double *K, double *Vxc, int nbf,
+                     const double *leaf_part, int nleaves, double *result,
+                     hipStream_t s);
+void launch_eval_basis_flat(const ShellDev *shells, int nshell,
+                            const double *exps, const double *coefs,
+                            const double *xyz, long long npts, int deriv,
+                            int nbf, double *out, hipStream_t s);
+void launch_decay_cutoff(const ShellDev *shells, int nshell,
+                         const double *exps, const double *coefs,
+                         double threshold, double *extents, hipStream_t s);
+void launch_functional(int id, long long n, const double *rho,
+                       const double *sigma, double *exc, double *vrho,
+                       double *vsigma, hipStream_t s);
+void launch_becke(int natoms, const double *atom_xyz, const double *radii,
+                  const double *dist, int ta, long long npts,
+                  const double *pts, const int *atom_of_point, double *w,
+                  hipStream_t s);
+
+} // namespace occxc

@@ -1,0 +1,212 @@
+"""Host-side mirror of occ::dft::DFT's XC interface over the occxc C ABI.
+
+    dft = DFT("pbe", basis, points, weights)        # DFT::DFT (src/dft/dft.cpp:19-35)
+    K = dft.compute_vxc(D)                          # DFT::compute_vxc (dft.h:395-429)
+    dft.exchange_correlation_energy()               # dft.h:83
+    dft.total_density()                             # N_elec, logged only in OCC (dft.h:384-385)
+
+Method names resolve to libxc components as get_dft_method does
+(src/dft/dft_method.cpp:161-227, share/methods/dft_methods.json).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+LIBXC_IDS = {"lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13,
+             "gga_x_pbe": 101, "gga_x_b88": 106, "gga_c_pbe": 130, "gga_c_lyp": 131}
+
+# method -> libxc components (entries of share/methods/dft_methods.json whose
+# components this build implements)
+METHODS = {"svwn5": ["lda_x", "lda_c_vwn"], "vwn5": ["lda_c_vwn"],
+           "pbe": ["gga_x_pbe", "gga_c_pbe"], "pbepbe": ["gga_x_pbe", "gga_c_pbe"],
+           "blyp": ["gga_x_b88", "gga_c_lyp"], "bpbe": ["gga_x_b88", "gga_c_pbe"],
+           "slater": ["lda_x"]}
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
+def functional_ids(method):
+    """libxc ids of a method name or of a list of libxc component names."""
+    if isinstance(method, str):
+        key = method.lower()
+        if key in METHODS:
+            comps = METHODS[key]
+        elif key.startswith("xc_") and key[3:] in LIBXC_IDS:
+            comps = [key[3:]]
+        elif key in LIBXC_IDS:
+            comps = [key]
+        else:
+            raise ValueError(f"Unknown functional name {method}")
+    else:
+        comps = list(method)
+    return [LIBXC_IDS[c] if isinstance(c, str) else int(c) for c in comps]
+
+
+def default_options():
+    o = _lib.COptions()
+    _lib.lib().occxc_default_options(ctypes.byref(o))
+    return o
+
+
+class XCContext:
+    """One occxc context: basis + grid + functional resident on one GPU."""
+
+    def __init__(self, basis, points, weights, funcs, precision="fp64", screen=True,
+                 density_threshold=1e-10, screening_threshold=1e-10, device=0):
+        L = _lib.lib()
+        self.nbf = basis.nbf
+        self._basis, self._keep = basis.c_struct()
+        self._pts = _lib.f64(points).reshape(-1)
+        self._wts = _lib.f64(weights)
+        if len(self._pts) != 3 * len(self._wts):
+            raise ValueError("points must be [npts, 3] matching weights")
+        self._grid = _lib.CGrid(len(self._wts), _lib.dptr(self._pts), _lib.dptr(self._wts))
+        self._ids = _lib.i32(funcs)
+        self._func = _lib.CFunctional(len(self._ids), _lib.iptr(self._ids))
+        o = default_options()
+        o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.screen = int(bool(screen))
+        o.density_threshold = density_threshold
+        o.screening_threshold = screening_threshold
+        o.device = device
+        self.options = o
+        h = ctypes.c_void_p()
+        _lib.check(L.occxc_context_create(ctypes.byref(self._basis), ctypes.byref(self._grid),
+                                          ctypes.byref(self._func), ctypes.byref(o),
+                                          ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        i = _lib.CContextInfo()
+        _lib.check(_lib.lib().occxc_context_get_info(self._h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def build(self, D):
+        D = np.asfortranarray(D, dtype=np.float64)
+        if D.shape != (self.nbf, self.nbf):
+            raise ValueError(f"D must be {self.nbf}x{self.nbf}")
+        V = np.zeros((self.nbf, self.nbf), order="F")
+        exc = ctypes.c_double()
+        nel = ctypes.c_double()
+        _lib.check(_lib.lib().occxc_build(self._h, D.ctypes.data_as(_lib._dp),
+                                          V.ctypes.data_as(_lib._dp), ctypes.byref(exc),
+                                          ctypes.byref(nel)))
+        return V, exc.value, nel.value
+
+    def build_device(self, d_D, d_V, d_result, stream=None):
+        _lib.check(_lib.lib().occxc_build_device(self._h, ctypes.c_void_p(d_D),
+                                                 ctypes.c_void_p(d_V),
+                                                 ctypes.c_void_p(d_result),
+                                                 ctypes.c_void_p(stream or 0)))
+
+    def evaluate_ao(self):
+        _lib.check(_lib.lib().occxc_context_evaluate_ao(self._h))
+
+    def synchronize(self):
+        _lib.check(_lib.lib().occxc_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.check(_lib.lib().occxc_context_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DFT:
+    """compute_vxc / exchange_correlation_energy for a restricted density."""
+
+    def __init__(self, method, basis, points, weights, precision="fp64", screen=True,
+                 device=0):
+        self.method = method
+        self.basis = basis
+        self.ids = functional_ids(method)
+        self.ctx = XCContext(basis, points, weights, self.ids, precision=precision,
+                             screen=screen, device=device)
+        self._exc = 0.0
+        self._nel = 0.0
+
+    def compute_vxc(self, D):
+        """Returns the matrix OCC's compute_vxc returns (Fock adds it, dft.h:435)."""
+        V, self._exc, self._nel = self.ctx.build(D)
+        return V
+
+    def exchange_correlation_energy(self):
+        return self._exc
+
+    def total_density(self):
+        return self._nel
+
+
+def evaluate_basis(basis, points, deriv=1, device=0):
+    """gto::evaluate_basis on the GPU: array [ncomp, nbf, npts]."""
+    st, keep = basis.c_struct()
+    pts = _lib.f64(points).reshape(-1)
+    n = len(pts) // 3
+    nc = 4 if deriv else 1
+    out = np.zeros((nc, basis.nbf, n))
+    _lib.check(_lib.lib().occxc_evaluate_basis(ctypes.byref(st), n, _lib.dptr(pts), int(deriv),
+                                               _lib.dptr(out), device))
+    return out
+
+
+def evaluate_decay_cutoff(basis, threshold=1e-10, device=0):
+    st, keep = basis.c_struct()
+    ext = np.zeros(basis.nsh)
+    _lib.check(_lib.lib().occxc_evaluate_decay_cutoff(ctypes.byref(st), threshold,
+                                                      _lib.dptr(ext), device))
+    return ext
+
+
+class DensityFunctional:
+    """DensityFunctional::evaluate for an unpolarised density on the GPU."""
+
+    def __init__(self, name):
+        ids = functional_ids(name)
+        if len(ids) != 1:
+            raise ValueError("DensityFunctional takes one libxc component")
+        self.id = ids[0]
+
+    def evaluate(self, rho, sigma=None, device=0):
+        rho = _lib.f64(rho)
+        n = len(rho)
+        exc, vrho = np.zeros(n), np.zeros(n)
+        gga = self.id > 100
+        sig = _lib.f64(sigma) if gga else None
+        vsig = np.zeros(n) if gga else None
+        _lib.check(_lib.lib().occxc_functional_evaluate(
+            self.id, n, _lib.dptr(rho), _lib.dptr(sig) if gga else None, _lib.dptr(exc),
+            _lib.dptr(vrho), _lib.dptr(vsig) if gga else None, device))
+        return exc, vrho, vsig
+
+
+def grid_hierarchy(points, weights):
+    pts = _lib.f64(points).reshape(-1)
+    w = _lib.f64(weights)
+    n = len(w)
+    cap = n // 32 + 2
+    sp = np.zeros(3 * n)
+    sw = np.zeros(n)
+    perm = np.zeros(n, dtype=np.int64)
+    off = np.zeros(cap, dtype=np.int64)
+    cnt = np.zeros(cap, dtype=np.int64)
+    cen = np.zeros(3 * cap)
+    rad = np.zeros(cap)
+    nl = np.zeros(1, dtype=np.int64)
+    _lib.check(_lib.lib().occxc_grid_hierarchy(n, _lib.dptr(pts), _lib.dptr(w), _lib.dptr(sp),
+                                               _lib.dptr(sw), _lib.llptr(perm), _lib.llptr(off),
+                                               _lib.llptr(cnt), _lib.dptr(cen), _lib.dptr(rad),
+                                               _lib.llptr(nl)))
+    k = int(nl[0])
+    return dict(points=sp.reshape(n, 3), weights=sw, perm=perm, offset=off[:k], count=cnt[:k],
+                center=cen[:3 * k].reshape(k, 3), radius=rad[:k])

@@ -1,0 +1,823 @@
+/*
+ * occxc_oracle.c -- CPU FP64 restatement of OCC's Kohn-Sham XC build.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Used by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py; the product
+ * library never links or calls it.
+ *
+ * What pins each part (the tests that do it are named in tests/):
+ *  - shell normalisation: unit self-overlap of every contracted spherical AO
+ *    (closed-form Gaussian integrals), tests/test_oracle_basis.py.
+ *  - collocation + CCA transform + common_fac: golden vectors written by
+ *    gau2grid (the reference's vendored C, compiled from its own sources into
+ *    oracle/_ref) and, where that library is absent, the committed fixture
+ *    tests/golden/gau2grid_collocation.npz; finite differences of phi vs grad.
+ *  - decay cutoff: brute-force restatement on the CPU in the test.
+ *  - hierarchy: the invariants of tests/dft_tests.cpp:1287-1462.
+ *  - functionals: the only reference golden for LDA/GGA (gga_x_pbe exc at
+ *    4 points, tests/dft_tests.cpp:109-111); uniform-gas limits; literature
+ *    exchange energies of the H atom (exact density); vrho/vsigma come from
+ *    complex-step differentiation of the energy density, checked against
+ *    central differences.  libxc itself (6.2.2) is absent: VWN5, PW92, PBE-c,
+ *    B88 and LYP energy expressions are pinned only by the papers' closed
+ *    forms and limits ("parity unpinned" at the libxc boundary).
+ *  - XC build: screened == dense (tests/dft_tests.cpp:904-931 restated),
+ *    N_elec = tr(2 D S) with analytic overlaps, brute-force tiny cases, and
+ *    the directional-derivative identity dExc/dD = Vxc.
+ */
+#include "occxc_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+int oracle_shell_size(int l, int pure) {
+  return pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
+}
+
+int oracle_nbf(const oracle_basis *b) {
+  int n = 0;
+  for (int s = 0; s < b->nshell; s++) n += oracle_shell_size(b->l[s], b->pure[s]);
+  return n;
+}
+
+/* ---------------------------------------------------------------- basis -- */
+
+/* shell.cpp:20-23: gint(n, a) = Gamma((n+1)/2) / (2 a^((n+1)/2)) */
+static double gint(int n, double alpha) {
+  double n1_2 = 0.5 * (n + 1);
+  return tgamma(n1_2) / (2.0 * pow(alpha, n1_2));
+}
+
+void oracle_normalize_shell(int l, int nprim, const double *exps,
+                            const double *raw, double *out) {
+  /* shell.cpp:180-186: c_p *= gto_norm(l, a_p), gto_norm = 1/sqrt(gint(2l+2, 2a)) */
+  for (int p = 0; p < nprim; p++)
+    out[p] = raw[p] / sqrt(gint(2 * l + 2, 2.0 * exps[p]));
+  /* shell.cpp:52-64: c /= sqrt(c^T E c), E_ij = gint(2l+2, a_i + a_j) */
+  double s = 0.0;
+  for (int i = 0; i < nprim; i++)
+    for (int j = 0; j < nprim; j++)
+      s += out[i] * out[j] * gint(2 * l + 2, exps[i] + exps[j]);
+  double f = 1.0 / sqrt(s);
+  for (int p = 0; p < nprim; p++) out[p] *= f;
+}
+
+double oracle_common_fac(int l, int pure) {
+  /* gto.cpp:7-16 */
+  if (l == 0) return 0.282094791773878143;
+  if (l == 1) return 0.488602511902919921;
+  return pure ? sqrt((2.0 * l + 1.0) / (4.0 * M_PI)) : 1.0;
+}
+
+static double fact(int n) {
+  double r = 1.0;
+  for (int i = 2; i <= n; i++) r *= i;
+  return r;
+}
+static double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  return fact(n) / (fact(k) * fact(n - k));
+}
+
+/* index of Cartesian (a,b,c) in CCA order: a from l down, then b from l-a down */
+static int cart_index(int l, int a, int b) {
+  int idx = 0;
+  for (int i = l; i >= 0; i--)
+    for (int j = l - i; j >= 0; j--) {
+      if (i == a && j == b) return idx;
+      idx++;
+    }
+  return -1;
+}
+
+/* Real solid harmonics S_lm as Cartesian polynomials (Helgaker, Jorgensen,
+ * Olsen, "Molecular Electronic-Structure Theory", eqs. 6.4.47-6.4.50), rows in
+ * CCA order m = -l..l, which is the order of gau2grid_transform.c. */
+void oracle_cart_to_sph(int l, double *out) {
+  int ncart = (l + 1) * (l + 2) / 2;
+  memset(out, 0, sizeof(double) * (2 * l + 1) * ncart);
+  for (int m = -l; m <= l; m++) {
+    int am = abs(m);
+    double norm = 1.0 / (pow(2.0, am) * fact(l)) *
+                  sqrt(2.0 * fact(l + am) * fact(l - am) / (m == 0 ? 2.0 : 1.0));
+    /* v runs over v_m, v_m+1, ... with v_m = 0 (m>=0) or 1/2 (m<0); use 2v */
+    int twov_m = (m < 0) ? 1 : 0;
+    for (int t = 0; t <= (l - am) / 2; t++)
+      for (int u = 0; u <= t; u++) {
+        /* v from v_m to floor(|m|/2 - v_m) + v_m, in steps of 1 */
+        int vmax2 = 2 * (int)floor(am / 2.0 - twov_m / 2.0) + twov_m;
+        for (int twov = twov_m; twov <= vmax2; twov += 2) {
+          double sgn = ((t + (twov - twov_m) / 2) % 2) ? -1.0 : 1.0;
+          double c = sgn * pow(0.25, t) * binom(l, t) * binom(l - t, am + t) *
+                     binom(t, u) * binom(am, twov);
+          int px = 2 * t + am - 2 * u - twov;
+          int py = 2 * u + twov;
+          int pz = l - 2 * t - am;
+          int ci = cart_index(l, px, py);
+          (void)pz;
+          out[(m + l) * ncart + ci] += norm * c;
+        }
+      }
+  }
+}
+
+static double ipow(double x, int n) {
+  double r = 1.0;
+  for (int i = 0; i < n; i++) r *= x;
+  return r;
+}
+
+#define ORACLE_MAX_L 6
+
+void oracle_eval_shell(int l, int pure, int nprim, const double *exps,
+                       const double *coefs, const double *center, long npts,
+                       const double *xyz, int deriv, double *phi, double *phx,
+                       double *phy, double *phz, long ld) {
+  int ncart = (l + 1) * (l + 2) / 2;
+  int nout = oracle_shell_size(l, pure);
+  double T[(2 * ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2];
+  if (pure) oracle_cart_to_sph(l, T);
+  double fac = oracle_common_fac(l, pure);
+  double cv[(ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2];
+  double cx[(ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2];
+  double cy[(ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2];
+  double cz[(ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2];
+  for (long p = 0; p < npts; p++) {
+    /* gau2grid: xc = x - center_x etc., R2, S0 = sum c e^{-a R2},
+     * S1 = sum -2 a c e^{-a R2} (gau2grid_deriv1.c:491-505) */
+    double x = xyz[3 * p] - center[0];
+    double y = xyz[3 * p + 1] - center[1];
+    double z = xyz[3 * p + 2] - center[2];
+    double r2 = x * x + y * y + z * z;
+    double S0 = 0.0, S1 = 0.0;
+    for (int k = 0; k < nprim; k++) {
+      double e = exp(-exps[k] * r2);
+      S0 += coefs[k] * e;
+      S1 += -2.0 * exps[k] * coefs[k] * e;
+    }
+    int idx = 0;
+    for (int a = l; a >= 0; a--)
+      for (int b = l - a; b >= 0; b--) {
+        int c = l - a - b;
+        double poly = ipow(x, a) * ipow(y, b) * ipow(z, c);
+        cv[idx] = poly * S0;
+        if (deriv) {
+          double dpx = a ? a * ipow(x, a - 1) * ipow(y, b) * ipow(z, c) : 0.0;
+          double dpy = b ? b * ipow(x, a) * ipow(y, b - 1) * ipow(z, c) : 0.0;
+          double dpz = c ? c * ipow(x, a) * ipow(y, b) * ipow(z, c - 1) : 0.0;
+          cx[idx] = dpx * S0 + poly * x * S1;
+          cy[idx] = dpy * S0 + poly * y * S1;
+          cz[idx] = dpz * S0 + poly * z * S1;
+        }
+        idx++;
+      }
+    for (int o = 0; o < nout; o++) {
+      double v = 0, vx = 0, vy = 0, vz = 0;
+      if (pure) {
+        for (int k = 0; k < ncart; k++) {
+          double t = T[o * ncart + k];
+          if (t == 0.0) continue;
+          v += t * cv[k];
+          if (deriv) {
+            vx += t * cx[k];
+            vy += t * cy[k];
+            vz += t * cz[k];
+          }
+        }
+      } else {
+        v = cv[o];
+        if (deriv) {
+          vx = cx[o];
+          vy = cy[o];
+          vz = cz[o];
+        }
+      }
+      phi[o * ld + p] = fac * v;
+      if (deriv) {
+        phx[o * ld + p] = fac * vx;
+        phy[o * ld + p] = fac * vy;
+        phz[o * ld + p] = fac * vz;
+      }
+    }
+  }
+}
+
+void oracle_decay_cutoff(const oracle_basis *b, double threshold,
+                         double *extents) {
+  /* gto.cpp:18-67: r = 0.5 .. 30 Bohr in 0.1 steps along +x from the shell
+   * origin; the extent is the first r where max_c |fac*phi_c| < threshold. */
+  const double max_radius = 30.0, increment = 0.1, min_radius = 0.5;
+  const int npts = (int)((max_radius - min_radius) / increment) + 1;
+  /* Eigen::VectorXd::LinSpaced(npts, lo, hi): lo + i*step, last point = hi */
+  double step = (max_radius - min_radius) / (double)(npts - 1);
+  double *xyz = malloc(sizeof(double) * 3 * npts);
+  double *vals = malloc(sizeof(double) * npts * 64);
+  for (int s = 0; s < b->nshell; s++) {
+    const double *o = b->origin + 3 * s;
+    for (int i = 0; i < npts; i++) {
+      double r = (i == npts - 1) ? max_radius : min_radius + (double)i * step;
+      xyz[3 * i] = 1.0 * r + o[0];
+      xyz[3 * i + 1] = 0.0 + o[1];
+      xyz[3 * i + 2] = 0.0 + o[2];
+    }
+    int l = b->l[s], pure = b->pure[s];
+    int nc = oracle_shell_size(l, pure);
+    oracle_eval_shell(l, pure, b->nprim[s], b->exps + b->prim_offset[s],
+                      b->coefs + b->prim_offset[s], o, npts, xyz, 0, vals, 0, 0,
+                      0, npts);
+    double result = 0.0;
+    for (int i = 0; i < npts; i++) {
+      result = i * increment + min_radius;
+      double v = 0.0;
+      for (int c = 0; c < nc; c++) {
+        double a = fabs(vals[c * npts + i]);
+        if (a > v) v = a;
+      }
+      if (v < threshold) break;
+    }
+    extents[s] = result;
+  }
+  free(xyz);
+  free(vals);
+}
+
+/* ------------------------------------------------------------ hierarchy -- */
+
+static uint64_t dilate_3d(uint64_t x) {
+  /* spread the low 21 bits of x to every third bit */
+  uint64_t r = 0;
+  for (int i = 0; i < 21; i++) r |= ((x >> i) & 1ULL) << (3 * i);
+  return r;
+}
+
+typedef struct {
+  uint64_t code;
+  long idx;
+} morton_pair;
+
+static int cmp_morton(const void *a, const void *b) {
+  const morton_pair *x = a, *y = b;
+  if (x->code < y->code) return -1;
+  if (x->code > y->code) return 1;
+  return (x->idx < y->idx) ? -1 : (x->idx > y->idx);
+}
+
+long oracle_build_hierarchy(long n, const double *xyz, const double *w,
+                            double *sxyz, double *sw, long *perm,
+                            long *leaf_offset, long *leaf_count,
+                            double *leaf_center, double *leaf_radius) {
+  if (n <= 0) return 0;
+  /* spatial_grid_hierarchy.cpp:43-98 */
+  double mn[3], mx[3], ext[3];
+  for (int k = 0; k < 3; k++) mn[k] = mx[k] = xyz[k];
+  for (long i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      double v = xyz[3 * i + k];
+      if (v < mn[k]) mn[k] = v;
+      if (v > mx[k]) mx[k] = v;
+    }
+  for (int k = 0; k < 3; k++) {
+    ext[k] = mx[k] - mn[k];
+    if (ext[k] < 1e-10) ext[k] = 1.0;
+  }
+  morton_pair *mp = malloc(sizeof(morton_pair) * n);
+  const uint64_t scale = (1ULL << 21) - 1;
+  for (long i = 0; i < n; i++) {
+    uint64_t q[3];
+    for (int k = 0; k < 3; k++) {
+      double t = (xyz[3 * i + k] - mn[k]) / ext[k];
+      if (t < 0.0) t = 0.0;
+      if (t > 1.0) t = 1.0;
+      q[k] = (uint64_t)(t * (double)scale);
+    }
+    mp[i].code = dilate_3d(q[0]) | (dilate_3d(q[1]) << 1) | (dilate_3d(q[2]) << 2);
+    mp[i].idx = i;
+  }
+  qsort(mp, n, sizeof(morton_pair), cmp_morton);
+  for (long i = 0; i < n; i++) {
+    long o = mp[i].idx;
+    perm[i] = o;
+    for (int k = 0; k < 3; k++) sxyz[3 * i + k] = xyz[3 * o + k];
+    sw[i] = w[o];
+  }
+  free(mp);
+  /* spatial_grid_hierarchy.cpp:100-147; settings .h:26-30 */
+  const long target = 128, minl = 32, maxl = 256;
+  long nl = 0, off = 0;
+  while (off < n) {
+    long rem = n - off, cnt;
+    if (rem <= maxl) cnt = rem;
+    else if (rem < 2 * minl) cnt = rem / 2;
+    else cnt = (target < rem) ? target : rem;
+    double c[3] = {0, 0, 0};
+    for (long i = 0; i < cnt; i++)
+      for (int k = 0; k < 3; k++) c[k] += sxyz[3 * (off + i) + k];
+    for (int k = 0; k < 3; k++) c[k] /= (double)cnt;
+    double r = 0.0;
+    for (long i = 0; i < cnt; i++) {
+      double dx = sxyz[3 * (off + i)] - c[0], dy = sxyz[3 * (off + i) + 1] - c[1],
+             dz = sxyz[3 * (off + i) + 2] - c[2];
+      double d = sqrt(dx * dx + dy * dy + dz * dz);
+      if (d > r) r = d;
+    }
+    leaf_offset[nl] = off;
+    leaf_count[nl] = cnt;
+    for (int k = 0; k < 3; k++) leaf_center[3 * nl + k] = c[k];
+    leaf_radius[nl] = r;
+    nl++;
+    off += cnt;
+  }
+  return nl;
+}
+
+/* ---------------------------------------------------------- functionals -- */
+/* Energy densities e(rho, sigma) = rho * eps written in complex arithmetic so
+ * that vrho = de/drho and vsigma = de/dsigma follow by complex-step
+ * differentiation (Squire & Trapp, SIAM Rev. 40, 110 (1998)): exact to
+ * rounding, and independent of any hand-derived derivative algebra. */
+
+typedef double complex cplx;
+
+/* threshold below which a point contributes nothing (libxc's own
+ * per-functional thresholds are not visible from OCC; this is our choice) */
+#define ORACLE_DENS_THR 1e-15
+#define ORACLE_SIGMA_MIN 1e-40
+
+static const double CX = 0.73855876638202240587; /* (3/4)(3/pi)^(1/3) */
+
+static cplx c_cbrt(cplx z) { return cpow(z, 1.0 / 3.0); }
+static cplx c_rs(cplx rho) { return c_cbrt(3.0 / (4.0 * M_PI * rho)); }
+
+/* Dirac/Slater exchange */
+static cplx e_lda_x(cplx rho) { return -CX * rho * c_cbrt(rho); }
+
+/* VWN functional V interpolation, S.H. Vosko, L. Wilk, M. Nusair,
+ * Can. J. Phys. 58, 1200 (1980), eq. 4.4, paramagnetic MC parameters */
+static cplx eps_vwn(cplx rs, double A, double b, double c, double x0) {
+  cplx x = csqrt(rs);
+  cplx X = x * x + b * x + c;
+  double X0 = x0 * x0 + b * x0 + c;
+  double Q = sqrt(4.0 * c - b * b);
+  cplx at = catan(Q / (2.0 * x + b));
+  return A * (clog(x * x / X) + 2.0 * b / Q * at -
+              b * x0 / X0 * (clog((x - x0) * (x - x0) / X) + 2.0 * (b + 2.0 * x0) / Q * at));
+}
+
+/* J.P. Perdew, Y. Wang, PRB 45, 13244 (1992), eq. 10, paramagnetic */
+static cplx eps_pw92(cplx rs, double A) {
+  const double a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
+  cplx srs = csqrt(rs);
+  cplx den = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * clog(1.0 + 1.0 / den);
+}
+
+/* PBE: J.P. Perdew, K. Burke, M. Ernzerhof, PRL 77, 3865 (1996) */
+static const double PBE_BETA = 0.06672455060314922;
+static const double PBE_KAPPA = 0.804;
+
+static cplx e_pbe_x(cplx rho, cplx sigma) {
+  double mu = PBE_BETA * M_PI * M_PI / 3.0;
+  cplx kf = c_cbrt(3.0 * M_PI * M_PI * rho);
+  cplx s2 = sigma / (4.0 * kf * kf * rho * rho);
+  cplx F = 1.0 + PBE_KAPPA - PBE_KAPPA / (1.0 + mu * s2 / PBE_KAPPA);
+  return e_lda_x(rho) * F;
+}
+
+static cplx e_pbe_c(cplx rho, cplx sigma) {
+  double gamma = (1.0 - log(2.0)) / (M_PI * M_PI);
+  cplx rs = c_rs(rho);
+  cplx ec = eps_pw92(rs, 0.0310907);
+  cplx kf = c_cbrt(3.0 * M_PI * M_PI * rho);
+  cplx ks = csqrt(4.0 * kf / M_PI);
+  cplx t2 = sigma / (4.0 * ks * ks * rho * rho);
+  cplx A = (PBE_BETA / gamma) / (cexp(-ec / gamma) - 1.0);
+  cplx At2 = A * t2;
+  cplx H = gamma * clog(1.0 + (PBE_BETA / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+
+/* A.D. Becke, PRA 38, 3098 (1988), eq. 8, written per spin channel with
+ * rho_s = rho/2 and |grad rho_s|^2 = sigma/4 */
+static cplx e_b88(cplx rho, cplx sigma) {
+  const double beta = 0.0042;
+  cplx rs_ = 0.5 * rho;
+  cplx r43 = rs_ * c_cbrt(rs_);
+  cplx x = csqrt(0.25 * sigma) / r43;
+  cplx g = beta * r43 * x * x / (1.0 + 6.0 * beta * x * casinh(x));
+  return e_lda_x(rho) - 2.0 * g;
+}
+
+/* C. Lee, W. Yang, R.G. Parr, PRB 37, 785 (1988) in the form of
+ * B. Miehlich, A. Savin, H. Stoll, H. Preuss, CPL 157, 200 (1989), eq. 2,
+ * evaluated at rho_a = rho_b = rho/2, grad rho_a = grad rho_b = grad rho / 2 */
+static cplx e_lyp(cplx rho, cplx sigma) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double CF = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  cplx ra = 0.5 * rho, rb = 0.5 * rho;
+  cplx gaa = 0.25 * sigma, gbb = 0.25 * sigma, gg = sigma;
+  cplx rm13 = 1.0 / c_cbrt(rho);
+  cplx den = 1.0 + d * rm13;
+  cplx omega = cexp(-c * rm13) / den * cpow(rho, -11.0 / 3.0);
+  cplx delta = c * rm13 + d * rm13 / den;
+  cplx t1 = -a * 4.0 / den * ra * rb / rho;
+  cplx br = ra * rb * (cpow(2.0, 11.0 / 3.0) * CF * (cpow(ra, 8.0 / 3.0) + cpow(rb, 8.0 / 3.0)) +
+                       (47.0 / 18.0 - 7.0 / 18.0 * delta) * gg -
+                       (2.5 - delta / 18.0) * (gaa + gbb) -
+                       (delta - 11.0) / 9.0 * (ra / rho * gaa + rb / rho * gbb));
+  br += -2.0 / 3.0 * rho * rho * gg + (2.0 / 3.0 * rho * rho - ra * ra) * gbb +
+        (2.0 / 3.0 * rho * rho - rb * rb) * gaa;
+  return t1 - a * b * omega * br;
+}
+
+int oracle_functional_family(int id) {
+  switch (id) {
+  case ORACLE_LDA_X:
+  case ORACLE_LDA_C_VWN:
+  case ORACLE_LDA_C_PW:
+  case ORACLE_LDA_C_PW_MOD:
+    return 0;
+  case ORACLE_GGA_X_PBE:
+  case ORACLE_GGA_X_B88:
+  case ORACLE_GGA_C_PBE:
+  case ORACLE_GGA_C_LYP:
+    return 1;
+  default:
+    return -1;
+  }
+}
+
+static cplx edens(int id, cplx rho, cplx sigma) {
+  switch (id) {
+  case ORACLE_LDA_X:
+    return e_lda_x(rho);
+  case ORACLE_LDA_C_VWN:
+    return rho * eps_vwn(c_rs(rho), 0.0310907, 3.72744, 12.9352, -0.10498);
+  case ORACLE_LDA_C_PW:
+    return rho * eps_pw92(c_rs(rho), 0.031091);
+  case ORACLE_LDA_C_PW_MOD:
+    return rho * eps_pw92(c_rs(rho), 0.0310907);
+  case ORACLE_GGA_X_PBE:
+    return e_pbe_x(rho, sigma);
+  case ORACLE_GGA_X_B88:
+    return e_b88(rho, sigma);
+  case ORACLE_GGA_C_PBE:
+    return e_pbe_c(rho, sigma);
+  case ORACLE_GGA_C_LYP:
+    return e_lyp(rho, sigma);
+  }
+  return 0.0;
+}
+
+double oracle_functional_energy_density(int id, double rho, double sigma) {
+  if (rho < ORACLE_DENS_THR) return 0.0;
+  if (sigma < ORACLE_SIGMA_MIN) sigma = ORACLE_SIGMA_MIN;
+  return creal(edens(id, rho, sigma));
+}
+
+int oracle_functional_eval(int id, long n, const double *rho,
+                           const double *sigma, double *exc, double *vrho,
+                           double *vsigma) {
+  int fam = oracle_functional_family(id);
+  if (fam < 0) return -1;
+  for (long i = 0; i < n; i++) {
+    double r = rho[i];
+    if (!(r >= ORACLE_DENS_THR)) continue;
+    double s = 0.0;
+    if (fam == 1) {
+      s = sigma[i];
+      if (s < ORACLE_SIGMA_MIN) s = ORACLE_SIGMA_MIN;
+    }
+    double hr = 1e-20 * r;
+    cplx v = edens(id, r + I * hr, s);
+    exc[i] += creal(v) / r;
+    vrho[i] += cimag(v) / hr;
+    if (fam == 1) {
+      double hs = 1e-20 * s;
+      cplx u = edens(id, r, s + I * hs);
+      vsigma[i] += cimag(u) / hs;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------- XC build -- */
+
+typedef struct {
+  int deriv;
+  int nfunc;
+  const int *ids;
+  double dthr;
+} block_cfg;
+
+/* kernels::process_grid_block<d, Restricted> (dft_kernels.cpp:8-70) with
+ * density::evaluate_density (density.cpp:8-43), set_params (dft_kernels.h:
+ * 16-58), Result::weight_by (functional.h:460-470) and xc_potential_matrix
+ * (xc_potential_matrix.cpp:9-41).  phi is component-major [c][m][n]. */
+static void process_block(const block_cfg *cfg, int m, long n,
+                          const double *phi, const double *Dloc,
+                          const double *wts, double *Kblk, double *energy,
+                          double *nel, double *work) {
+  const double *p0 = phi;
+  const double *px = phi + (long)m * n;
+  const double *py = phi + 2L * m * n;
+  const double *pz = phi + 3L * m * n;
+  double *T = work;              /* [m][n] */
+  double *rho = T + (long)m * n; /* [4][n] */
+  double *exc = rho + 4 * n, *vrho = exc + n, *vsig = vrho + n, *sig = vsig + n;
+  double *A = sig + n; /* [m][n] */
+
+  /* T = phi * D  (Dphi in density.cpp:14) */
+  memset(T, 0, sizeof(double) * m * n);
+  for (int nu = 0; nu < m; nu++) {
+    double *t = T + (long)nu * n;
+    for (int mu = 0; mu < m; mu++) {
+      double dm = Dloc[mu + (long)nu * m];
+      const double *f = p0 + (long)mu * n;
+      for (long p = 0; p < n; p++) t[p] += dm * f[p];
+    }
+  }
+  memset(rho, 0, sizeof(double) * 4 * n);
+  for (int nu = 0; nu < m; nu++) {
+    const double *t = T + (long)nu * n;
+    const double *f = p0 + (long)nu * n;
+    for (long p = 0; p < n; p++) rho[p] += f[p] * t[p];
+    if (cfg->deriv) {
+      const double *fx = px + (long)nu * n, *fy = py + (long)nu * n,
+                   *fz = pz + (long)nu * n;
+      for (long p = 0; p < n; p++) {
+        rho[n + p] += 2.0 * fx[p] * t[p];
+        rho[2 * n + p] += 2.0 * fy[p] * t[p];
+        rho[3 * n + p] += 2.0 * fz[p] * t[p];
+      }
+    }
+  }
+  /* dft_kernels.cpp:29: N_elec before the skip */
+  double ne = 0.0, rmax = -1e300;
+  for (long p = 0; p < n; p++) {
+    ne += rho[p] * wts[p];
+    if (rho[p] > rmax) rmax = rho[p];
+  }
+  *nel += ne;
+  if (rmax < cfg->dthr) return; /* dft_kernels.cpp:38-40 */
+
+  for (long p = 0; p < n; p++) {
+    exc[p] = vrho[p] = vsig[p] = 0.0;
+    sig[p] = rho[n + p] * rho[n + p] + rho[2 * n + p] * rho[2 * n + p] +
+             rho[3 * n + p] * rho[3 * n + p];
+  }
+  for (int f = 0; f < cfg->nfunc; f++)
+    oracle_functional_eval(cfg->ids[f], n, rho, sig, exc, vrho, vsig);
+  for (long p = 0; p < n; p++) {
+    exc[p] *= wts[p];
+    vrho[p] *= wts[p];
+    vsig[p] *= wts[p];
+  }
+  double e = 0.0;
+  for (long p = 0; p < n; p++) e += rho[p] * exc[p];
+  *energy += e;
+
+  if (!cfg->deriv) {
+    /* Vxc = phi^T (vrho o phi) */
+    for (int nu = 0; nu < m; nu++) {
+      const double *f = p0 + (long)nu * n;
+      double *a = A + (long)nu * n;
+      for (long p = 0; p < n; p++) a[p] = vrho[p] * f[p];
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *a = A + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double *f = p0 + (long)mu * n;
+        double s = 0.0;
+        for (long p = 0; p < n; p++) s += f[p] * a[p];
+        Kblk[mu + (long)nu * m] += s;
+      }
+    }
+  } else {
+    /* A = 0.5 vrho phi + 2 vsigma (grad rho . grad phi);  K += A'phi + phi'A */
+    for (int nu = 0; nu < m; nu++) {
+      const double *f = p0 + (long)nu * n, *fx = px + (long)nu * n,
+                   *fy = py + (long)nu * n, *fz = pz + (long)nu * n;
+      double *a = A + (long)nu * n;
+      for (long p = 0; p < n; p++)
+        a[p] = 0.5 * vrho[p] * f[p] +
+               2.0 * vsig[p] *
+                   (rho[n + p] * fx[p] + rho[2 * n + p] * fy[p] + rho[3 * n + p] * fz[p]);
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *a = A + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double *f = p0 + (long)mu * n;
+        double s = 0.0;
+        for (long p = 0; p < n; p++) s += f[p] * a[p];
+        Kblk[mu + (long)nu * m] += s;
+        Kblk[nu + (long)mu * m] += s;
+      }
+    }
+  }
+}
+
+typedef struct {
+  const oracle_basis *b;
+  int *first_bf;
+  int nbf;
+} basis_info;
+
+static void eval_shells(const oracle_basis *b, const int *sig, int nsig,
+                        long n, const double *xyz, int deriv, double *phi,
+                        int m) {
+  int off = 0;
+  for (int k = 0; k < nsig; k++) {
+    int s = sig[k];
+    int l = b->l[s], pure = b->pure[s];
+    double *p0 = phi + (long)off * n;
+    double *px = deriv ? phi + ((long)m + off) * n : 0;
+    double *py = deriv ? phi + (2L * m + off) * n : 0;
+    double *pz = deriv ? phi + (3L * m + off) * n : 0;
+    oracle_eval_shell(l, pure, b->nprim[s], b->exps + b->prim_offset[s],
+                      b->coefs + b->prim_offset[s], b->origin + 3 * s, n, xyz,
+                      deriv, p0, px, py, pz, n);
+    off += oracle_shell_size(l, pure);
+  }
+}
+
+int oracle_xc_build(const oracle_basis *b, long npts, const double *xyz,
+                    const double *w, int nfunc, const int *func_ids,
+                    const double *D, double density_threshold,
+                    double screening_threshold, int screened, int nthreads,
+                    double *K, double *exc_out, double *nelec_out) {
+  if (!b || npts < 0 || nfunc <= 0) return -1;
+  int deriv = 0;
+  for (int f = 0; f < nfunc; f++) {
+    int fam = oracle_functional_family(func_ids[f]);
+    if (fam < 0) return -1;
+    if (fam > deriv) deriv = fam;
+  }
+  const int nbf = oracle_nbf(b);
+  const int nsh = b->nshell;
+  int *first_bf = malloc(sizeof(int) * (nsh + 1));
+  first_bf[0] = 0;
+  for (int s = 0; s < nsh; s++)
+    first_bf[s + 1] = first_bf[s] + oracle_shell_size(b->l[s], b->pure[s]);
+  /* D2 = 2 D (dft.h:205), read through selfadjointView<Upper> */
+  double *D2 = malloc(sizeof(double) * nbf * nbf);
+  for (int j = 0; j < nbf; j++)
+    for (int i = 0; i < nbf; i++) {
+      int r = i < j ? i : j, c = i < j ? j : i;
+      D2[i + (long)j * nbf] = 2.0 * D[r + (long)c * nbf];
+    }
+  memset(K, 0, sizeof(double) * nbf * nbf);
+  block_cfg cfg = {deriv, nfunc, func_ids, density_threshold};
+
+  /* work decomposition: list of (offset, count) blocks over a point array */
+  const double *pxyz = xyz, *pw = w;
+  double *sxyz = 0, *sw = 0, *lcen = 0, *lrad = 0, *ext = 0;
+  long *perm = 0, *loff = 0, *lcnt = 0;
+  long nblk;
+  if (screened) {
+    sxyz = malloc(sizeof(double) * 3 * (npts + 1));
+    sw = malloc(sizeof(double) * (npts + 1));
+    perm = malloc(sizeof(long) * (npts + 1));
+    long cap = npts / 32 + 2;
+    loff = malloc(sizeof(long) * cap);
+    lcnt = malloc(sizeof(long) * cap);
+    lcen = malloc(sizeof(double) * 3 * cap);
+    lrad = malloc(sizeof(double) * cap);
+    nblk = oracle_build_hierarchy(npts, xyz, w, sxyz, sw, perm, loff, lcnt, lcen, lrad);
+    pxyz = sxyz;
+    pw = sw;
+    ext = malloc(sizeof(double) * nsh);
+    oracle_decay_cutoff(b, screening_threshold, ext);
+  } else {
+    nblk = (npts + 127) / 128;
+    loff = malloc(sizeof(long) * (nblk + 1));
+    lcnt = malloc(sizeof(long) * (nblk + 1));
+    for (long i = 0; i < nblk; i++) {
+      loff[i] = 128 * i;
+      lcnt[i] = (npts - 128 * i < 128) ? npts - 128 * i : 128;
+    }
+  }
+
+  double energy = 0.0, nel = 0.0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(+ : energy, nel)
+  {
+    double *Kloc = calloc((size_t)nbf * nbf, sizeof(double));
+    int *sig = malloc(sizeof(int) * (nsh + 1));
+    int *gidx = malloc(sizeof(int) * (nbf + 1));
+    long maxn = 256;
+    double *phi = malloc(sizeof(double) * 4 * (size_t)nbf * maxn);
+    double *Dloc = malloc(sizeof(double) * (size_t)nbf * nbf);
+    double *Kblk = malloc(sizeof(double) * (size_t)nbf * nbf);
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 9 * maxn));
+#pragma omp for schedule(dynamic, 4)
+    for (long li = 0; li < nblk; li++) {
+      long n = lcnt[li];
+      if (n == 0) continue;
+      int nsig = 0;
+      if (screened) {
+        /* dft.h:314-320 */
+        for (int s = 0; s < nsh; s++) {
+          const double *o = b->origin + 3 * s;
+          double dx = o[0] - lcen[3 * li], dy = o[1] - lcen[3 * li + 1],
+                 dz = o[2] - lcen[3 * li + 2];
+          double dist = sqrt(dx * dx + dy * dy + dz * dz);
+          if (dist - ext[s] < lrad[li]) sig[nsig++] = s;
+        }
+        if (nsig == 0) continue;
+      } else {
+        for (int s = 0; s < nsh; s++) sig[nsig++] = s;
+      }
+      int m = 0;
+      for (int k = 0; k < nsig; k++) {
+        int s = sig[k];
+        for (int q = first_bf[s]; q < first_bf[s + 1]; q++) gidx[m++] = q;
+      }
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++)
+          Dloc[i + (long)j * m] = D2[gidx[i] + (long)gidx[j] * nbf];
+      eval_shells(b, sig, nsig, n, pxyz + 3 * loff[li], deriv, phi, m);
+      memset(Kblk, 0, sizeof(double) * m * m);
+      process_block(&cfg, m, n, phi, Dloc, pw + loff[li], Kblk, &energy, &nel, work);
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++)
+          Kloc[gidx[i] + (long)gidx[j] * nbf] += Kblk[i + (long)j * m];
+    }
+#pragma omp critical
+    {
+      for (long i = 0; i < (long)nbf * nbf; i++) K[i] += Kloc[i];
+    }
+    free(Kloc);
+    free(sig);
+    free(gidx);
+    free(phi);
+    free(Dloc);
+    free(Kblk);
+    free(work);
+  }
+  *exc_out = energy;
+  *nelec_out = nel;
+  free(first_bf);
+  free(D2);
+  free(sxyz);
+  free(sw);
+  free(perm);
+  free(loff);
+  free(lcnt);
+  free(lcen);
+  free(lrad);
+  free(ext);
+  return 0;
+}
+
+long oracle_screened_work(const oracle_basis *b, long npts, const double *xyz,
+                          const double *w, double screening_threshold,
+                          long maxleaves, long *leaf_n, long *leaf_m) {
+  const int nsh = b->nshell;
+  double *sxyz = malloc(sizeof(double) * 3 * (npts + 1));
+  double *sw = malloc(sizeof(double) * (npts + 1));
+  long *perm = malloc(sizeof(long) * (npts + 1));
+  long cap = npts / 32 + 2;
+  long *loff = malloc(sizeof(long) * cap), *lcnt = malloc(sizeof(long) * cap);
+  double *lcen = malloc(sizeof(double) * 3 * cap), *lrad = malloc(sizeof(double) * cap);
+  long nl = oracle_build_hierarchy(npts, xyz, w, sxyz, sw, perm, loff, lcnt, lcen, lrad);
+  double *ext = malloc(sizeof(double) * nsh);
+  oracle_decay_cutoff(b, screening_threshold, ext);
+  long k = 0;
+  for (long li = 0; li < nl; li++) {
+    long m = 0;
+    for (int s = 0; s < nsh; s++) {
+      const double *o = b->origin + 3 * s;
+      double dx = o[0] - lcen[3 * li], dy = o[1] - lcen[3 * li + 1],
+             dz = o[2] - lcen[3 * li + 2];
+      if (sqrt(dx * dx + dy * dy + dz * dz) - ext[s] < lrad[li])
+        m += oracle_shell_size(b->l[s], b->pure[s]);
+    }
+    if (m == 0) continue;
+    if (k < maxleaves) {
+      leaf_n[k] = lcnt[li];
+      leaf_m[k] = m;
+    }
+    k++;
+  }
+  free(sxyz);
+  free(sw);
+  free(perm);
+  free(loff);
+  free(lcnt);
+  free(lcen);
+  free(lrad);
+  free(ext);
+  return k;
+}

@@ -1,0 +1,87 @@
+"""The C ABI library loads, exports every symbol include/occxc.h declares,
+and its host-side argument checks return status codes (no GPU needed: every
+call here is rejected before any device work)."""
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+
+from occ_amd import _lib
+
+HEADER = pathlib.Path(__file__).resolve().parent.parent / "include" / "occxc.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(occxc_[a-z_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, f"{n} missing from the ctypes signature table"
+
+
+def test_version_and_empty_error():
+    L = _lib.lib()
+    assert L.occxc_version().startswith(b"occxc")
+    assert L.occxc_last_error_message() is not None
+
+
+def test_default_options():
+    o = _lib.COptions()
+    _lib.lib().occxc_default_options(ctypes.byref(o))
+    assert o.precision == 0 and o.screen == 1
+    assert o.density_threshold == 1e-10 and o.screening_threshold == 1e-10
+
+
+def test_context_create_rejects_bad_input_without_gpu():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    # null basis / grid / functional
+    st = L.occxc_context_create(None, None, None, None, ctypes.byref(h))
+    assert st == 1
+    ids = np.array([9999], dtype=np.int32)
+    f = _lib.CFunctional(1, _lib.iptr(ids))
+    pts = np.zeros(3)
+    w = np.ones(1)
+    g = _lib.CGrid(1, _lib.dptr(pts), _lib.dptr(w))
+    st = L.occxc_context_create(None, ctypes.byref(g), ctypes.byref(f), None, ctypes.byref(h))
+    assert st == 2  # unknown functional id -> UNSUPPORTED
+    assert b"not found" in L.occxc_last_error_message()
+    o = _lib.COptions()
+    L.occxc_default_options(ctypes.byref(o))
+    o.precision = 7
+    st = L.occxc_context_create(None, ctypes.byref(g), ctypes.byref(f), ctypes.byref(o),
+                                ctypes.byref(h))
+    assert st == 1
+
+
+def test_unsupported_angular_momentum_rejected():
+    from occ_amd.basis import Shell, AOBasis
+    from occ_amd.dft import evaluate_decay_cutoff
+    b = AOBasis([(8, 0, 0, 0)], [Shell(5, [1.0], [1.0], (0, 0, 0), pure=True)])
+    with pytest.raises(_lib.OccxcError) as e:
+        evaluate_decay_cutoff(b)
+    assert e.value.status == 2
+
+
+def test_functional_rejects_missing_sigma():
+    L = _lib.lib()
+    r = np.ones(4)
+    out = np.zeros(4)
+    st = L.occxc_functional_evaluate(101, 4, _lib.dptr(r), None, _lib.dptr(out),
+                                     _lib.dptr(out), None, 0)
+    assert st == 1
+
+
+def test_radial_grid_rejects_unknown_method():
+    r = np.zeros(4)
+    n = np.zeros(1, dtype=np.int32)
+    st = _lib.lib().occxc_radial_grid(42, 4, 1.0, _lib.dptr(r), _lib.dptr(r), _lib.iptr(n))
+    assert st == 2

@@ -1,0 +1,128 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle, through
+the C ABI.  Tolerances (stated per result): |dExc| <= 1e-6 Ha,
+max|dVxc| <= 1e-7, |dN_elec| <= 1e-8 for the FP64 path."""
+import numpy as np
+import pytest
+
+import grid_ref
+import oracle
+from conftest import random_density
+from occ_amd import AOBasis, molecules
+from occ_amd.dft import DFT, DensityFunctional, evaluate_basis, evaluate_decay_cutoff
+from occ_amd.grid import GridSettings, MolecularGrid
+
+pytestmark = pytest.mark.gpu
+
+TOL_E, TOL_V, TOL_N = 1e-6, 1e-7, 1e-8
+
+
+def _grid(atoms, nrad=31, nang=110):
+    g = MolecularGrid(atoms, GridSettings(radial_points=nrad, angular_points=nang))
+    return g.build(device=0)
+
+
+def _xc_case(atoms, basis_name, method, D_seed, pure, nrad=31, nang=110, precision="fp64",
+             screen=True):
+    basis = AOBasis.load(atoms, basis_name, pure=pure)
+    pts, wts = _grid(atoms, nrad, nang)
+    nocc = max(1, molecules.nuclear_charge(atoms) // 2)
+    D = random_density(basis.nbf, min(nocc, basis.nbf), D_seed)
+    print(f"case {basis_name} {method} nbf={basis.nbf} npts={len(wts)} {precision}", flush=True)
+    dft = DFT(method, basis, pts, wts, precision=precision, screen=screen)
+    V = dft.compute_vxc(D)
+    ob = oracle.OracleBasis.from_flat(basis.flat())
+    K, exc, nel = oracle.xc_build(ob, pts, wts, dft.ids, D, screened=screen)
+    return (abs(dft.exchange_correlation_energy() - exc), float(np.abs(V - K).max()),
+            abs(dft.total_density() - nel), exc, nel)
+
+
+def test_evaluate_basis_matches_oracle(rng):
+    atoms = molecules.water_cluster(2, seed=3)
+    for pure in (True, False):
+        basis = AOBasis.load(atoms, "def2-svp", pure=pure)
+        pts = rng.normal(size=(500, 3)) * 3.0
+        out = evaluate_basis(basis, pts, deriv=1)
+        ob = oracle.OracleBasis.from_flat(basis.flat())
+        fb = basis.first_bf()
+        for s, sh in enumerate(basis.shells):
+            ref = oracle.eval_shell(sh.l, sh.pure, sh.exponents, sh.coefficients, sh.origin, pts)
+            got = out[:, fb[s]:fb[s] + sh.size(), :]
+            np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
+        assert ob.nbf == basis.nbf
+
+
+def test_evaluate_basis_l3_l4(rng):
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "def2-tzvp", pure=True)  # O has f
+    pts = rng.normal(size=(300, 3)) * 2.0
+    out = evaluate_basis(basis, pts, deriv=1)
+    fb = basis.first_bf()
+    for s, sh in enumerate(basis.shells):
+        ref = oracle.eval_shell(sh.l, sh.pure, sh.exponents, sh.coefficients, sh.origin, pts)
+        np.testing.assert_allclose(out[:, fb[s]:fb[s] + sh.size(), :], ref, rtol=1e-12,
+                                   atol=1e-14)
+
+
+def test_decay_cutoff_matches_oracle():
+    atoms = molecules.water_cluster(2, seed=5)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    ob = oracle.OracleBasis.from_flat(basis.flat())
+    np.testing.assert_array_equal(evaluate_decay_cutoff(basis, 1e-10),
+                                  oracle.decay_cutoff(ob, 1e-10))
+
+
+@pytest.mark.parametrize("name,fid", [("lda_x", 1), ("lda_c_vwn", 7), ("lda_c_pw_mod", 13),
+                                      ("gga_x_pbe", 101), ("gga_x_b88", 106),
+                                      ("gga_c_pbe", 130), ("gga_c_lyp", 131)])
+def test_functional_matches_oracle(rng, name, fid):
+    rho = 10 ** rng.uniform(-12, 3, 4000)
+    sigma = 10 ** rng.uniform(-14, 2, 4000) * rho ** (8 / 3)
+    exc, vr, vs = DensityFunctional(name).evaluate(rho, sigma)
+    e0, v0, s0 = oracle.functional_eval(fid, rho, sigma)
+    np.testing.assert_allclose(exc, e0, rtol=1e-11, atol=1e-300)
+    np.testing.assert_allclose(vr, v0, rtol=1e-10, atol=1e-300)
+    if fid > 100:
+        np.testing.assert_allclose(vs, s0, rtol=1e-10, atol=1e-300)
+
+
+def test_becke_partition_matches_numpy():
+    atoms = molecules.water_cluster(2, seed=7)
+    g = MolecularGrid(atoms, GridSettings(radial_points=21, angular_points=50))
+    pts, w_gpu = g.build(device=0)
+    _, w_cpu = grid_ref.molecular_grid(g)
+    np.testing.assert_allclose(w_gpu, w_cpu, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+def test_xc_water_631g(method):
+    de, dv, dn, exc, nel = _xc_case(molecules.water(), "6-31g", method, 11, pure=False)
+    print(f"water/6-31G {method}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
+    assert de <= TOL_E and dv <= TOL_V and dn <= TOL_N
+
+
+@pytest.mark.parametrize("method", ["pbe", "blyp", "svwn5"])
+def test_xc_water_dimer_def2svp(method):
+    de, dv, dn, exc, nel = _xc_case(molecules.water_cluster(2, seed=1), "def2-svp", method, 12,
+                                    pure=True)
+    print(f"(H2O)2/def2-SVP {method}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
+    assert de <= TOL_E and dv <= TOL_V and dn <= TOL_N
+
+
+def test_xc_dense_path():
+    de, dv, dn, exc, nel = _xc_case(molecules.water(), "6-31g", "pbe", 13, pure=False,
+                                    screen=False)
+    assert de <= TOL_E and dv <= TOL_V and dn <= TOL_N
+
+
+def test_xc_alkane_cartesian_d():
+    de, dv, dn, exc, nel = _xc_case(molecules.alkane(3), "def2-svp", "pbe", 14, pure=False,
+                                    nrad=21, nang=50)
+    assert de <= TOL_E and dv <= TOL_V and dn <= TOL_N
+
+
+@pytest.mark.parametrize("precision", ["fp32"])
+def test_xc_mixed_precision_reported(precision):
+    de, dv, dn, exc, nel = _xc_case(molecules.water_cluster(2, seed=1), "def2-svp", "pbe", 12,
+                                    pure=True, precision=precision)
+    print(f"{precision}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
+    assert de <= TOL_E
