@@ -142,6 +142,23 @@ occxc_status occxc_synchronize(occxc_context *ctx);
  * the context's stream. */
 occxc_status occxc_context_evaluate_ao(occxc_context *ctx);
 
+/* Per-stage device timing, the counterpart of OCC's occ::timing categories
+ * dft_xc / gto (include/occ/core/timings.h:11-133): when enabled, every build
+ * records HIP events around its stages on the stream it runs on.
+ * occxc_context_get_timing waits for them and returns the summed
+ * milliseconds and the number of timed launches per stage. */
+enum {
+  OCCXC_STAGE_PREPARE = 0,   /* D2 = 2 sym(D), K = 0 */
+  OCCXC_STAGE_DENSITY = 1,   /* rho, grad rho, functional, coefficients */
+  OCCXC_STAGE_VXC = 2,       /* phi^T A per leaf into K */
+  OCCXC_STAGE_FINALIZE = 3,  /* K + K^T, Exc / N_elec reduction */
+  OCCXC_STAGE_AO = 4,        /* AO values of every leaf */
+  OCCXC_NSTAGES = 5
+};
+occxc_status occxc_context_set_timing(occxc_context *ctx, int enable);
+occxc_status occxc_context_get_timing(occxc_context *ctx, double *ms,
+                                      long long *counts);
+
 /* --- stage-level entry points (each mirrors one reference function) --- */
 
 /* gto::evaluate_basis (src/gto/gto.cpp:69-140): phi on npts points for the

@@ -67,6 +67,8 @@ SIGNATURES = {
                                           ctypes.c_void_p, ctypes.c_void_p]),
     "occxc_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "occxc_context_evaluate_ao": (ctypes.c_int, [ctypes.c_void_p]),
+    "occxc_context_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "occxc_context_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _llp]),
     "occxc_evaluate_basis": (ctypes.c_int, [ctypes.POINTER(CBasis), ctypes.c_longlong, _dp,
                                             ctypes.c_int, _dp, ctypes.c_int]),
     "occxc_evaluate_decay_cutoff": (ctypes.c_int, [ctypes.POINTER(CBasis), ctypes.c_double,

@@ -226,6 +226,11 @@ struct occxc_context {
   DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
   ContractPlan plan;
   BuildArgs args{};
+  // stage timing: pending event sets, summed on read-out
+  bool timing = false;
+  std::vector<std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>> pending;
+  double stage_ms[OCCXC_NSTAGES] = {0};
+  long long stage_n[OCCXC_NSTAGES] = {0};
   bool ao_ready = false;
   size_t device_bytes = 0;
 };
@@ -362,7 +367,8 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
                       " significant basis functions");
     L.m = m;
     L.mp = (int)round_up(m, kAoPad);
-    for (int q = m; q < L.mp; ++q) ao_map.push_back(-1);
+    // padding AOs map to AO 0: their values are zero, so they contribute nothing
+    for (int q = m; q < L.mp; ++q) ao_map.push_back(0);
     L.ao_off = ao_off;
     ao_off += (long long)c->ncomp * L.mp * L.np;
     const long long o = h.off[li];
@@ -442,7 +448,7 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
 
-  c->plan = make_contract_plan(c->leaves, c->deriv, opt->precision, c->stream);
+  c->plan = make_contract_plan(c->leaves, c->nbf, c->deriv, opt->precision, c->stream);
   c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
                     c->d_ao_map.bytes() + c->d_leaves.bytes() + c->plan.bytes;
 
@@ -451,19 +457,63 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   c->ao_ready = true;
 }
 
+// Records a pair of events around fn() on stream s when timing is enabled.
+struct StageTimer {
+  occxc_context *c;
+  hipStream_t s;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> set;
+  template <typename F> void run(int stage, F &&fn) {
+    if (!c->timing) {
+      fn();
+      return;
+    }
+    hipEvent_t a, b;
+    OCCXC_HIP(hipEventCreate(&a));
+    OCCXC_HIP(hipEventCreate(&b));
+    OCCXC_HIP(hipEventRecord(a, s));
+    fn();
+    OCCXC_HIP(hipEventRecord(b, s));
+    set.push_back({stage, {a, b}});
+  }
+  ~StageTimer() {
+    if (!set.empty()) c->pending.push_back(std::move(set));
+  }
+};
+
+void collect_timing(occxc_context *c) {
+  for (auto &set : c->pending)
+    for (auto &e : set) {
+      float ms = 0.f;
+      OCCXC_HIP(hipEventSynchronize(e.second.second));
+      OCCXC_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+      c->stage_ms[e.first] += ms;
+      c->stage_n[e.first] += 1;
+      (void)hipEventDestroy(e.second.first);
+      (void)hipEventDestroy(e.second.second);
+    }
+  c->pending.clear();
+}
+
 void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_result,
                hipStream_t s) {
   const BuildArgs &a = c->args;
   const long long nn = (long long)c->nbf * c->nbf;
-  launch_prepare_density(d_D, c->d_D2.get(), c->nbf, s);
-  launch_zero(c->d_K.get(), nn, s);
-  if (c->opt.precision == OCCXC_PREC_FP64 && !c->plan.use_mfma) {
-    launch_density_functional(a, c->d_ao.get(), false, s);
-    launch_vxc(a, c->d_ao.get(), false, s);
+  StageTimer t{c, s, {}};
+  t.run(OCCXC_STAGE_PREPARE, [&] {
+    launch_prepare_density(d_D, c->d_D2.get(), c->nbf, s);
+    launch_zero(c->d_K.get(), nn, s);
+  });
+  if (!c->plan.use_mfma) {
+    const bool fp32 = c->opt.precision != OCCXC_PREC_FP64;
+    t.run(OCCXC_STAGE_DENSITY, [&] { launch_density_functional(a, c->d_ao.get(), fp32, s); });
+    t.run(OCCXC_STAGE_VXC, [&] { launch_vxc(a, c->d_ao.get(), fp32, s); });
   } else {
-    run_contractions(c->plan, a, c->d_ao.get(), c->opt.precision, s);
+    t.run(OCCXC_STAGE_DENSITY, [&] { run_density(c->plan, a, c->d_ao.get(), c->opt.precision, s); });
+    t.run(OCCXC_STAGE_VXC, [&] { run_vxc(c->plan, a, c->d_ao.get(), c->opt.precision, s); });
   }
-  launch_finalize(c->d_K.get(), d_V, c->nbf, c->d_part.get(), c->nleaves, d_result, s);
+  t.run(OCCXC_STAGE_FINALIZE, [&] {
+    launch_finalize(c->d_K.get(), d_V, c->nbf, c->d_part.get(), c->nleaves, d_result, s);
+  });
 }
 
 } // namespace
@@ -513,6 +563,10 @@ occxc_status occxc_context_destroy(occxc_context *ctx) {
         (void)hipStreamDestroy(ctx->stream);
       }
       ctx->stream = nullptr;
+      try {
+        collect_timing(ctx);
+      } catch (...) {
+      }
       delete ctx;
     }
   });
@@ -574,7 +628,35 @@ occxc_status occxc_context_evaluate_ao(occxc_context *ctx) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     DeviceGuard g(ctx->device);
-    launch_ao_eval(ctx->args, ctx->d_ao.get(), ctx->opt.precision != 0, ctx->stream);
+    StageTimer t{ctx, ctx->stream, {}};
+    t.run(OCCXC_STAGE_AO, [&] {
+      launch_ao_eval(ctx->args, ctx->d_ao.get(), ctx->opt.precision != 0, ctx->stream);
+    });
+  });
+}
+
+occxc_status occxc_context_set_timing(occxc_context *ctx, int enable) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    collect_timing(ctx);
+    for (int i = 0; i < OCCXC_NSTAGES; ++i) {
+      ctx->stage_ms[i] = 0.0;
+      ctx->stage_n[i] = 0;
+    }
+    ctx->timing = enable != 0;
+  });
+}
+
+occxc_status occxc_context_get_timing(occxc_context *ctx, double *ms, long long *counts) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && ms && counts, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    collect_timing(ctx);
+    for (int i = 0; i < OCCXC_NSTAGES; ++i) {
+      ms[i] = ctx->stage_ms[i];
+      counts[i] = ctx->stage_n[i];
+    }
   });
 }
 

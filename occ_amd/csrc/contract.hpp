@@ -7,18 +7,23 @@
 #include <cstddef>
 #include <vector>
 
+#include "common.hpp"
 #include "kernels.hpp"
 
 namespace occxc {
 
 struct ContractPlan {
-  bool use_mfma = false;
+  bool use_mfma = false;   // false: the FP64 reference kernels of kernels.hip
+  int max_np = 0;          // largest padded leaf
+  DevBuf<float> d2f;       // FP32 copy of the doubled density (FP32 modes)
   size_t bytes = 0;
 };
 
-ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, int deriv, int precision,
-                                hipStream_t s);
-void run_contractions(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
-                      hipStream_t s);
+ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, int nbf, int deriv,
+                                int precision, hipStream_t s);
+void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
+                 hipStream_t s);
+void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
+             hipStream_t s);
 
 } // namespace occxc

@@ -431,6 +431,9 @@ __global__ void k_becke(int natoms, const double *__restrict__ axyz,
                         const double *__restrict__ dist, int ta, long long npts,
                         const double *__restrict__ pts,
                         const int *__restrict__ atom_of_point, double *__restrict__ w) {
+  // no fused multiply-add here: the cell functions subtract nearly equal
+  // numbers near cell boundaries, so keep the reference's separate roundings
+#pragma clang fp contract(off)
   const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (p >= npts) return;
   const double x = pts[3 * p], y = pts[3 * p + 1], z = pts[3 * p + 2];

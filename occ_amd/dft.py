@@ -105,6 +105,18 @@ class XCContext:
                                                  ctypes.c_void_p(d_result),
                                                  ctypes.c_void_p(stream or 0)))
 
+    STAGES = ("prepare", "density", "vxc", "finalize", "ao")
+
+    def set_timing(self, enable=True):
+        _lib.check(_lib.lib().occxc_context_set_timing(self._h, int(bool(enable))))
+
+    def timing(self):
+        """{stage: (total ms, launches)} of the timed builds since set_timing."""
+        ms = np.zeros(len(self.STAGES))
+        n = np.zeros(len(self.STAGES), dtype=np.int64)
+        _lib.check(_lib.lib().occxc_context_get_timing(self._h, _lib.dptr(ms), _lib.llptr(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
+
     def evaluate_ao(self):
         _lib.check(_lib.lib().occxc_context_evaluate_ao(self._h))
 

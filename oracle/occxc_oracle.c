@@ -356,6 +356,8 @@ typedef double complex cplx;
 static const double CX = 0.73855876638202240587; /* (3/4)(3/pi)^(1/3) */
 
 static cplx c_cbrt(cplx z) { return cpow(z, 1.0 / 3.0); }
+/* log(1 + u) = 2 atanh(u / (2 + u)): analytic, and accurate for small u */
+static cplx c_log1p(cplx u) { return 2.0 * catanh(u / (2.0 + u)); }
 static cplx c_rs(cplx rho) { return c_cbrt(3.0 / (4.0 * M_PI * rho)); }
 
 /* Dirac/Slater exchange */
@@ -378,7 +380,7 @@ static cplx eps_pw92(cplx rs, double A) {
   const double a1 = 0.21370, b1 = 7.5957, b2 = 3.5876, b3 = 1.6382, b4 = 0.49294;
   cplx srs = csqrt(rs);
   cplx den = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
-  return -2.0 * A * (1.0 + a1 * rs) * clog(1.0 + 1.0 / den);
+  return -2.0 * A * (1.0 + a1 * rs) * c_log1p(1.0 / den);
 }
 
 /* PBE: J.P. Perdew, K. Burke, M. Ernzerhof, PRL 77, 3865 (1996) */
@@ -402,7 +404,7 @@ static cplx e_pbe_c(cplx rho, cplx sigma) {
   cplx t2 = sigma / (4.0 * ks * ks * rho * rho);
   cplx A = (PBE_BETA / gamma) / (cexp(-ec / gamma) - 1.0);
   cplx At2 = A * t2;
-  cplx H = gamma * clog(1.0 + (PBE_BETA / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  cplx H = gamma * c_log1p((PBE_BETA / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
   return rho * (ec + H);
 }
 

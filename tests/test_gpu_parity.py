@@ -79,18 +79,23 @@ def test_functional_matches_oracle(rng, name, fid):
     sigma = 10 ** rng.uniform(-14, 2, 4000) * rho ** (8 / 3)
     exc, vr, vs = DensityFunctional(name).evaluate(rho, sigma)
     e0, v0, s0 = oracle.functional_eval(fid, rho, sigma)
-    np.testing.assert_allclose(exc, e0, rtol=1e-11, atol=1e-300)
-    np.testing.assert_allclose(vr, v0, rtol=1e-10, atol=1e-300)
+    np.testing.assert_allclose(exc, e0, rtol=1e-11, atol=1e-200)
+    np.testing.assert_allclose(vr, v0, rtol=1e-10, atol=1e-200)
     if fid > 100:
-        np.testing.assert_allclose(vs, s0, rtol=1e-10, atol=1e-300)
+        np.testing.assert_allclose(vs, s0, rtol=1e-10, atol=1e-200)
 
 
 def test_becke_partition_matches_numpy():
+    # compared as partition fractions P_own / sum_k P_k: near cell boundaries
+    # the cell function 0.5 (1 - v) cancels, so weights there agree in
+    # absolute fraction, not in relative value
     atoms = molecules.water_cluster(2, seed=7)
     g = MolecularGrid(atoms, GridSettings(radial_points=21, angular_points=50))
     pts, w_gpu = g.build(device=0)
     _, w_cpu = grid_ref.molecular_grid(g)
-    np.testing.assert_allclose(w_gpu, w_cpu, rtol=1e-12, atol=1e-300)
+    _, w0, _ = g.unpartitioned()
+    assert np.all(np.abs(w_gpu - w_cpu) <= 1e-10 * w0 + 1e-300)
+    assert np.abs(w_gpu - w_cpu).max() / w0.max() < 1e-14
 
 
 @pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
@@ -120,9 +125,59 @@ def test_xc_alkane_cartesian_d():
     assert de <= TOL_E and dv <= TOL_V and dn <= TOL_N
 
 
+def test_mfma_matches_reference_kernels(monkeypatch):
+    """The MFMA contractions against the FP64 reference kernels of kernels.hip."""
+    atoms = molecules.water_cluster(2, seed=4)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 10, 21)
+    out = {}
+    for ref in (False, True):
+        if ref:
+            monkeypatch.setenv("OCCXC_REFERENCE_KERNELS", "1")
+        dft = DFT("blyp", basis, pts, wts)
+        out[ref] = (dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density())
+    assert abs(out[True][1] - out[False][1]) <= 1e-10
+    assert np.abs(out[True][0] - out[False][0]).max() <= 1e-11
+    assert abs(out[True][2] - out[False][2]) <= 1e-11
+
+
 @pytest.mark.parametrize("precision", ["fp32"])
 def test_xc_mixed_precision_reported(precision):
     de, dv, dn, exc, nel = _xc_case(molecules.water_cluster(2, seed=1), "def2-svp", "pbe", 12,
                                     pure=True, precision=precision)
     print(f"{precision}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
     assert de <= TOL_E
+
+
+def test_cpp_driver_drop_in(tmp_path):
+    """A C++ caller through include/occxc_dft.hpp gives the Python result and
+    repeats it exactly on the second build of a resident context."""
+    import subprocess
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 5, 31)
+    f = basis.flat()
+    ids = np.array([101, 130], dtype=np.int32)
+    np.array([len(f["l"]), len(f["exponents"]), len(wts), basis.nbf, len(ids)],
+             dtype=np.int64).tofile(tmp_path / "header.bin")
+    for k in ("l", "pure", "nprim"):
+        f[k].astype(np.int32).tofile(tmp_path / f"{k}.bin")
+    for k in ("origin", "exponents", "coefficients"):
+        f[k].astype(np.float64).tofile(tmp_path / f"{k}.bin")
+    ids.tofile(tmp_path / "ids.bin")
+    np.ascontiguousarray(pts).tofile(tmp_path / "points.bin")
+    wts.tofile(tmp_path / "weights.bin")
+    np.asfortranarray(D).ravel(order="F").tofile(tmp_path / "density.bin")
+    exe = __import__("pathlib").Path(__file__).parent / "native" / "xc_driver"
+    out = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = [l.split() for l in out.stdout.strip().splitlines()]
+    assert lines[0][2:] == lines[1][2:]
+    dft = DFT("pbe", basis, pts, wts)
+    V = dft.compute_vxc(D)
+    assert float(lines[0][3]) == pytest.approx(dft.exchange_correlation_energy(), abs=1e-12)
+    assert float(lines[0][5]) == pytest.approx(dft.total_density(), abs=1e-12)
+    cs = float(np.sum(V.ravel(order="F") * ((np.arange(V.size) % 7) + 1)))
+    assert float(lines[0][7]) == pytest.approx(cs, rel=1e-12)
