@@ -20,9 +20,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "fp32cache": 78.6}
 PEAK_HBM_GBS = 8000.0
-DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3-f32acc"}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3-f32acc", "fp32cache": "f64 (f32 AO cache)"}
 
 
 def log(*a):
@@ -51,9 +51,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="h2o32")
-    ap.add_argument("--modes", default="fp64,fp32")
+    ap.add_argument("--modes", default="fp64,fp32cache,fp32,bf16x3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle (no accuracy, "
                     "no cpu_baseline)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device (a known-size kernel against "
+                         "which profiler byte counters are scaled)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the XC build shards "
@@ -62,7 +65,15 @@ def main():
         sys.exit(2)
 
     from occ_amd import AOBasis
-    from occ_amd.device import DeviceArray, synchronize
+    from occ_amd.device import DeviceArray, copy_device, synchronize
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        a, b = DeviceArray(n), DeviceArray(n)
+        copy_device(b.ptr, a.ptr, 8 * n)
+        synchronize()
+        log(f"copy calibration: {8 * n} bytes device to device")
+        a.free()
+        b.free()
     from occ_amd.dft import XCContext, functional_ids
     from occ_amd.grid import GridSettings, MolecularGrid
 

@@ -62,9 +62,10 @@ enum {
 };
 
 typedef enum {
-  OCCXC_PREC_FP64 = 0,   /* FP64 AO cache and contractions */
-  OCCXC_PREC_FP32 = 1,   /* FP32 AO cache, FP32 contractions, FP32 accumulation */
-  OCCXC_PREC_BF16X3 = 2  /* FP32 AO cache, 3-term split-BF16 MFMA, FP32 accumulation */
+  OCCXC_PREC_FP64 = 0,       /* FP64 AO cache and contractions */
+  OCCXC_PREC_FP32 = 1,       /* FP32 AO cache, FP32 contractions, FP32 accumulation */
+  OCCXC_PREC_BF16X3 = 2,     /* FP32 AO cache, 3-term split-BF16 MFMA, FP32 accumulation */
+  OCCXC_PREC_FP32_CACHE = 3  /* FP32 AO cache, FP64 density matrix, FP64 contractions */
 } occxc_precision;
 
 typedef struct {
@@ -130,9 +131,31 @@ occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc,
 
 /* Same build on device-resident buffers, enqueued on `stream` (a hipStream_t,
  * NULL for the context's own stream) without host synchronisation.
- * d_result receives {Exc, N_elec}. */
+ * d_result receives {Exc, N_elec} (per molecule for a batch context, whose
+ * D and Vxc are packed as occxc_batch_layout describes). */
 occxc_status occxc_build_device(occxc_context *ctx, const double *d_D,
                                 double *d_Vxc, double *d_result, void *stream);
+
+/* --- batches: many molecules in one context, one launch per stage ----- */
+
+/* A context over nmol molecules (each its own basis and grid; shared
+ * functional and options).  Leaves of all molecules are processed by the same
+ * kernel launches. */
+occxc_status occxc_batch_create(int nmol, const occxc_basis *bases,
+                                const occxc_grid *grids,
+                                const occxc_functional *func,
+                                const occxc_options *opt, occxc_context **ctx);
+
+/* Host build of every molecule: D[m], Vxc[m] are nbf_m x nbf_m column-major. */
+occxc_status occxc_batch_build(occxc_context *ctx, const double *const *D,
+                               double *const *Vxc, double *exc, double *nelec);
+
+/* Molecule count, per-molecule nbf and the element offsets of each
+ * molecule's nbf x nbf block in the packed device layout used by
+ * occxc_build_device (D and Vxc packed back to back; d_result holds
+ * {Exc, N_elec} per molecule). nbf / offsets may be NULL. */
+occxc_status occxc_batch_layout(const occxc_context *ctx, int *nmol, int *nbf,
+                                long long *offsets);
 
 /* Wait for work enqueued on the context's stream. */
 occxc_status occxc_synchronize(occxc_context *ctx);

@@ -63,6 +63,13 @@ SIGNATURES = {
     "occxc_context_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "occxc_context_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CContextInfo)]),
     "occxc_build": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "occxc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CBasis),
+                                          ctypes.POINTER(CGrid), ctypes.POINTER(CFunctional),
+                                          ctypes.POINTER(COptions),
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "occxc_batch_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_dp),
+                                         ctypes.POINTER(_dp), _dp, _dp]),
+    "occxc_batch_layout": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _llp]),
     "occxc_build_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),
     "occxc_synchronize": (ctypes.c_int, [ctypes.c_void_p]),

@@ -21,7 +21,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/occxc.h"
+#include "occxc.h"
 #include "common.hpp"
 #include "contract.hpp"
 #include "kernels.hpp"
@@ -211,6 +211,10 @@ struct occxc_context {
   hipStream_t stream = nullptr;
   occxc_options opt{};
   int nbf = 0, nsh = 0, deriv = 0, ncomp = 1, nfunc = 0;
+  int nmol = 0, maxnbf = 0;
+  long long mtot = 0;
+  std::vector<int> mol_nbf;
+  std::vector<long long> mol_off;
   int funcs[kMaxFunc] = {0};
   long long npts = 0, ptot = 0;
   int nleaves = 0;
@@ -220,6 +224,7 @@ struct occxc_context {
   DevBuf<ShellDev> d_shells;
   DevBuf<double> d_exps, d_coefs;
   DevBuf<LeafDev> d_leaves;
+  DevBuf<MolDev> d_mols;
   DevBuf<int> d_leaf_shells, d_ao_map, d_active;
   DevBuf<double> d_px, d_py, d_pz, d_pw;
   DevBuf<char> d_ao;
@@ -254,15 +259,14 @@ template <typename F> occxc_status guarded(F &&f) {
   }
 }
 
-void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid *grid,
-                   const occxc_functional *func, const occxc_options *opt) {
+void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
+                   const occxc_grid *grids, const occxc_functional *func,
+                   const occxc_options *opt) {
   c->opt = *opt;
-  OCCXC_REQUIRE(opt->precision >= 0 && opt->precision <= 2, OCCXC_ERR_INVALID_ARGUMENT,
+  OCCXC_REQUIRE(nmol >= 1 && bases && grids, OCCXC_ERR_INVALID_ARGUMENT,
+                "need at least one molecule (basis and grid)");
+  OCCXC_REQUIRE(opt->precision >= 0 && opt->precision <= 3, OCCXC_ERR_INVALID_ARGUMENT,
                 "options: unknown precision mode");
-  OCCXC_REQUIRE(grid && grid->npts >= 0 && (grid->npts == 0 || (grid->points && grid->weights)),
-                OCCXC_ERR_INVALID_ARGUMENT, "grid: null pointer");
-  OCCXC_REQUIRE(grid->npts < (1LL << 31) - kMaxLeafPts * 4, OCCXC_ERR_UNSUPPORTED,
-                "grid: more than 2^31 points");
   OCCXC_REQUIRE(func && func->nfunc >= 1 && func->nfunc <= kMaxFunc && func->ids,
                 OCCXC_ERR_INVALID_ARGUMENT, "functional: need 1..8 functional ids");
   int deriv = 0;
@@ -276,114 +280,158 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   c->nfunc = func->nfunc;
   c->deriv = deriv;
   c->ncomp = deriv ? 4 : 1;
-  HostBasis hb = make_basis(basis);
-  c->nbf = hb.nbf;
-  c->nsh = hb.nsh;
-  for (long long i = 0; i < grid->npts; ++i) {
-    OCCXC_REQUIRE(std::isfinite(grid->points[3 * i]) && std::isfinite(grid->points[3 * i + 1]) &&
-                      std::isfinite(grid->points[3 * i + 2]) && std::isfinite(grid->weights[i]),
-                  OCCXC_ERR_INVALID_ARGUMENT, "grid: non-finite point or weight");
+
+  // molecules: bases concatenated (shell and primitive offsets shifted)
+  std::vector<HostBasis> hbs;
+  std::vector<ShellDev> shells;
+  std::vector<double> exps, coefs;
+  std::vector<int> shell_base(nmol + 1, 0);
+  long long npts_total = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const occxc_grid *grid = &grids[m];
+    OCCXC_REQUIRE(grid->npts >= 0 && (grid->npts == 0 || (grid->points && grid->weights)),
+                  OCCXC_ERR_INVALID_ARGUMENT, "grid: null pointer");
+    for (long long i = 0; i < grid->npts; ++i)
+      OCCXC_REQUIRE(std::isfinite(grid->points[3 * i]) && std::isfinite(grid->points[3 * i + 1]) &&
+                        std::isfinite(grid->points[3 * i + 2]) && std::isfinite(grid->weights[i]),
+                    OCCXC_ERR_INVALID_ARGUMENT, "grid: non-finite point or weight");
+    npts_total += grid->npts;
+    hbs.push_back(make_basis(&bases[m]));
+    HostBasis &hb = hbs.back();
+    const int pbase = (int)exps.size();
+    for (ShellDev sh : hb.shells) {
+      sh.poff += pbase;
+      shells.push_back(sh);
+    }
+    exps.insert(exps.end(), hb.exps.begin(), hb.exps.end());
+    coefs.insert(coefs.end(), hb.coefs.begin(), hb.coefs.end());
+    shell_base[m + 1] = shell_base[m] + hb.nsh;
+    c->mol_nbf.push_back(hb.nbf);
   }
+  OCCXC_REQUIRE(npts_total < (1LL << 31) - (long long)kMaxLeafPts * 4 * nmol,
+                OCCXC_ERR_UNSUPPORTED, "grid: more than 2^31 points in one context");
+  c->nmol = nmol;
+  c->nsh = (int)shells.size();
+  c->nbf = hbs[0].nbf;
+  c->npts = npts_total;
+  std::vector<MolDev> mols(nmol);
+  long long moff = 0;
+  for (int m = 0; m < nmol; ++m) {
+    mols[m].moff = moff;
+    mols[m].nbf = hbs[m].nbf;
+    c->mol_off.push_back(moff);
+    moff += (long long)hbs[m].nbf * hbs[m].nbf;
+    c->maxnbf = std::max(c->maxnbf, hbs[m].nbf);
+  }
+  c->mtot = moff;
 
   DeviceGuard guard(opt->device);
   c->device = opt->device;
   ensure_device_tables(c->device);
   OCCXC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  c->d_shells.alloc(hb.shells.size());
-  c->d_shells.upload(hb.shells.data(), hb.shells.size(), c->stream);
-  c->d_exps.alloc(hb.exps.size());
-  c->d_exps.upload(hb.exps.data(), hb.exps.size(), c->stream);
-  c->d_coefs.alloc(hb.coefs.size());
-  c->d_coefs.upload(hb.coefs.data(), hb.coefs.size(), c->stream);
+  c->d_shells.alloc(shells.size());
+  c->d_shells.upload(shells.data(), shells.size(), c->stream);
+  c->d_exps.alloc(exps.size());
+  c->d_exps.upload(exps.data(), exps.size(), c->stream);
+  c->d_coefs.alloc(coefs.size());
+  c->d_coefs.upload(coefs.data(), coefs.size(), c->stream);
 
   // shell extents on the device (gto.cpp:18-67)
-  std::vector<double> ext(hb.nsh, 30.0);
+  std::vector<double> ext(shells.size(), 30.0);
   const bool screened = opt->screen != 0 && opt->screening_threshold > 0.0;
   if (screened) {
-    DevBuf<double> d_ext(hb.nsh);
-    launch_decay_cutoff(c->d_shells.get(), hb.nsh, c->d_exps.get(), c->d_coefs.get(),
+    DevBuf<double> d_ext(shells.size());
+    launch_decay_cutoff(c->d_shells.get(), (int)shells.size(), c->d_exps.get(), c->d_coefs.get(),
                         opt->screening_threshold, d_ext.get(), c->stream);
-    d_ext.download(ext.data(), hb.nsh, c->stream);
+    d_ext.download(ext.data(), shells.size(), c->stream);
     OCCXC_HIP(hipStreamSynchronize(c->stream));
   }
 
-  // leaves: Morton hierarchy (screened) or 128-point chunks in input order
-  Hierarchy h;
-  const long long n = grid->npts;
-  c->npts = n;
-  if (screened) {
-    h = build_hierarchy(n, grid->points, grid->weights);
-  } else {
-    h.xyz.assign(grid->points, grid->points + 3 * n);
-    h.w.assign(grid->weights, grid->weights + n);
-    for (long long o = 0; o < n; o += 128) {
-      h.off.push_back(o);
-      h.cnt.push_back(std::min<long long>(128, n - o));
-      h.center.insert(h.center.end(), {0.0, 0.0, 0.0});
-      h.radius.push_back(0.0);
-    }
-  }
-  const long long nl_all = (long long)h.off.size();
-  // significant shells per leaf (dft.h:314-320)
-  std::vector<std::vector<int>> sig(nl_all);
-  parallel_for(nl_all, [&](long long lo, long long hi) {
-    for (long long li = lo; li < hi; ++li) {
-      if (h.cnt[li] == 0) continue;
-      auto &v = sig[li];
-      for (int s = 0; s < hb.nsh; ++s) {
-        if (screened) {
-          const ShellDev &sh = hb.shells[s];
-          const double dx = sh.x - h.center[3 * li], dy = sh.y - h.center[3 * li + 1],
-                       dz = sh.z - h.center[3 * li + 2];
-          const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
-          if (dist - ext[s] < h.radius[li]) v.push_back(s);
-        } else {
-          v.push_back(s);
-        }
-      }
-    }
-  });
   std::vector<int> leaf_shells, ao_map;
   std::vector<double> px, py, pz, pw;
   long long pt_off = 0, ao_off = 0;
   double msum = 0.0;
-  for (long long li = 0; li < nl_all; ++li) {
-    if (sig[li].empty()) continue;
-    LeafDev L{};
-    L.n = (int)h.cnt[li];
-    L.np = (int)round_up(L.n, kPtPad);
-    L.pt_off = (int)pt_off;
-    L.sh_off = (int)leaf_shells.size();
-    L.nsig = (int)sig[li].size();
-    L.map_off = (int)ao_map.size();
-    int m = 0;
-    for (int s : sig[li]) {
-      leaf_shells.push_back(s);
-      for (int q = 0; q < hb.shells[s].size; ++q) ao_map.push_back(hb.shells[s].bf0 + q);
-      m += hb.shells[s].size;
+  for (int mol = 0; mol < nmol; ++mol) {
+    const occxc_grid *grid = &grids[mol];
+    const HostBasis &hb = hbs[mol];
+    const int sb = shell_base[mol];
+    // leaves: Morton hierarchy (screened) or 128-point chunks in input order
+    Hierarchy h;
+    const long long n = grid->npts;
+    if (screened) {
+      h = build_hierarchy(n, grid->points, grid->weights);
+    } else {
+      h.xyz.assign(grid->points, grid->points + 3 * n);
+      h.w.assign(grid->weights, grid->weights + n);
+      for (long long o = 0; o < n; o += 128) {
+        h.off.push_back(o);
+        h.cnt.push_back(std::min<long long>(128, n - o));
+        h.center.insert(h.center.end(), {0.0, 0.0, 0.0});
+        h.radius.push_back(0.0);
+      }
     }
-    OCCXC_REQUIRE(m <= kMaxLocalAo, OCCXC_ERR_UNSUPPORTED,
-                  "a grid leaf has more than " + std::to_string(kMaxLocalAo) +
-                      " significant basis functions");
-    L.m = m;
-    L.mp = (int)round_up(m, kAoPad);
-    // padding AOs map to AO 0: their values are zero, so they contribute nothing
-    for (int q = m; q < L.mp; ++q) ao_map.push_back(0);
-    L.ao_off = ao_off;
-    ao_off += (long long)c->ncomp * L.mp * L.np;
-    const long long o = h.off[li];
-    for (int p = 0; p < L.np; ++p) {
-      const long long src = o + (p < L.n ? p : 0);
-      px.push_back(h.xyz[3 * src]);
-      py.push_back(h.xyz[3 * src + 1]);
-      pz.push_back(h.xyz[3 * src + 2]);
-      pw.push_back(p < L.n ? h.w[src] : 0.0);
+    const long long nl_all = (long long)h.off.size();
+    // significant shells per leaf (dft.h:314-320)
+    std::vector<std::vector<int>> sig(nl_all);
+    parallel_for(nl_all, [&](long long lo, long long hi) {
+      for (long long li = lo; li < hi; ++li) {
+        if (h.cnt[li] == 0) continue;
+        auto &v = sig[li];
+        for (int s = 0; s < hb.nsh; ++s) {
+          if (screened) {
+            const ShellDev &sh = hb.shells[s];
+            const double dx = sh.x - h.center[3 * li], dy = sh.y - h.center[3 * li + 1],
+                         dz = sh.z - h.center[3 * li + 2];
+            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist - ext[sb + s] < h.radius[li]) v.push_back(s);
+          } else {
+            v.push_back(s);
+          }
+        }
+      }
+    });
+    mols[mol].leaf_begin = (int)c->leaves.size();
+    for (long long li = 0; li < nl_all; ++li) {
+      if (sig[li].empty()) continue;
+      LeafDev L{};
+      L.n = (int)h.cnt[li];
+      L.np = (int)round_up(L.n, kPtPad);
+      L.pt_off = (int)pt_off;
+      L.sh_off = (int)leaf_shells.size();
+      L.nsig = (int)sig[li].size();
+      L.map_off = (int)ao_map.size();
+      L.mol = mol;
+      int m = 0;
+      for (int s : sig[li]) {
+        leaf_shells.push_back(sb + s);
+        for (int q = 0; q < hb.shells[s].size; ++q) ao_map.push_back(hb.shells[s].bf0 + q);
+        m += hb.shells[s].size;
+      }
+      OCCXC_REQUIRE(m <= kMaxLocalAo, OCCXC_ERR_UNSUPPORTED,
+                    "a grid leaf has more than " + std::to_string(kMaxLocalAo) +
+                        " significant basis functions");
+      L.m = m;
+      L.mp = (int)round_up(m, kAoPad);
+      // padding AOs map to AO 0: their values are zero, so they contribute nothing
+      for (int q = m; q < L.mp; ++q) ao_map.push_back(0);
+      L.ao_off = ao_off;
+      ao_off += (long long)c->ncomp * L.mp * L.np;
+      const long long o = h.off[li];
+      for (int p = 0; p < L.np; ++p) {
+        const long long src = o + (p < L.n ? p : 0);
+        px.push_back(h.xyz[3 * src]);
+        py.push_back(h.xyz[3 * src + 1]);
+        pz.push_back(h.xyz[3 * src + 2]);
+        pw.push_back(p < L.n ? h.w[src] : 0.0);
+      }
+      pt_off += L.np;
+      msum += m;
+      c->flops += 4.0 * L.n * (double)m * m;
+      c->ao_write_bytes +=
+          (double)L.n * m * c->ncomp * (opt->precision == 0 ? 8.0 : 4.0) + 32.0 * L.n;
+      c->leaves.push_back(L);
     }
-    pt_off += L.np;
-    msum += m;
-    c->flops += 4.0 * L.n * (double)m * m;
-    c->ao_write_bytes += (double)L.n * m * c->ncomp * (opt->precision == 0 ? 8.0 : 4.0) + 32.0 * L.n;
-    c->leaves.push_back(L);
+    mols[mol].leaf_end = (int)c->leaves.size();
   }
   c->nleaves = (int)c->leaves.size();
   c->ptot = pt_off;
@@ -392,6 +440,8 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   c->ao_elem_size = opt->precision == 0 ? 8 : 4;
 
   // device layout
+  c->d_mols.alloc(mols.size());
+  c->d_mols.upload(mols.data(), mols.size(), c->stream);
   c->d_leaves.alloc(c->leaves.size());
   c->d_leaves.upload(c->leaves.data(), c->leaves.size(), c->stream);
   c->d_leaf_shells.alloc(leaf_shells.size());
@@ -409,11 +459,11 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   size_t free_b = 0, total_b = 0;
   OCCXC_HIP(hipMemGetInfo(&free_b, &total_b));
   const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
-  OCCXC_REQUIRE(ao_bytes + (size_t)6 * c->nbf * c->nbf * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
+  OCCXC_REQUIRE(ao_bytes + (size_t)6 * c->mtot * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
                 "resident AO values need " + std::to_string(ao_bytes >> 20) + " MiB, " +
                     std::to_string(free_b >> 20) + " MiB free");
   c->d_ao.alloc(ao_bytes);
-  const size_t nn = (size_t)c->nbf * c->nbf;
+  const size_t nn = (size_t)c->mtot;
   c->d_D.alloc(nn);
   c->d_D2.alloc(nn);
   c->d_K.alloc(nn);
@@ -421,10 +471,12 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   c->d_coef.alloc((size_t)4 * std::max<long long>(c->ptot, 1));
   c->d_part.alloc((size_t)2 * std::max(c->nleaves, 1));
   c->d_active.alloc((size_t)std::max(c->nleaves, 1));
-  c->d_result.alloc(2);
+  c->d_result.alloc((size_t)2 * nmol);
 
   BuildArgs &a = c->args;
-  a.nbf = c->nbf;
+  a.nmol = nmol;
+  a.maxnbf = c->maxnbf;
+  a.mtot = c->mtot;
   a.nleaves = c->nleaves;
   a.deriv = c->deriv;
   a.ncomp = c->ncomp;
@@ -433,6 +485,7 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   a.dthr = opt->density_threshold;
   a.ptot = c->ptot;
   a.leaves = c->d_leaves.get();
+  a.mols = c->d_mols.get();
   a.shells = c->d_shells.get();
   a.leaf_shells = c->d_leaf_shells.get();
   a.ao_map = c->d_ao_map.get();
@@ -448,7 +501,7 @@ void context_setup(occxc_context *c, const occxc_basis *basis, const occxc_grid 
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
 
-  c->plan = make_contract_plan(c->leaves, c->nbf, c->deriv, opt->precision, c->stream);
+  c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream);
   c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
                     c->d_ao_map.bytes() + c->d_leaves.bytes() + c->plan.bytes;
 
@@ -497,11 +550,10 @@ void collect_timing(occxc_context *c) {
 void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_result,
                hipStream_t s) {
   const BuildArgs &a = c->args;
-  const long long nn = (long long)c->nbf * c->nbf;
   StageTimer t{c, s, {}};
   t.run(OCCXC_STAGE_PREPARE, [&] {
-    launch_prepare_density(d_D, c->d_D2.get(), c->nbf, s);
-    launch_zero(c->d_K.get(), nn, s);
+    launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
+    launch_zero(c->d_K.get(), c->mtot, s);
   });
   if (!c->plan.use_mfma) {
     const bool fp32 = c->opt.precision != OCCXC_PREC_FP64;
@@ -512,7 +564,8 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
     t.run(OCCXC_STAGE_VXC, [&] { run_vxc(c->plan, a, c->d_ao.get(), c->opt.precision, s); });
   }
   t.run(OCCXC_STAGE_FINALIZE, [&] {
-    launch_finalize(c->d_K.get(), d_V, c->nbf, c->d_part.get(), c->nleaves, d_result, s);
+    launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
+                    d_result, s);
   });
 }
 
@@ -529,9 +582,9 @@ void occxc_default_options(occxc_options *opt) {
   opt->device = 0;
 }
 
-occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *grid,
-                                  const occxc_functional *func, const occxc_options *opt,
-                                  occxc_context **ctx) {
+occxc_status occxc_batch_create(int nmol, const occxc_basis *bases, const occxc_grid *grids,
+                                const occxc_functional *func, const occxc_options *opt,
+                                occxc_context **ctx) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "ctx: null pointer");
     *ctx = nullptr;
@@ -540,7 +593,7 @@ occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *gr
     if (opt) o = *opt;
     auto c = std::make_unique<occxc_context>();
     try {
-      context_setup(c.get(), basis, grid, func, &o);
+      context_setup(c.get(), nmol, bases, grids, func, &o);
     } catch (...) {
       if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
@@ -551,6 +604,16 @@ occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *gr
     }
     *ctx = c.release();
   });
+}
+
+occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *grid,
+                                  const occxc_functional *func, const occxc_options *opt,
+                                  occxc_context **ctx) {
+  if (!basis || !grid) {
+    set_last_error("basis/grid: null pointer");
+    return OCCXC_ERR_INVALID_ARGUMENT;
+  }
+  return occxc_batch_create(1, basis, grid, func, opt, ctx);
 }
 
 occxc_status occxc_context_destroy(occxc_context *ctx) {
@@ -577,7 +640,7 @@ occxc_status occxc_context_get_info(const occxc_context *ctx, occxc_context_info
     OCCXC_REQUIRE(ctx && info, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     info->npts = ctx->npts;
     info->nleaves = ctx->nleaves;
-    info->nbf = ctx->nbf;
+    info->nbf = ctx->nmol == 1 ? ctx->nbf : ctx->maxnbf;
     info->nshell = ctx->nsh;
     info->mean_local_nbf = ctx->mean_m;
     info->contraction_flops = ctx->flops;
@@ -597,22 +660,58 @@ occxc_status occxc_build_device(occxc_context *ctx, const double *d_D, double *d
   });
 }
 
-occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc, double *exc,
-                         double *nelec) {
+occxc_status occxc_batch_build(occxc_context *ctx, const double *const *D, double *const *Vxc,
+                               double *exc, double *nelec) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && Vxc && exc && nelec, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     DeviceGuard g(ctx->device);
-    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
-    for (size_t i = 0; i < nn; ++i)
-      OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
-    ctx->d_D.upload(D, nn, ctx->stream);
+    for (int m = 0; m < ctx->nmol; ++m) {
+      OCCXC_REQUIRE(D[m] && Vxc[m], OCCXC_ERR_INVALID_ARGUMENT, "null D or Vxc pointer");
+      const size_t nn = (size_t)ctx->mol_nbf[m] * ctx->mol_nbf[m];
+      for (size_t i = 0; i < nn; ++i)
+        OCCXC_REQUIRE(std::isfinite(D[m][i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
+    }
+    for (int m = 0; m < ctx->nmol; ++m) {
+      const size_t nn = (size_t)ctx->mol_nbf[m] * ctx->mol_nbf[m];
+      OCCXC_HIP(hipMemcpyAsync(ctx->d_D.get() + ctx->mol_off[m], D[m], nn * sizeof(double),
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
     run_build(ctx, ctx->d_D.get(), ctx->d_V.get(), ctx->d_result.get(), ctx->stream);
-    double res[2];
-    ctx->d_V.download(Vxc, nn, ctx->stream);
-    ctx->d_result.download(res, 2, ctx->stream);
+    std::vector<double> res(2 * ctx->nmol);
+    for (int m = 0; m < ctx->nmol; ++m) {
+      const size_t nn = (size_t)ctx->mol_nbf[m] * ctx->mol_nbf[m];
+      OCCXC_HIP(hipMemcpyAsync(Vxc[m], ctx->d_V.get() + ctx->mol_off[m], nn * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    }
+    ctx->d_result.download(res.data(), res.size(), ctx->stream);
     OCCXC_HIP(hipStreamSynchronize(ctx->stream));
-    *exc = res[0];
-    *nelec = res[1];
+    for (int m = 0; m < ctx->nmol; ++m) {
+      exc[m] = res[2 * m];
+      nelec[m] = res[2 * m + 1];
+    }
+  });
+}
+
+occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc, double *exc,
+                         double *nelec) {
+  if (ctx && ctx->nmol != 1) {
+    set_last_error("occxc_build on a batch context: use occxc_batch_build");
+    return OCCXC_ERR_INVALID_ARGUMENT;
+  }
+  const double *Ds[1] = {D};
+  double *Vs[1] = {Vxc};
+  return occxc_batch_build(ctx, Ds, Vs, exc, nelec);
+}
+
+occxc_status occxc_batch_layout(const occxc_context *ctx, int *nmol, int *nbf,
+                                long long *offsets) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && nmol, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    *nmol = ctx->nmol;
+    for (int m = 0; m < ctx->nmol; ++m) {
+      if (nbf) nbf[m] = ctx->mol_nbf[m];
+      if (offsets) offsets[m] = ctx->mol_off[m];
+    }
   });
 }
 

@@ -10,7 +10,7 @@
 #include <string>
 #include <utility>
 
-#include "../../include/occxc.h"
+#include "occxc.h"
 
 namespace occxc {
 

@@ -46,8 +46,8 @@ template <> struct Mfma<double> {
   }
   __device__ static int row(int g, int r) { return g + 4 * r; }
   // values at rows row(g, 0..3) of a point-fastest array
-  __device__ static d4 rows4(const double *base, int g) {
-    return d4{base[g], base[g + 4], base[g + 8], base[g + 12]};
+  template <typename S> __device__ static d4 rows4(const S *base, int g) {
+    return d4{(double)base[g], (double)base[g + 4], (double)base[g + 8], (double)base[g + 12]};
   }
 };
 template <> struct Mfma<float> {
@@ -58,14 +58,36 @@ template <> struct Mfma<float> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   __device__ static int row(int g, int r) { return 4 * g + r; }
-  __device__ static f4 rows4(const float *base, int g) {
+  template <typename S> __device__ static f4 rows4(const S *base, int g) {
     return *reinterpret_cast<const f4 *>(base + 4 * g);
   }
 };
 
+// Four consecutive stored AO values (S = storage type), converted to the
+// arithmetic type T.
+template <typename S> struct Vec4Of;
+template <> struct Vec4Of<float> { using type = f4; };
+template <> struct Vec4Of<double> { using type = d4; };
+
 template <typename T>
 __device__ __forceinline__ typename Mfma<T>::vec_t load4(const T *p) {
   return *reinterpret_cast<const typename Mfma<T>::vec_t *>(p);
+}
+template <typename T, typename S>
+__device__ __forceinline__ typename Mfma<T>::vec_t load4_as(const S *p) {
+  const auto v = *reinterpret_cast<const typename Vec4Of<S>::type *>(p);
+  return typename Mfma<T>::vec_t{(T)v[0], (T)v[1], (T)v[2], (T)v[3]};
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Give XCD x (blocks with blockIdx % 8 == x) a contiguous range of the
+// Morton-ordered leaves, so the D and K entries it touches form a compact
+// spatial block that stays in its L2.  A bijection of [0, n): correctness does
+// not depend on the dispatch order.
+__device__ __forceinline__ int xcd_leaf(int b, int n) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, i = b / kXcd, q = n / kXcd, r = n % kXcd;
+  return x * q + min(x, r) + i;
 }
 
 __device__ __forceinline__ double sum16(double v) {
@@ -105,89 +127,12 @@ __device__ double block_max_256(double v, double *scratch) {
   return r;
 }
 
-// ------------------------------------------------------------- density --
-template <typename T>
-__global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const T *__restrict__ ao,
-                                                      const T *__restrict__ D2t) {
-  using M = Mfma<T>;
-  using acc_t = typename M::acc_t;
-  __shared__ double s_q[4][kMaxLeafPts];
-  __shared__ double s_red[4];
-  const int leaf = blockIdx.x;
-  const LeafDev L = a.leaves[leaf];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int l16 = lane & 15, g = lane >> 4;
-  const T *phi = ao + L.ao_off;
-  const long long cs = (long long)L.mp * L.np;
-  const int *map = a.ao_map + L.map_off;
-  const bool deriv = a.deriv != 0;
-  const int nblk = L.np >> 5;
-  for (int blk = wave; blk < nblk; blk += 4) {
-    const int p0 = blk * 32;
-    double part[2][4][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[i][r][q] = 0.0;
-    for (int c = 0; c < L.mp; c += 32) {
-      acc_t acc[2][2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = M::zero();
-      const long long gn0 = map[c + l16], gn1 = map[c + 16 + l16];
-      for (int mc = 0; mc < L.mp; mc += 16) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          const int mu = mc + 4 * g + s;
-          const long long gm = (long long)map[mu] * a.nbf;
-          const T *row = phi + (long long)mu * L.np + p0 + l16;
-          const T a0 = row[0], a1 = row[16];
-          const T b0 = D2t[gm + gn0], b1 = D2t[gm + gn1];
-          acc[0][0] = M::mma(a0, b0, acc[0][0]);
-          acc[0][1] = M::mma(a0, b1, acc[0][1]);
-          acc[1][0] = M::mma(a1, b0, acc[1][0]);
-          acc[1][1] = M::mma(a1, b1, acc[1][1]);
-        }
-      }
-      // rho and grad rho partial sums for this lane's points and AO
-#pragma unroll
-      for (int pt = 0; pt < 2; ++pt)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-          const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16;
-          const auto f = M::rows4(phi + off, g);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) part[pt][r][0] += (double)f[r] * (double)acc[pt][nt][r];
-          if (deriv) {
-            const auto fx = M::rows4(phi + cs + off, g);
-            const auto fy = M::rows4(phi + 2 * cs + off, g);
-            const auto fz = M::rows4(phi + 3 * cs + off, g);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const double t = 2.0 * (double)acc[pt][nt][r];
-              part[pt][r][1] += (double)fx[r] * t;
-              part[pt][r][2] += (double)fy[r] * t;
-              part[pt][r][3] += (double)fz[r] * t;
-            }
-          }
-        }
-    }
-#pragma unroll
-    for (int pt = 0; pt < 2; ++pt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const double v = sum16(part[pt][r][q]);
-          if (l16 == 0) s_q[q][p0 + pt * 16 + M::row(g, r)] = v;
-        }
-  }
-  __syncthreads();
-
-  // functional, weights, Vxc coefficients, leaf partials (one thread per point)
+// Per point (one thread each, blockDim = 256): N_elec before the leaf skip,
+// the max-rho skip, the functional in FP64, Result::weight_by, the Vxc
+// coefficients c0 = w vrho / 2, c_k = 2 w vsigma d_k rho, and the leaf's
+// {Exc, N_elec} partials (dft_kernels.cpp:8-70, xc_potential_matrix.cpp:19-41).
+__device__ void density_epilogue(const BuildArgs &a, const LeafDev &L, int leaf,
+                                 double (*s_q)[kMaxLeafPts], double *s_red) {
   const int p = threadIdx.x;
   const bool live = p < L.np, real = p < L.n;
   const double rho = live ? s_q[0][p] : 0.0;
@@ -223,69 +168,567 @@ __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const T *__re
   }
 }
 
-// ----------------------------------------------------------------- Vxc --
-template <typename T>
-__global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const T *__restrict__ ao, int ld) {
+// ------------------------------------------------------------- density --
+// T = Phi D_loc for one leaf.  Each wavefront owns a 32-point block; all four
+// walk the same (nu chunk of 64, mu chunk of 32) sequence so that the D_loc
+// tile of the step is gathered once into LDS and shared.  Phi (A operand)
+// comes from the resident AO block, D (B operand) from LDS.
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__restrict__ ao,
+                                                      const T *__restrict__ D2t) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
-  constexpr int CH = 32;
-  extern __shared__ double smem[];
-  double(*s_c)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(smem);
-  T *s_A = reinterpret_cast<T *>(smem + 4 * kMaxLeafPts);
-  const int leaf = blockIdx.x;
-  if (!a.leaf_active[leaf]) return;
+  constexpr int NV = 64, KC = 32, LDD = NV + 16 / (int)sizeof(T);
+  __shared__ double s_q[4][kMaxLeafPts];
+  __shared__ double s_red[4];
+  __shared__ T s_D[KC * LDD];
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
-  const T *phi = ao + L.ao_off;
+  const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
   const int *map = a.ao_map + L.map_off;
   const bool deriv = a.deriv != 0;
+  const int nblk = L.np >> 5;
+  const int nround = (nblk + 3) >> 2;
+  for (int round = 0; round < nround; ++round) {
+    const int blk = round * 4 + wave;
+    const bool mine = blk < nblk;
+    const int p0 = (mine ? blk : 0) * 32;
+    double part[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[i][r][q] = 0.0;
+    for (int c = 0; c < L.mp; c += NV) {
+      const int ntc = min(NV, L.mp - c) >> 4;  // 16-wide nu tiles in this chunk (2 or 4)
+      acc_t acc[2][4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = M::zero();
+      for (int k = 0; k < L.mp; k += KC) {
+        // gather D_loc[k..k+32)[c..c+16*ntc) into LDS (nu fastest: coalesced)
+        for (int idx = threadIdx.x; idx < KC * NV; idx += 256) {
+          const int vl = idx & (NV - 1), ml = idx >> 6;
+          T v = static_cast<T>(0);
+          if (vl < 16 * ntc) v = D2t[MD.moff + (long long)map[k + ml] * MD.nbf + map[c + vl]];
+          s_D[ml * LDD + vl] = v;
+        }
+        __syncthreads();
+        if (mine) {
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+              const int ml = 16 * j + 4 * g + s4;
+              const S *row = phi + (long long)(k + ml) * L.np + p0 + l16;
+              const T a0 = static_cast<T>(row[0]), a1 = static_cast<T>(row[16]);
+              const T *drow = s_D + ml * LDD + l16;
+#pragma unroll
+              for (int nt = 0; nt < 4; ++nt) {
+                if (nt < ntc) {
+                  const T b = drow[16 * nt];
+                  acc[0][nt] = M::mma(a0, b, acc[0][nt]);
+                  acc[1][nt] = M::mma(a1, b, acc[1][nt]);
+                }
+              }
+            }
+        }
+        __syncthreads();
+      }
+      if (mine) {
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) {
+            if (nt >= ntc) continue;
+            const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16;
+            const auto f = M::template rows4<S>(phi + off, g);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part[pt][r][0] += (double)f[r] * (double)acc[pt][nt][r];
+            if (deriv) {
+              const auto fx = M::template rows4<S>(phi + cs + off, g);
+              const auto fy = M::template rows4<S>(phi + 2 * cs + off, g);
+              const auto fz = M::template rows4<S>(phi + 3 * cs + off, g);
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const double t = 2.0 * (double)acc[pt][nt][r];
+                part[pt][r][1] += (double)fx[r] * t;
+                part[pt][r][2] += (double)fy[r] * t;
+                part[pt][r][3] += (double)fz[r] * t;
+              }
+            }
+          }
+      }
+    }
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double v = sum16(part[pt][r][q]);
+          if (mine && l16 == 0) s_q[q][p0 + pt * 16 + M::row(g, r)] = v;
+        }
+  }
+  __syncthreads();
+
+  density_epilogue(a, L, leaf, s_q, s_red);
+}
+
+// Per-wavefront variant: each wavefront owns a 32-point block and walks all
+// (nu chunk of 32, mu) pairs on its own, with no block-wide synchronisation.
+// The leaf's AO-map column offsets (map[mu] * nbf) sit in LDS, and the
+// operands of the next 16-mu step are loaded before the MFMAs of the current
+// one are issued.
+constexpr int kMapCap = 2048;
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256) void k_density_wave(BuildArgs a, const S *__restrict__ ao,
+                                                      const T *__restrict__ D2t) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  __shared__ double s_q[4][kMaxLeafPts];
+  __shared__ double s_red[4];
+  __shared__ int s_col[kMapCap];
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const S *phi = ao + L.ao_off;
+  const T *Dm = D2t + MD.moff;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  for (int i = threadIdx.x; i < L.mp; i += blockDim.x) s_col[i] = map[i] * MD.nbf;
+  __syncthreads();
+  const int nblk = L.np >> 5;
+  for (int blk = wave; blk < nblk; blk += 4) {
+    const int p0 = blk * 32;
+    double part[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[i][r][q] = 0.0;
+    for (int c = 0; c < L.mp; c += 32) {
+      acc_t acc[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = M::zero();
+      const int gn0 = map[c + l16], gn1 = map[c + 16 + l16];
+      T ca[4][2], cb[4][2];
+      auto load = [&](int mc, T (&xa)[4][2], T (&xb)[4][2]) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int mu = mc + 4 * g + s4;
+          const S *row = phi + (long long)mu * L.np + p0 + l16;
+          const int col = s_col[mu];
+          xa[s4][0] = static_cast<T>(row[0]);
+          xa[s4][1] = static_cast<T>(row[16]);
+          xb[s4][0] = Dm[col + gn0];
+          xb[s4][1] = Dm[col + gn1];
+        }
+      };
+      load(0, ca, cb);
+      for (int mc = 0; mc < L.mp; mc += 16) {
+        T na[4][2], nb[4][2];
+        const bool more = mc + 16 < L.mp;
+        if (more) load(mc + 16, na, nb);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          acc[0][0] = M::mma(ca[s4][0], cb[s4][0], acc[0][0]);
+          acc[0][1] = M::mma(ca[s4][0], cb[s4][1], acc[0][1]);
+          acc[1][0] = M::mma(ca[s4][1], cb[s4][0], acc[1][0]);
+          acc[1][1] = M::mma(ca[s4][1], cb[s4][1], acc[1][1]);
+        }
+        if (more) {
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) {
+            ca[s4][0] = na[s4][0];
+            ca[s4][1] = na[s4][1];
+            cb[s4][0] = nb[s4][0];
+            cb[s4][1] = nb[s4][1];
+          }
+        }
+      }
+#pragma unroll
+      for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+          const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16;
+          const auto f = M::template rows4<S>(phi + off, g);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) part[pt][r][0] += (double)f[r] * (double)acc[pt][nt][r];
+          if (deriv) {
+            const auto fx = M::template rows4<S>(phi + cs + off, g);
+            const auto fy = M::template rows4<S>(phi + 2 * cs + off, g);
+            const auto fz = M::template rows4<S>(phi + 3 * cs + off, g);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const double t = 2.0 * (double)acc[pt][nt][r];
+              part[pt][r][1] += (double)fx[r] * t;
+              part[pt][r][2] += (double)fy[r] * t;
+              part[pt][r][3] += (double)fz[r] * t;
+            }
+          }
+        }
+    }
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double v = sum16(part[pt][r][q]);
+          if (l16 == 0) s_q[q][p0 + pt * 16 + M::row(g, r)] = v;
+        }
+  }
+  __syncthreads();
+  density_epilogue(a, L, leaf, s_q, s_red);
+}
+
+// ----------------------------------------------------------------- Vxc --
+// K_loc = Phi^T A for one leaf, A = c0 Phi + c . grad Phi built CH AOs at a
+// time in LDS; each wavefront owns 16-row mu tiles (Phi from the resident AO
+// block) and multiplies them with the CH-wide A chunk.  K_loc^T is scattered
+// into the global accumulator (Vxc = K + K^T afterwards).
+template <typename S, typename T, int CH>
+__global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restrict__ ao) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  using vec_t = typename M::vec_t;
+  constexpr int NT = CH / 16;
+  constexpr int PSEG = 128;  // points per register-resident Phi segment
+  extern __shared__ double smem[];
+  double(*s_c)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(smem);
+  T *s_A = reinterpret_cast<T *>(smem + 4 * kMaxLeafPts);
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  if (!a.leaf_active[leaf]) return;
+  const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const S *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  // chunk width: CH AOs for leaves of <= 128 points, half that above
+  // (the LDS buffer holds CH x (128 + pad) = CH/2 x (256 + 2 pad) elements)
+  const int chw = L.np <= 128 ? CH : CH / 2;
+  const int ld = L.np + 16 / (int)sizeof(T);
   for (int i = threadIdx.x; i < L.np; i += blockDim.x)
     for (int k = 0; k < 4; ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
   __syncthreads();
   const int ntile = L.mp >> 4;
-  for (int c = 0; c < L.mp; c += CH) {
-    for (int idx = threadIdx.x; idx < CH * L.np; idx += blockDim.x) {
+  for (int c = 0; c < L.mp; c += chw) {
+    const int ntc = min(chw, L.mp - c) >> 4;
+    const int nel = 16 * ntc * L.np;
+    // A chunk: each thread handles 4 consecutive points of a row per step
+#pragma unroll 4
+    for (int idx = 4 * threadIdx.x; idx < nel; idx += 4 * blockDim.x) {
+      const int vl = idx / L.np, p = idx - vl * L.np;
+      const long long o = (long long)(c + vl) * L.np + p;
+      const vec_t f0 = load4_as<T>(phi + o);
+      vec_t fx, fy, fz;
+      if (deriv) {
+        fx = load4_as<T>(phi + cs + o);
+        fy = load4_as<T>(phi + 2 * cs + o);
+        fz = load4_as<T>(phi + 3 * cs + o);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double v = s_c[0][p + e] * (double)f0[e];
+        if (deriv)
+          v += s_c[1][p + e] * (double)fx[e] + s_c[2][p + e] * (double)fy[e] +
+               s_c[3][p + e] * (double)fz[e];
+        s_A[vl * ld + p + e] = static_cast<T>(v);
+      }
+    }
+    __syncthreads();
+    for (int mt = wave; mt < ntile; mt += 4) {
+      acc_t acc[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j] = M::zero();
+      const S *arow = phi + (long long)(mt * 16 + l16) * L.np + 4 * g;
+      const T *brow = s_A + l16 * ld + 4 * g;
+      for (int p0 = 0; p0 < L.np; p0 += PSEG) {
+        // the wave's Phi rows for this point segment, all loads in flight at once
+        vec_t av[PSEG / 16];
+#pragma unroll
+        for (int q = 0; q < PSEG / 16; ++q)
+          if (p0 + 16 * q < L.np) av[q] = load4_as<T>(arow + p0 + 16 * q);
+#pragma unroll
+        for (int q = 0; q < PSEG / 16; ++q) {
+          if (p0 + 16 * q >= L.np) break;
+#pragma unroll
+          for (int j = 0; j < NT; ++j) {
+            if (j < ntc) {
+              const vec_t bv = load4(brow + 16 * j * ld + p0 + 16 * q);
+#pragma unroll
+              for (int s4 = 0; s4 < 4; ++s4) acc[j] = M::mma(av[q][s4], bv[s4], acc[j]);
+            }
+          }
+        }
+      }
+      // acc[j][r] = K_loc(mu = mt*16 + row(g, r), nu = c + 16 j + l16); store K^T
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int nu = c + 16 * j + l16;
+        if (j >= ntc || nu >= L.m) continue;
+        const long long gn = map[nu];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = mt * 16 + M::row(g, r);
+          if (mu < L.m)
+            atomicAdd(a.K + MD.moff + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ split-BF16 path --
+// FP32 values are split exactly into three BF16 terms x = x0 + x1 + x2
+// (8 significant bits each); FP64 values (the density) into the three leading
+// BF16 terms.  A product keeps the six cross terms with i + j <= 2 on
+// v_mfma_f32_16x16x32_bf16 (16 x 16 outputs, K = 32), smallest first, with
+// FP32 accumulation.  Operand layout: lane l supplies A[l%16][8(l/16)+i] and
+// B[8(l/16)+i][l%16], i = 0..7; accumulator register r holds
+// D[4(l/16)+r][l%16] (checked on the device).
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = static_cast<__bf16>(x);
+  const float r1 = x - static_cast<float>(h);
+  m = static_cast<__bf16>(r1);
+  l = static_cast<__bf16>(r1 - static_cast<float>(m));
+}
+__device__ __forceinline__ void split3(double x, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = static_cast<__bf16>(static_cast<float>(x));
+  const double r1 = x - static_cast<double>(static_cast<float>(h));
+  m = static_cast<__bf16>(static_cast<float>(r1));
+  const double r2 = r1 - static_cast<double>(static_cast<float>(m));
+  l = static_cast<__bf16>(static_cast<float>(r2));
+}
+__device__ __forceinline__ void split8(const float (&x)[8], bf8 (&o)[3]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    __bf16 h, m, l;
+    split3(x[i], h, m, l);
+    o[0][i] = h;
+    o[1][i] = m;
+    o[2][i] = l;
+  }
+}
+__device__ __forceinline__ f4 mma3(const bf8 (&a)[3], const bf8 (&b)[3], f4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
+  return c;
+}
+__device__ __forceinline__ bf8 lds8(const __bf16 *p) { return *reinterpret_cast<const bf8 *>(p); }
+
+// T = Phi D_loc with split-BF16 operands: D_loc tiles (nu-major, mu fastest)
+// are split from FP64 into three BF16 planes in LDS; Phi (FP32 AO block) is
+// split in registers.
+__global__ __launch_bounds__(256) void k_density_bf16(BuildArgs a, const float *__restrict__ ao) {
+  constexpr int NV = 64, KC = 32, LDK = KC + 8;
+  __shared__ double s_q[4][kMaxLeafPts];
+  __shared__ double s_red[4];
+  __shared__ __attribute__((aligned(16))) __bf16 s_D[3][NV * LDK];
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const float *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  const int nblk = L.np >> 5;
+  const int nround = (nblk + 3) >> 2;
+  for (int round = 0; round < nround; ++round) {
+    const int blk = round * 4 + wave;
+    const bool mine = blk < nblk;
+    const int p0 = (mine ? blk : 0) * 32;
+    double part[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[i][r][q] = 0.0;
+    for (int c = 0; c < L.mp; c += NV) {
+      const int ntc = min(NV, L.mp - c) >> 4;
+      f4 acc[2][4];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int k = 0; k < L.mp; k += KC) {
+        for (int idx = threadIdx.x; idx < KC * NV; idx += 256) {
+          const int vl = idx & (NV - 1), ml = idx >> 6;
+          double v = 0.0;
+          if (vl < 16 * ntc) v = a.D2[MD.moff + (long long)map[k + ml] * MD.nbf + map[c + vl]];
+          __bf16 h, m, l;
+          split3(v, h, m, l);
+          s_D[0][vl * LDK + ml] = h;
+          s_D[1][vl * LDK + ml] = m;
+          s_D[2][vl * LDK + ml] = l;
+        }
+        __syncthreads();
+        if (mine) {
+          bf8 av[2][3];
+#pragma unroll
+          for (int pt = 0; pt < 2; ++pt) {
+            float x[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+              x[i] = phi[(long long)(k + 8 * g + i) * L.np + p0 + 16 * pt + l16];
+            split8(x, av[pt]);
+          }
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) {
+            if (nt < ntc) {
+              const int o = (16 * nt + l16) * LDK + 8 * g;
+              const bf8 bv[3] = {lds8(&s_D[0][o]), lds8(&s_D[1][o]), lds8(&s_D[2][o])};
+              acc[0][nt] = mma3(av[0], bv, acc[0][nt]);
+              acc[1][nt] = mma3(av[1], bv, acc[1][nt]);
+            }
+          }
+        }
+        __syncthreads();
+      }
+      if (mine) {
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+          for (int nt = 0; nt < 4; ++nt) {
+            if (nt >= ntc) continue;
+            const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16 + 4 * g;
+            const f4 f = load4(phi + off);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) part[pt][r][0] += (double)f[r] * (double)acc[pt][nt][r];
+            if (deriv) {
+              const f4 fx = load4(phi + cs + off), fy = load4(phi + 2 * cs + off),
+                       fz = load4(phi + 3 * cs + off);
+#pragma unroll
+              for (int r = 0; r < 4; ++r) {
+                const double t = 2.0 * (double)acc[pt][nt][r];
+                part[pt][r][1] += (double)fx[r] * t;
+                part[pt][r][2] += (double)fy[r] * t;
+                part[pt][r][3] += (double)fz[r] * t;
+              }
+            }
+          }
+      }
+    }
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double v = sum16(part[pt][r][q]);
+          if (mine && l16 == 0) s_q[q][p0 + pt * 16 + 4 * g + r] = v;
+        }
+  }
+  __syncthreads();
+  density_epilogue(a, L, leaf, s_q, s_red);
+}
+
+// K_loc = Phi^T A with split-BF16 operands: the A chunk (64 AOs for leaves of
+// <= 128 points, 32 above) is split into three BF16 planes in LDS, Phi rows
+// (8 consecutive points per lane) are split in registers.
+__global__ __launch_bounds__(256) void k_vxc_bf16(BuildArgs a, const float *__restrict__ ao) {
+  constexpr int CH = 64, PADP = 8;
+  extern __shared__ double smem[];
+  double(*s_c)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(smem);
+  __bf16 *s_A = reinterpret_cast<__bf16 *>(smem + 4 * kMaxLeafPts);
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  if (!a.leaf_active[leaf]) return;
+  const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const float *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  const int chw = L.np <= 128 ? CH : CH / 2;
+  const int ld = L.np + PADP;
+  const int plane = chw * ld;
+  for (int i = threadIdx.x; i < L.np; i += blockDim.x)
+    for (int k = 0; k < 4; ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+  __syncthreads();
+  const int ntile = L.mp >> 4;
+  for (int c = 0; c < L.mp; c += chw) {
+    const int ntc = min(chw, L.mp - c) >> 4;
+    for (int idx = threadIdx.x; idx < 16 * ntc * L.np; idx += blockDim.x) {
       const int vl = idx / L.np, p = idx - vl * L.np;
       const long long o = (long long)(c + vl) * L.np + p;
       double v = s_c[0][p] * (double)phi[o];
       if (deriv)
         v += s_c[1][p] * (double)phi[cs + o] + s_c[2][p] * (double)phi[2 * cs + o] +
              s_c[3][p] * (double)phi[3 * cs + o];
-      s_A[vl * ld + p] = static_cast<T>(v);
+      __bf16 h, m, l;
+      split3(v, h, m, l);
+      s_A[vl * ld + p] = h;
+      s_A[plane + vl * ld + p] = m;
+      s_A[2 * plane + vl * ld + p] = l;
     }
     __syncthreads();
     for (int mt = wave; mt < ntile; mt += 4) {
-      acc_t acc[2] = {M::zero(), M::zero()};
-      const T *arow = phi + (long long)(mt * 16 + l16) * L.np + 4 * g;
-      const T *b0row = s_A + l16 * ld + 4 * g;
-      const T *b1row = s_A + (16 + l16) * ld + 4 * g;
-      for (int pc = 0; pc < L.np; pc += 16) {
-        const auto av = load4(arow + pc);
-        const auto b0 = load4(b0row + pc);
-        const auto b1 = load4(b1row + pc);
+      f4 acc[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          acc[0] = M::mma(av[s], b0[s], acc[0]);
-          acc[1] = M::mma(av[s], b1[s], acc[1]);
+      for (int j = 0; j < 4; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
+      const float *arow = phi + (long long)(mt * 16 + l16) * L.np + 8 * g;
+      for (int pc = 0; pc < L.np; pc += 32) {
+        const f4 x0 = load4(arow + pc), x1 = load4(arow + pc + 4);
+        const float x[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+        bf8 av[3];
+        split8(x, av);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j < ntc) {
+            const int o = (16 * j + l16) * ld + pc + 8 * g;
+            const bf8 bv[3] = {lds8(s_A + o), lds8(s_A + plane + o), lds8(s_A + 2 * plane + o)};
+            acc[j] = mma3(av, bv, acc[j]);
+          }
         }
       }
-      // acc[nt][r] = K_loc(mu = mt*16 + row(g, r), nu = c + nt*16 + l16); store K^T
 #pragma unroll
-      for (int nt = 0; nt < 2; ++nt) {
-        const int nu = c + nt * 16 + l16;
-        if (nu >= L.m) continue;
+      for (int j = 0; j < 4; ++j) {
+        const int nu = c + 16 * j + l16;
+        if (j >= ntc || nu >= L.m) continue;
         const long long gn = map[nu];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int mu = mt * 16 + M::row(g, r);
-          if (mu < L.m) atomicAdd(a.K + gn + (long long)map[mu] * a.nbf, (double)acc[nt][r]);
+          const int mu = mt * 16 + 4 * g + r;
+          if (mu < L.m)
+            atomicAdd(a.K + MD.moff + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
         }
       }
     }
     __syncthreads();
   }
+}
+
+static size_t vxc_bf16_smem() {
+  return sizeof(double) * 4 * kMaxLeafPts + sizeof(__bf16) * 3 * 64 * (128 + 8);
 }
 
 __global__ void k_to_float(const double *__restrict__ x, float *__restrict__ y, long long n) {
@@ -301,29 +744,42 @@ void check_launch(const char *what) {
 
 } // namespace
 
-static size_t vxc_smem(const ContractPlan &p, size_t elem, int *ld) {
-  const int pad = (int)(16 / elem);
-  *ld = p.max_np + pad;
-  return sizeof(double) * 4 * kMaxLeafPts + elem * 32 * (size_t)(*ld);
+// LDS of k_vxc_mfma<T, 64>: coefficients + a 64 x (128 + pad) A chunk
+static size_t vxc_smem(size_t elem) {
+  const size_t pad = 16 / elem;
+  return sizeof(double) * 4 * kMaxLeafPts + elem * 64 * (128 + pad);
 }
 
-ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, int nbf, int deriv,
+ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
                                 int precision, hipStream_t s) {
   (void)deriv;
   (void)s;
   ContractPlan p;
   p.use_mfma = std::getenv("OCCXC_REFERENCE_KERNELS") == nullptr;
-  for (const auto &L : leaves) p.max_np = std::max(p.max_np, L.np);
-  if (precision != 0) {
-    p.d2f.alloc((size_t)std::max(nbf, 1) * std::max(nbf, 1));
+  for (const auto &L : leaves) {
+    p.max_np = std::max(p.max_np, L.np);
+    p.max_mp = std::max(p.max_mp, L.mp);
+  }
+  // density kernel: per-wavefront (0) or LDS-shared D tiles (1); the
+  // per-wavefront kernel keeps the leaf's AO map in LDS and needs mp <= 2048
+  const char *dv = std::getenv("OCCXC_DENSITY_VARIANT");
+  p.density_variant = dv ? std::atoi(dv) : 0;
+  if (p.max_mp > kMapCap) p.density_variant = 1;
+  if (precision == 1) {
+    p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
     p.bytes += p.d2f.bytes();
   }
-  int ld = 0;
-  const size_t sm64 = vxc_smem(p, sizeof(double), &ld), sm32 = vxc_smem(p, sizeof(float), &ld);
-  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<double>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm64));
-  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<float>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm32));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<double, double, 64>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_smem(sizeof(double))));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<float, double, 64>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_smem(sizeof(double))));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<float, float, 64>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_smem(sizeof(float))));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_bf16),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)vxc_bf16_smem()));
   return p;
 }
 
@@ -331,14 +787,32 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
                  hipStream_t s) {
   if (a.nleaves == 0) return;
   if (precision == 0) {
-    k_density_mfma<double><<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
+    if (plan.density_variant == 0)
+      k_density_wave<double, double>
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
+    else
+      k_density_mfma<double, double>
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
+  } else if (precision == 3) {
+    if (plan.density_variant == 0)
+      k_density_wave<float, double>
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2);
+    else
+      k_density_mfma<float, double>
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2);
+  } else if (precision == 2) {
+    k_density_bf16<<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao));
   } else {
-    const long long nn = (long long)a.nbf * a.nbf;
-    k_to_float<<<(int)std::min<long long>((nn + 255) / 256, 4096), 256, 0, s>>>(
-        a.D2, plan.d2f.get(), nn);
+    const long long nn = a.mtot;
+    k_to_float<<<(int)std::max<long long>(1, std::min<long long>((nn + 255) / 256, 4096)), 256,
+                 0, s>>>(a.D2, plan.d2f.get(), nn);
     check_launch("k_to_float");
-    k_density_mfma<float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
-                                                    plan.d2f.get());
+    if (plan.density_variant == 0)
+      k_density_wave<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
+                                                             plan.d2f.get());
+    else
+      k_density_mfma<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
+                                                             plan.d2f.get());
   }
   check_launch("k_density_mfma");
 }
@@ -346,14 +820,18 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s) {
   if (a.nleaves == 0) return;
-  int ld = 0;
-  if (precision == 0) {
-    const size_t sm = vxc_smem(plan, sizeof(double), &ld);
-    k_vxc_mfma<double><<<a.nleaves, 256, sm, s>>>(a, static_cast<const double *>(ao), ld);
-  } else {
-    const size_t sm = vxc_smem(plan, sizeof(float), &ld);
-    k_vxc_mfma<float><<<a.nleaves, 256, sm, s>>>(a, static_cast<const float *>(ao), ld);
-  }
+  (void)plan;
+  if (precision == 0)
+    k_vxc_mfma<double, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
+        a, static_cast<const double *>(ao));
+  else if (precision == 3)
+    k_vxc_mfma<float, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
+        a, static_cast<const float *>(ao));
+  else if (precision == 2)
+    k_vxc_bf16<<<a.nleaves, 256, vxc_bf16_smem(), s>>>(a, static_cast<const float *>(ao));
+  else
+    k_vxc_mfma<float, float, 64><<<a.nleaves, 256, vxc_smem(sizeof(float)), s>>>(
+        a, static_cast<const float *>(ao));
   check_launch("k_vxc_mfma");
 }
 

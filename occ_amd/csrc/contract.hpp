@@ -14,12 +14,14 @@ namespace occxc {
 
 struct ContractPlan {
   bool use_mfma = false;   // false: the FP64 reference kernels of kernels.hip
-  int max_np = 0;          // largest padded leaf
+  int max_np = 0;          // largest padded leaf (points)
+  int max_mp = 0;          // largest padded leaf (AOs)
+  int density_variant = 0; // 0 per-wavefront, 1 LDS-shared D tiles
   DevBuf<float> d2f;       // FP32 copy of the doubled density (FP32 modes)
   size_t bytes = 0;
 };
 
-ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, int nbf, int deriv,
+ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
                                 int precision, hipStream_t s);
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s);

@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void k_density_functional(BuildArgs a,
   __shared__ double s_red[4];
   const int leaf = blockIdx.x;
   const LeafDev L = a.leaves[leaf];
+  const MolDev M = a.mols[L.mol];
   for (int i = threadIdx.x; i < L.m; i += blockDim.x) s_map[i] = a.ao_map[L.map_off + i];
   __syncthreads();
   const int p = threadIdx.x;
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(256) void k_density_functional(BuildArgs a,
   double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
   if (live) {
     for (int nu = 0; nu < L.m; ++nu) {
-      const double *Dcol = a.D2 + (long long)s_map[nu] * a.nbf;
+      const double *Dcol = a.D2 + M.moff + (long long)s_map[nu] * M.nbf;
       double t = 0.0;
       for (int mu = 0; mu < L.m; ++mu) t += Dcol[s_map[mu]] * ld(phi + (long long)mu * L.np + p);
       rho += ld(phi + (long long)nu * L.np + p) * t;
@@ -287,6 +288,7 @@ __global__ __launch_bounds__(256) void k_vxc(BuildArgs a, const T *__restrict__ 
   const int leaf = blockIdx.x;
   if (!a.leaf_active[leaf]) return;
   const LeafDev L = a.leaves[leaf];
+  const MolDev M = a.mols[L.mol];
   for (int i = threadIdx.x; i < L.m; i += blockDim.x) s_map[i] = a.ao_map[L.map_off + i];
   for (int i = threadIdx.x; i < L.np; i += blockDim.x)
     for (int c = 0; c < 4; ++c) s_c[c][i] = a.coef[c * a.ptot + L.pt_off + i];
@@ -307,7 +309,7 @@ __global__ __launch_bounds__(256) void k_vxc(BuildArgs a, const T *__restrict__ 
     } else {
       for (int p = 0; p < L.np; ++p) s += ld(fm + p) * s_c[0][p] * ld(fn + p);
     }
-    atomicAdd(a.K + s_map[mu] + (long long)s_map[nu] * a.nbf, s);
+    atomicAdd(a.K + M.moff + s_map[mu] + (long long)s_map[nu] * M.nbf, s);
   }
 }
 
@@ -317,39 +319,51 @@ __global__ void k_zero(double *__restrict__ p, long long n) {
     p[k] = 0.0;
 }
 
-__global__ void k_prepare_density(const double *__restrict__ D, double *__restrict__ D2, int nbf) {
+// blockIdx.y = molecule; D2 = 2 D with the upper triangle mirrored
+__global__ void k_prepare_density(const double *__restrict__ D, double *__restrict__ D2,
+                                  const MolDev *__restrict__ mols) {
+  const MolDev M = mols[blockIdx.y];
+  const int nbf = M.nbf;
   const long long n = (long long)nbf * nbf;
+  const double *Dm = D + M.moff;
   for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(k % nbf), j = (int)(k / nbf);
     const int r = i < j ? i : j, c = i < j ? j : i;
-    D2[k] = 2.0 * D[r + (long long)c * nbf];
+    D2[M.moff + k] = 2.0 * Dm[r + (long long)c * nbf];
   }
 }
 
-__global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V, int nbf) {
+// blockIdx.y = molecule; Vxc = K + K^T
+__global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V,
+                                  const MolDev *__restrict__ mols) {
+  const MolDev M = mols[blockIdx.y];
+  const int nbf = M.nbf;
   const long long n = (long long)nbf * nbf;
+  const double *Km = K + M.moff;
   for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(k % nbf), j = (int)(k / nbf);
-    V[k] = K[k] + K[j + (long long)i * nbf];
+    V[M.moff + k] = Km[k] + Km[j + (long long)i * nbf];
   }
 }
 
+// one workgroup per molecule: {Exc, N_elec} over its leaves in a fixed order
 __global__ __launch_bounds__(256) void k_finalize_scalars(const double *__restrict__ part,
-                                                          int nleaves,
+                                                          const MolDev *__restrict__ mols,
                                                           double *__restrict__ result) {
   __shared__ double s_red[4];
+  const MolDev M = mols[blockIdx.x];
   double e = 0.0, n = 0.0;
-  for (int i = threadIdx.x; i < nleaves; i += 256) {
+  for (int i = M.leaf_begin + threadIdx.x; i < M.leaf_end; i += 256) {
     e += part[2 * i];
     n += part[2 * i + 1];
   }
   e = block_sum_256(e, s_red);
   n = block_sum_256(n, s_red);
   if (threadIdx.x == 0) {
-    result[0] = e;
-    result[1] = n;
+    result[2 * blockIdx.x] = e;
+    result[2 * blockIdx.x + 1] = n;
   }
 }
 
@@ -533,8 +547,16 @@ void launch_zero(double *p, long long n, hipStream_t s) {
   check_launch();
 }
 
-void launch_prepare_density(const double *D, double *D2, int nbf, hipStream_t s) {
-  k_prepare_density<<<grid_for((long long)nbf * nbf, 256), 256, 0, s>>>(D, D2, nbf);
+static dim3 mol_grid(int maxnbf, int nmol) {
+  long long gx = ((long long)maxnbf * maxnbf + 255) / 256;
+  gx = gx < 1 ? 1 : (gx > 1024 ? 1024 : gx);
+  return dim3((unsigned)gx, (unsigned)nmol);
+}
+
+void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int nmol,
+                            int maxnbf, hipStream_t s) {
+  if (nmol == 0) return;
+  k_prepare_density<<<mol_grid(maxnbf, nmol), 256, 0, s>>>(D, D2, mols);
   check_launch();
 }
 
@@ -565,11 +587,12 @@ void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s) {
   check_launch();
 }
 
-void launch_finalize(const double *K, double *Vxc, int nbf, const double *leaf_part,
-                     int nleaves, double *result, hipStream_t s) {
-  k_finalize_matrix<<<grid_for((long long)nbf * nbf, 256), 256, 0, s>>>(K, Vxc, nbf);
+void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol, int maxnbf,
+                     const double *leaf_part, double *result, hipStream_t s) {
+  if (nmol == 0) return;
+  k_finalize_matrix<<<mol_grid(maxnbf, nmol), 256, 0, s>>>(K, Vxc, mols);
   check_launch();
-  k_finalize_scalars<<<1, 256, 0, s>>>(leaf_part, nleaves, result);
+  k_finalize_scalars<<<nmol, 256, 0, s>>>(leaf_part, mols, result);
   check_launch();
 }
 

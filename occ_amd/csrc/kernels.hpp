@@ -32,12 +32,26 @@ struct LeafDev {
   int mp;       // padded local AOs (multiple of kAoPad)
   int sh_off;   // offset into leaf_shells
   int nsig;     // significant shells
-  int map_off;  // offset into ao_map (mp entries, -1 for padding)
+  int map_off;  // offset into ao_map (mp entries; padding entries map to AO 0)
+  int mol;      // molecule of this leaf (index into BuildArgs::mols)
   long long ao_off;  // element offset of this leaf's AO block [ncomp][mp][np]
 };
 
-struct BuildArgs {
+// One molecule of a context: its nbf x nbf blocks (D2, K, Vxc) start at moff
+// in the concatenated per-context arrays; its active leaves are
+// [leaf_begin, leaf_end).
+struct MolDev {
+  long long moff;
   int nbf;
+  int leaf_begin;
+  int leaf_end;
+  int pad_;
+};
+
+struct BuildArgs {
+  int nmol;
+  int maxnbf;     // largest nbf over the molecules
+  long long mtot; // sum of nbf^2 over the molecules
   int nleaves;
   int deriv;
   int ncomp;
@@ -46,14 +60,15 @@ struct BuildArgs {
   double dthr;
   long long ptot;  // total padded points
   const LeafDev *leaves;
+  const MolDev *mols;
   const ShellDev *shells;
   const int *leaf_shells;
   const int *ao_map;
   const double *exps;
   const double *coefs;
   const double *px, *py, *pz, *pw;
-  const double *D2;     // doubled, symmetrised density (nbf x nbf)
-  double *K;            // accumulated phi^T A (nbf x nbf)
+  const double *D2;     // doubled, symmetrised densities (concatenated nbf x nbf)
+  double *K;            // accumulated phi^T A (concatenated nbf x nbf)
   double *coef;         // [4][ptot] per-point Vxc coefficients
   double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
   int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
@@ -62,13 +77,14 @@ struct BuildArgs {
 // host-callable launchers (kernels.hip)
 void upload_sph_table();
 void launch_zero(double *p, long long n, hipStream_t s);
-void launch_prepare_density(const double *D, double *D2, int nbf, hipStream_t s);
+void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int nmol,
+                            int maxnbf, hipStream_t s);
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s);
 void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32,
                                hipStream_t s);
 void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s);
-void launch_finalize(const double *K, double *Vxc, int nbf,
-                     const double *leaf_part, int nleaves, double *result,
+void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol,
+                     int maxnbf, const double *leaf_part, double *result,
                      hipStream_t s);
 void launch_eval_basis_flat(const ShellDev *shells, int nshell,
                             const double *exps, const double *coefs,

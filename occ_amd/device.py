@@ -63,5 +63,11 @@ class DeviceArray:
             pass
 
 
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy (hipMemcpyDeviceToDevice)."""
+    _check(hip().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, 3),
+           "hipMemcpy D2D")
+
+
 def synchronize():
     _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")

@@ -24,7 +24,7 @@ METHODS = {"svwn5": ["lda_x", "lda_c_vwn"], "vwn5": ["lda_c_vwn"],
            "blyp": ["gga_x_b88", "gga_c_lyp"], "bpbe": ["gga_x_b88", "gga_c_pbe"],
            "slater": ["lda_x"]}
 
-PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32cache": 3}
 
 
 def functional_ids(method):
@@ -133,6 +133,77 @@ class XCContext:
             self.close()
         except Exception:
             pass
+
+
+class XCBatch:
+    """Many molecules in one occxc context: one launch per stage for all."""
+
+    def __init__(self, bases, grids, funcs, precision="fp64", screen=True, device=0):
+        L = _lib.lib()
+        n = len(bases)
+        if n != len(grids) or n == 0:
+            raise ValueError("need one (points, weights) grid per basis")
+        self._keep = []
+        cb = (_lib.CBasis * n)()
+        cg = (_lib.CGrid * n)()
+        for i, (b, (pts, w)) in enumerate(zip(bases, grids)):
+            st, keep = b.c_struct()
+            pts = _lib.f64(pts).reshape(-1)
+            w = _lib.f64(w)
+            self._keep += [keep, pts, w]
+            cb[i] = st
+            cg[i] = _lib.CGrid(len(w), _lib.dptr(pts), _lib.dptr(w))
+        self._cb, self._cg = cb, cg
+        self.nbf = [b.nbf for b in bases]
+        self._ids = _lib.i32(funcs)
+        self._func = _lib.CFunctional(len(self._ids), _lib.iptr(self._ids))
+        o = default_options()
+        o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        o.screen = int(bool(screen))
+        o.device = device
+        h = ctypes.c_void_p()
+        _lib.check(L.occxc_batch_create(n, cb, cg, ctypes.byref(self._func), ctypes.byref(o),
+                                        ctypes.byref(h)))
+        self._h = h
+        nm = np.zeros(1, dtype=np.int32)
+        nbf = np.zeros(n, dtype=np.int32)
+        off = np.zeros(n, dtype=np.int64)
+        _lib.check(L.occxc_batch_layout(h, _lib.iptr(nm), _lib.iptr(nbf), _lib.llptr(off)))
+        self.offsets = off
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build(self, Ds):
+        n = len(self.nbf)
+        Ds = [np.asfortranarray(D, dtype=np.float64) for D in Ds]
+        Vs = [np.zeros((k, k), order="F") for k in self.nbf]
+        dp = (_lib._dp * n)(*[D.ctypes.data_as(_lib._dp) for D in Ds])
+        vp = (_lib._dp * n)(*[V.ctypes.data_as(_lib._dp) for V in Vs])
+        exc = np.zeros(n)
+        nel = np.zeros(n)
+        _lib.check(_lib.lib().occxc_batch_build(self._h, dp, vp, _lib.dptr(exc),
+                                                _lib.dptr(nel)))
+        return Vs, exc, nel
+
+    def build_device(self, d_D, d_V, d_result, stream=None):
+        _lib.check(_lib.lib().occxc_build_device(self._h, ctypes.c_void_p(d_D),
+                                                 ctypes.c_void_p(d_V),
+                                                 ctypes.c_void_p(d_result),
+                                                 ctypes.c_void_p(stream or 0)))
+
+    def info(self):
+        i = _lib.CContextInfo()
+        _lib.check(_lib.lib().occxc_context_get_info(self._h, ctypes.byref(i)))
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    set_timing = XCContext.set_timing
+    timing = XCContext.timing
+    STAGES = XCContext.STAGES
+    synchronize = XCContext.synchronize
+    close = XCContext.close
+    __del__ = XCContext.__del__
 
 
 class DFT:

@@ -51,14 +51,20 @@ def test_context_create_rejects_bad_input_without_gpu():
     pts = np.zeros(3)
     w = np.ones(1)
     g = _lib.CGrid(1, _lib.dptr(pts), _lib.dptr(w))
-    st = L.occxc_context_create(None, ctypes.byref(g), ctypes.byref(f), None, ctypes.byref(h))
+    from occ_amd import AOBasis, molecules
+    b, keep = AOBasis.load(molecules.water(), "sto-3g").c_struct()
+    st = L.occxc_context_create(ctypes.byref(b), ctypes.byref(g), ctypes.byref(f), None,
+                                ctypes.byref(h))
     assert st == 2  # unknown functional id -> UNSUPPORTED
     assert b"not found" in L.occxc_last_error_message()
     o = _lib.COptions()
     L.occxc_default_options(ctypes.byref(o))
     o.precision = 7
-    st = L.occxc_context_create(None, ctypes.byref(g), ctypes.byref(f), ctypes.byref(o),
-                                ctypes.byref(h))
+    st = L.occxc_context_create(ctypes.byref(b), ctypes.byref(g), ctypes.byref(f),
+                                ctypes.byref(o), ctypes.byref(h))
+    assert st == 1
+    st = L.occxc_batch_create(0, ctypes.byref(b), ctypes.byref(g), ctypes.byref(f), None,
+                              ctypes.byref(h))
     assert st == 1
 
 
