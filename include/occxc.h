@@ -150,6 +150,14 @@ occxc_status occxc_batch_create(int nmol, const occxc_basis *bases,
 occxc_status occxc_batch_build(occxc_context *ctx, const double *const *D,
                                double *const *Vxc, double *exc, double *nelec);
 
+/* Device bytes a batch context over these molecules would hold (AO values,
+ * matrices, per-point data), from the same hierarchy and screening as
+ * occxc_batch_create but without the large allocations: for packing many
+ * molecules into contexts that fit device memory. */
+occxc_status occxc_batch_estimate_bytes(int nmol, const occxc_basis *bases,
+                                        const occxc_grid *grids, const occxc_functional *func,
+                                        const occxc_options *opt, double *bytes);
+
 /* Molecule count, per-molecule nbf and the element offsets of each
  * molecule's nbf x nbf block in the packed device layout used by
  * occxc_build_device (D and Vxc packed back to back; d_result holds
@@ -181,6 +189,62 @@ enum {
 occxc_status occxc_context_set_timing(occxc_context *ctx, int enable);
 occxc_status occxc_context_get_timing(occxc_context *ctx, double *ms,
                                       long long *counts);
+
+/* Replay builds from a captured HIP graph.  When enabled, the first
+ * occxc_build_device (or occxc_batch_build) on a given (D, Vxc, result,
+ * stream) captures the build's kernel sequence once; later builds on the same
+ * pointers and stream launch the graph (one launch instead of one per
+ * kernel).  Other pointers re-capture.  Stage timing is not recorded for
+ * graph launches.  The graph holds no memset nodes: every buffer the build
+ * clears is cleared by a kernel.  Replaces OCC's per-call loop set-up in
+ * compute_K_dft (include/occ/dft/dft.h:190-388), which the reference redoes on
+ * every SCF iteration. */
+occxc_status occxc_context_set_graph(occxc_context *ctx, int enable);
+
+/* --- serving: XC builds from many concurrent SCFs on one GPU ---
+ *
+ * No counterpart in OCC, which runs one SCF per process and calls
+ * DFT::compute_vxc once per iteration (scf_impl.h:539, dft.h:395-429).  A
+ * server keeps each admitted SCF's context (grid leaves, AO values) resident
+ * between its builds, admits SCFs by device-memory footprint against a budget
+ * (evicting idle ones, least recently used first, when the next SCF does not
+ * fit), and runs builds of different SCFs concurrently on `slots` HIP
+ * streams; builds of one SCF run in submission order.  Every function is
+ * thread-safe. */
+typedef struct occxc_server occxc_server;
+
+typedef struct {
+  int device;
+  int slots;             /* builds in flight at once, 1..64 */
+  double memory_budget;  /* device bytes for resident SCFs; 0: 90% of free memory */
+} occxc_server_options;
+
+typedef struct {
+  long long scfs_opened, scfs_admitted, scfs_closed, scfs_resident, scfs_waiting;
+  long long admissions_deferred, evictions;
+  long long builds_submitted, builds_completed, builds_failed;
+  double resident_bytes, budget_bytes;
+} occxc_server_stats;
+
+void occxc_server_default_options(occxc_server_options *opt);
+occxc_status occxc_server_create(const occxc_server_options *opt, occxc_server **srv);
+/* Runs every pending build, frees every SCF, then stops the workers. */
+occxc_status occxc_server_destroy(occxc_server *srv);
+/* Registers an SCF (inputs copied; opt->device is the server's). */
+occxc_status occxc_server_open(occxc_server *srv, const occxc_basis *basis,
+                               const occxc_grid *grid, const occxc_functional *func,
+                               const occxc_options *opt, long long *scf);
+/* Queues one build: D (nbf x nbf, column-major, as occxc_build) is copied at
+ * once; Vxc, exc and nelec are written before the ticket completes. */
+occxc_status occxc_server_submit(occxc_server *srv, long long scf, const double *D,
+                                 double *Vxc, double *exc, double *nelec, long long *ticket);
+/* Waits for a ticket; returns the build's status and its latency from submit
+ * to completion.  A ticket is waited for once. */
+occxc_status occxc_server_wait(occxc_server *srv, long long ticket, double *latency_ms);
+occxc_status occxc_server_poll(occxc_server *srv, long long ticket, int *done);
+/* The SCF's pending builds still run; then its context is freed. */
+occxc_status occxc_server_close(occxc_server *srv, long long scf);
+occxc_status occxc_server_get_stats(occxc_server *srv, occxc_server_stats *stats);
 
 /* --- stage-level entry points (each mirrors one reference function) --- */
 

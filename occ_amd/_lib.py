@@ -53,6 +53,19 @@ class CContextInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_double)]
 
 
+class CServerOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("slots", ctypes.c_int),
+                ("memory_budget", ctypes.c_double)]
+
+
+class CServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in (
+        "scfs_opened", "scfs_admitted", "scfs_closed", "scfs_resident", "scfs_waiting",
+        "admissions_deferred", "evictions", "builds_submitted", "builds_completed",
+        "builds_failed")] + [("resident_bytes", ctypes.c_double),
+                             ("budget_bytes", ctypes.c_double)]
+
+
 # every symbol declared in include/occxc.h: name -> (restype, argtypes)
 SIGNATURES = {
     "occxc_default_options": (None, [ctypes.POINTER(COptions)]),
@@ -70,11 +83,16 @@ SIGNATURES = {
     "occxc_batch_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_dp),
                                          ctypes.POINTER(_dp), _dp, _dp]),
     "occxc_batch_layout": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _llp]),
+    "occxc_batch_estimate_bytes": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CBasis),
+                                                  ctypes.POINTER(CGrid),
+                                                  ctypes.POINTER(CFunctional),
+                                                  ctypes.POINTER(COptions), _dp]),
     "occxc_build_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),
     "occxc_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "occxc_context_evaluate_ao": (ctypes.c_int, [ctypes.c_void_p]),
     "occxc_context_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "occxc_context_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "occxc_context_get_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, _llp]),
     "occxc_evaluate_basis": (ctypes.c_int, [ctypes.POINTER(CBasis), ctypes.c_longlong, _dp,
                                             ctypes.c_int, _dp, ctypes.c_int]),
@@ -88,6 +106,19 @@ SIGNATURES = {
                                          _ip]),
     "occxc_becke_partition": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int,
                                              ctypes.c_longlong, _dp, _ip, _dp, ctypes.c_int]),
+    "occxc_server_default_options": (None, [ctypes.POINTER(CServerOptions)]),
+    "occxc_server_create": (ctypes.c_int, [ctypes.POINTER(CServerOptions),
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "occxc_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "occxc_server_open": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CBasis),
+                                         ctypes.POINTER(CGrid), ctypes.POINTER(CFunctional),
+                                         ctypes.POINTER(COptions), _llp]),
+    "occxc_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, _dp,
+                                           _dp, _llp]),
+    "occxc_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp]),
+    "occxc_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _ip]),
+    "occxc_server_close": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "occxc_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CServerStats)]),
     "occxc_last_error_message": (ctypes.c_char_p, []),
     "occxc_version": (ctypes.c_char_p, []),
 }

@@ -17,12 +17,14 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "occxc.h"
 #include "common.hpp"
+#include "context.hpp"
 #include "contract.hpp"
 #include "kernels.hpp"
 #include "xc_functionals.hpp"
@@ -60,13 +62,13 @@ static double common_fac(int l, int pure) {
   return pure ? std::sqrt((2.0 * l + 1.0) / (4.0 * 3.14159265358979323846)) : 1.0;
 }
 
-static bool g_table_ready[64] = {false};
+// The spherical-transform table in constant memory, uploaded once per device
+// (contexts may be created from several host threads at once).
+static std::once_flag g_table_once[64];
 
 static void ensure_device_tables(int device) {
-  if (device >= 0 && device < 64 && !g_table_ready[device]) {
-    upload_sph_table();
-    g_table_ready[device] = true;
-  }
+  OCCXC_REQUIRE(device >= 0 && device < 64, OCCXC_ERR_INVALID_ARGUMENT, "device ordinal");
+  std::call_once(g_table_once[device], [] { upload_sph_table(); });
 }
 
 struct HostBasis {
@@ -206,62 +208,11 @@ static long long round_up(long long v, long long m) { return (v + m - 1) / m * m
 
 using namespace occxc;
 
-struct occxc_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  occxc_options opt{};
-  int nbf = 0, nsh = 0, deriv = 0, ncomp = 1, nfunc = 0;
-  int nmol = 0, maxnbf = 0;
-  long long mtot = 0;
-  std::vector<int> mol_nbf;
-  std::vector<long long> mol_off;
-  int funcs[kMaxFunc] = {0};
-  long long npts = 0, ptot = 0;
-  int nleaves = 0;
-  double mean_m = 0.0, flops = 0.0, ao_write_bytes = 0.0;
-  size_t ao_elems = 0, ao_elem_size = 8;
-  std::vector<LeafDev> leaves;
-  DevBuf<ShellDev> d_shells;
-  DevBuf<double> d_exps, d_coefs;
-  DevBuf<LeafDev> d_leaves;
-  DevBuf<MolDev> d_mols;
-  DevBuf<int> d_leaf_shells, d_ao_map, d_active;
-  DevBuf<double> d_px, d_py, d_pz, d_pw;
-  DevBuf<char> d_ao;
-  DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
-  ContractPlan plan;
-  BuildArgs args{};
-  // stage timing: pending event sets, summed on read-out
-  bool timing = false;
-  std::vector<std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>> pending;
-  double stage_ms[OCCXC_NSTAGES] = {0};
-  long long stage_n[OCCXC_NSTAGES] = {0};
-  bool ao_ready = false;
-  size_t device_bytes = 0;
-};
-
-namespace {
-
-template <typename F> occxc_status guarded(F &&f) {
-  try {
-    clear_last_error();
-    f();
-    return OCCXC_OK;
-  } catch (const Error &e) {
-    set_last_error(e.what());
-    return e.status;
-  } catch (const std::bad_alloc &) {
-    set_last_error("host allocation failed");
-    return OCCXC_ERR_OUT_OF_MEMORY;
-  } catch (const std::exception &e) {
-    set_last_error(e.what());
-    return OCCXC_ERR_INTERNAL;
-  }
-}
+namespace occxc {
 
 void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                    const occxc_grid *grids, const occxc_functional *func,
-                   const occxc_options *opt) {
+                   const occxc_options *opt, size_t byte_limit) {
   c->opt = *opt;
   OCCXC_REQUIRE(nmol >= 1 && bases && grids, OCCXC_ERR_INVALID_ARGUMENT,
                 "need at least one molecule (basis and grid)");
@@ -456,9 +407,17 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   c->d_pz.upload(pz.data(), pz.size(), c->stream);
   c->d_pw.alloc(pw.size());
   c->d_pw.upload(pw.data(), pw.size(), c->stream);
+  const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
+  // device memory the context will hold: AO values, the nbf^2 matrices
+  // (D, D2, K, V and the FP32 copies), per-point data and the leaf tables
+  const size_t planned = ao_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
+                         c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes();
+  c->device_bytes = planned;
+  OCCXC_REQUIRE(byte_limit == 0 || planned <= byte_limit, OCCXC_ERR_OUT_OF_MEMORY,
+                "context needs " + std::to_string(planned >> 20) + " MiB, limit " +
+                    std::to_string(byte_limit >> 20) + " MiB");
   size_t free_b = 0, total_b = 0;
   OCCXC_HIP(hipMemGetInfo(&free_b, &total_b));
-  const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
   OCCXC_REQUIRE(ao_bytes + (size_t)6 * c->mtot * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
                 "resident AO values need " + std::to_string(ao_bytes >> 20) + " MiB, " +
                     std::to_string(free_b >> 20) + " MiB free");
@@ -481,6 +440,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.deriv = c->deriv;
   a.ncomp = c->ncomp;
   a.nfunc = c->nfunc;
+  a.lmax = 0;
+  for (const auto &sh : shells) a.lmax = std::max(a.lmax, sh.l);
   for (int f = 0; f < kMaxFunc; ++f) a.func_ids[f] = c->funcs[f];
   a.dthr = opt->density_threshold;
   a.ptot = c->ptot;
@@ -503,7 +464,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
 
   c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream);
   c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
-                    c->d_ao_map.bytes() + c->d_leaves.bytes() + c->plan.bytes;
+                    c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
+                    c->plan.bytes;
 
   launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
   OCCXC_HIP(hipStreamSynchronize(c->stream));
@@ -569,7 +531,61 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
   });
 }
 
-} // namespace
+void drop_graph(occxc_context *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  c->graph_exec = nullptr;
+  for (auto &k : c->graph_key) k = nullptr;
+}
+
+// One build on stream s: directly, or through the context's captured graph.
+void launch_build(occxc_context *c, const double *d_D, double *d_V, double *d_result,
+                  hipStream_t s) {
+  if (!c->use_graph) {
+    run_build(c, d_D, d_V, d_result, s);
+    return;
+  }
+  const void *key[4] = {d_D, d_V, d_result, s};
+  if (!c->graph_exec || !std::equal(key, key + 4, c->graph_key)) {
+    drop_graph(c);
+    const bool timing = c->timing;
+    c->timing = false;
+    hipGraph_t g = nullptr;
+    OCCXC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      run_build(c, d_D, d_V, d_result, s);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      c->timing = timing;
+      throw;
+    }
+    c->timing = timing;
+    OCCXC_HIP(hipStreamEndCapture(s, &g));
+    const hipError_t e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    OCCXC_HIP(e);
+    std::copy(key, key + 4, c->graph_key);
+  }
+  OCCXC_HIP(hipGraphLaunch(c->graph_exec, s));
+}
+
+void context_destroy(occxc_context *c) {
+  if (!c) return;
+  DeviceGuard g(c->device);
+  if (c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
+  c->stream = nullptr;
+  try {
+    collect_timing(c);
+  } catch (...) {
+  }
+  drop_graph(c);
+  delete c;
+}
+
+} // namespace occxc
 
 extern "C" {
 
@@ -606,6 +622,29 @@ occxc_status occxc_batch_create(int nmol, const occxc_basis *bases, const occxc_
   });
 }
 
+occxc_status occxc_batch_estimate_bytes(int nmol, const occxc_basis *bases,
+                                        const occxc_grid *grids, const occxc_functional *func,
+                                        const occxc_options *opt, double *bytes) {
+  return guarded([&] {
+    OCCXC_REQUIRE(bytes, OCCXC_ERR_INVALID_ARGUMENT, "bytes: null pointer");
+    occxc_options o;
+    occxc_default_options(&o);
+    if (opt) o = *opt;
+    auto c = std::make_unique<occxc_context>();
+    try {
+      context_setup(c.get(), nmol, bases, grids, func, &o, 1);
+    } catch (const Error &e) {
+      if (c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->stream = nullptr;
+      }
+      if (e.status != OCCXC_ERR_OUT_OF_MEMORY || c->device_bytes == 0) throw;
+    }
+    *bytes = (double)c->device_bytes;
+  });
+}
+
 occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *grid,
                                   const occxc_functional *func, const occxc_options *opt,
                                   occxc_context **ctx) {
@@ -619,19 +658,7 @@ occxc_status occxc_context_create(const occxc_basis *basis, const occxc_grid *gr
 occxc_status occxc_context_destroy(occxc_context *ctx) {
   return guarded([&] {
     if (!ctx) return;
-    {
-      DeviceGuard g(ctx->device);
-      if (ctx->stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
-      }
-      ctx->stream = nullptr;
-      try {
-        collect_timing(ctx);
-      } catch (...) {
-      }
-      delete ctx;
-    }
+    context_destroy(ctx);
   });
 }
 
@@ -656,7 +683,7 @@ occxc_status occxc_build_device(occxc_context *ctx, const double *d_D, double *d
     OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     DeviceGuard g(ctx->device);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
-    run_build(ctx, d_D, d_Vxc, d_result, s);
+    launch_build(ctx, d_D, d_Vxc, d_result, s);
   });
 }
 
@@ -676,7 +703,7 @@ occxc_status occxc_batch_build(occxc_context *ctx, const double *const *D, doubl
       OCCXC_HIP(hipMemcpyAsync(ctx->d_D.get() + ctx->mol_off[m], D[m], nn * sizeof(double),
                                hipMemcpyHostToDevice, ctx->stream));
     }
-    run_build(ctx, ctx->d_D.get(), ctx->d_V.get(), ctx->d_result.get(), ctx->stream);
+    launch_build(ctx, ctx->d_D.get(), ctx->d_V.get(), ctx->d_result.get(), ctx->stream);
     std::vector<double> res(2 * ctx->nmol);
     for (int m = 0; m < ctx->nmol; ++m) {
       const size_t nn = (size_t)ctx->mol_nbf[m] * ctx->mol_nbf[m];
@@ -731,6 +758,16 @@ occxc_status occxc_context_evaluate_ao(occxc_context *ctx) {
     t.run(OCCXC_STAGE_AO, [&] {
       launch_ao_eval(ctx->args, ctx->d_ao.get(), ctx->opt.precision != 0, ctx->stream);
     });
+  });
+}
+
+occxc_status occxc_context_set_graph(occxc_context *ctx, int enable) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    DeviceGuard g(ctx->device);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->use_graph = enable != 0;
   });
 }
 

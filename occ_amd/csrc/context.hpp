@@ -1,0 +1,87 @@
+// context.hpp -- the occxc context (opaque in the C ABI) and the internal
+// entry points shared by the C ABI (api.cpp) and the serving loop
+// (server.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "occxc.h"
+#include "common.hpp"
+#include "contract.hpp"
+#include "kernels.hpp"
+
+struct occxc_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  occxc_options opt{};
+  int nbf = 0, nsh = 0, deriv = 0, ncomp = 1, nfunc = 0;
+  int nmol = 0, maxnbf = 0;
+  long long mtot = 0;
+  std::vector<int> mol_nbf;
+  std::vector<long long> mol_off;
+  int funcs[occxc::kMaxFunc] = {0};
+  long long npts = 0, ptot = 0;
+  int nleaves = 0;
+  double mean_m = 0.0, flops = 0.0, ao_write_bytes = 0.0;
+  size_t ao_elems = 0, ao_elem_size = 8;
+  std::vector<occxc::LeafDev> leaves;
+  occxc::DevBuf<occxc::ShellDev> d_shells;
+  occxc::DevBuf<double> d_exps, d_coefs;
+  occxc::DevBuf<occxc::LeafDev> d_leaves;
+  occxc::DevBuf<occxc::MolDev> d_mols;
+  occxc::DevBuf<int> d_leaf_shells, d_ao_map, d_active;
+  occxc::DevBuf<double> d_px, d_py, d_pz, d_pw;
+  occxc::DevBuf<char> d_ao;
+  occxc::DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
+  occxc::ContractPlan plan;
+  occxc::BuildArgs args{};
+  // stage timing: pending event sets, summed on read-out
+  bool timing = false;
+  std::vector<std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>> pending;
+  double stage_ms[OCCXC_NSTAGES] = {0};
+  long long stage_n[OCCXC_NSTAGES] = {0};
+  bool ao_ready = false;
+  size_t device_bytes = 0;
+  // captured build graph and the (D, Vxc, result, stream) it was captured on
+  bool use_graph = false;
+  hipGraphExec_t graph_exec = nullptr;
+  const void *graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace occxc {
+
+// Sets up c for nmol molecules.  With byte_limit > 0 it throws
+// OCCXC_ERR_OUT_OF_MEMORY before the large allocations when the context would
+// need more device memory than that (c->device_bytes then holds the need).
+void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
+                   const occxc_grid *grids, const occxc_functional *func,
+                   const occxc_options *opt, size_t byte_limit = 0);
+// One build on stream s, directly or through the captured graph.
+void launch_build(occxc_context *c, const double *d_D, double *d_V, double *d_result,
+                  hipStream_t s);
+// Waits for the context's work and frees it.
+void context_destroy(occxc_context *c);
+
+// Runs f, turning exceptions into a status and the thread's last error.
+template <typename F> occxc_status guarded(F &&f) {
+  try {
+    clear_last_error();
+    f();
+    return OCCXC_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return OCCXC_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return OCCXC_ERR_INTERNAL;
+  }
+}
+
+} // namespace occxc

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__re
 constexpr int kMapCap = 2048;
 
 template <typename S, typename T>
-__global__ __launch_bounds__(256) void k_density_wave(BuildArgs a, const S *__restrict__ ao,
+__global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *__restrict__ ao,
                                                       const T *__restrict__ D2t) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -396,18 +396,134 @@ __global__ __launch_bounds__(256) void k_density_wave(BuildArgs a, const S *__re
   density_epilogue(a, L, leaf, s_q, s_red);
 }
 
+// LDS-staged variant: the leaf's Phi is staged in LDS kBlkPts points at a
+// time (all mp AO rows, converted to T), so Phi is read from HBM once; the
+// four wavefronts share it as the A operand and split the 16-wide nu tiles
+// (wave w: tiles w, w + 4, ...).  D_loc columns (B operand) are gathered
+// through the AO map, one 16-mu step ahead of the MFMAs.  Phi(nu, p) for rho
+// comes from the same LDS block, grad Phi from HBM (each element once).  The
+// four wavefronts' partial sums are added in a fixed order.
+constexpr int kBlkPts = 32;
+constexpr int kBlkMaxMp = 256;
+constexpr int kBlkLd = kBlkPts + 1;
+
+static size_t density_blk_smem(int max_mp) {
+  return sizeof(double) * (4 * kMaxLeafPts + 4 * 4 * kBlkPts + 4) + sizeof(int) * kBlkMaxMp +
+         sizeof(double) * (size_t)max_mp * kBlkLd;
+}
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256, 2) void k_density_blk(BuildArgs a, const S *__restrict__ ao,
+                                                        const T *__restrict__ D2t) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  extern __shared__ double smem[];
+  double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(smem);
+  double *s_w = smem + 4 * kMaxLeafPts;  // [wave][q][kBlkPts]
+  double *s_red = s_w + 4 * 4 * kBlkPts;
+  int *s_col = reinterpret_cast<int *>(s_red + 4);
+  T *s_phi = reinterpret_cast<T *>(s_col + kBlkMaxMp);  // [mp][kBlkLd]
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const S *phi = ao + L.ao_off;
+  const T *Dm = D2t + MD.moff;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  for (int i = t; i < L.mp; i += blockDim.x) s_col[i] = map[i] * MD.nbf;
+  const int ntile = L.mp >> 4;
+  for (int p0 = 0; p0 < L.np; p0 += kBlkPts) {
+    __syncthreads();
+    for (int idx = t; idx < L.mp * kBlkPts; idx += blockDim.x) {
+      const int mu = idx / kBlkPts, p = idx - mu * kBlkPts;
+      s_phi[mu * kBlkLd + p] = static_cast<T>(phi[(long long)mu * L.np + p0 + p]);
+    }
+    __syncthreads();
+    double part[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[i][r][q] = 0.0;
+    for (int nt = wave; nt < ntile; nt += 4) {
+      acc_t acc[2] = {M::zero(), M::zero()};
+      const int nu = nt * 16 + l16;
+      const int gn = map[nu];
+      T cb[4];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) cb[s4] = Dm[s_col[4 * g + s4] + gn];
+      for (int mc = 0; mc < L.mp; mc += 16) {
+        T nb[4];
+        const bool more = mc + 16 < L.mp;
+        if (more) {
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) nb[s4] = Dm[s_col[mc + 16 + 4 * g + s4] + gn];
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const T *row = s_phi + (mc + 4 * g + s4) * kBlkLd + l16;
+          acc[0] = M::mma(row[0], cb[s4], acc[0]);
+          acc[1] = M::mma(row[16], cb[s4], acc[1]);
+        }
+        if (more) {
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) cb[s4] = nb[s4];
+        }
+      }
+      // acc[pt][r] = T(p0 + 16 pt + row(g, r), nu)
+      const T *prow = s_phi + nu * kBlkLd;
+      const long long goff = (long long)nu * L.np + p0;
+#pragma unroll
+      for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int p = 16 * pt + M::row(g, r);
+          const double tv = (double)acc[pt][r];
+          part[pt][r][0] += (double)prow[p] * tv;
+          if (deriv) {
+            part[pt][r][1] += (double)phi[cs + goff + p] * 2.0 * tv;
+            part[pt][r][2] += (double)phi[2 * cs + goff + p] * 2.0 * tv;
+            part[pt][r][3] += (double)phi[3 * cs + goff + p] * 2.0 * tv;
+          }
+        }
+    }
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double v = sum16(part[pt][r][q]);
+          if (l16 == 0) s_w[(wave * 4 + q) * kBlkPts + 16 * pt + M::row(g, r)] = v;
+        }
+    __syncthreads();
+    if (t < 4 * kBlkPts) {
+      const int q = t / kBlkPts, p = t - q * kBlkPts;
+      s_q[q][p0 + p] = s_w[q * kBlkPts + p] + s_w[(4 + q) * kBlkPts + p] +
+                       s_w[(8 + q) * kBlkPts + p] + s_w[(12 + q) * kBlkPts + p];
+    }
+  }
+  __syncthreads();
+  density_epilogue(a, L, leaf, s_q, s_red);
+}
+
 // ----------------------------------------------------------------- Vxc --
 // K_loc = Phi^T A for one leaf, A = c0 Phi + c . grad Phi built CH AOs at a
 // time in LDS; each wavefront owns 16-row mu tiles (Phi from the resident AO
 // block) and multiplies them with the CH-wide A chunk.  K_loc^T is scattered
 // into the global accumulator (Vxc = K + K^T afterwards).
 template <typename S, typename T, int CH>
-__global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restrict__ ao) {
+__global__ __launch_bounds__(256, 2) void k_vxc_mfma(BuildArgs a, const S *__restrict__ ao) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   using vec_t = typename M::vec_t;
   constexpr int NT = CH / 16;
-  constexpr int PSEG = 128;  // points per register-resident Phi segment
+  // points per register-resident Phi segment (FP64: 64, to fit two waves per SIMD)
+  constexpr int PSEG = sizeof(T) == 8 ? 64 : 128;
   extern __shared__ double smem[];
   double(*s_c)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(smem);
   T *s_A = reinterpret_cast<T *>(smem + 4 * kMaxLeafPts);
@@ -415,11 +531,12 @@ __global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restri
   if (!a.leaf_active[leaf]) return;
   const LeafDev L = a.leaves[leaf];
   const MolDev MD = a.mols[L.mol];
+  double *Km = a.K + MD.moff;
+  const int *map = a.ao_map + L.map_off;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
   const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
-  const int *map = a.ao_map + L.map_off;
   const bool deriv = a.deriv != 0;
   // chunk width: CH AOs for leaves of <= 128 points, half that above
   // (the LDS buffer holds CH x (128 + pad) = CH/2 x (256 + 2 pad) elements)
@@ -433,7 +550,7 @@ __global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restri
     const int ntc = min(chw, L.mp - c) >> 4;
     const int nel = 16 * ntc * L.np;
     // A chunk: each thread handles 4 consecutive points of a row per step
-#pragma unroll 4
+#pragma unroll 2
     for (int idx = 4 * threadIdx.x; idx < nel; idx += 4 * blockDim.x) {
       const int vl = idx / L.np, p = idx - vl * L.np;
       const long long o = (long long)(c + vl) * L.np + p;
@@ -479,7 +596,8 @@ __global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restri
           }
         }
       }
-      // acc[j][r] = K_loc(mu = mt*16 + row(g, r), nu = c + 16 j + l16); store K^T
+      // acc[j][r] = K_loc(mu = mt*16 + row(g, r), nu = c + 16 j + l16); scatter
+      // K_loc^T into K (Vxc = K + K^T afterwards)
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         const int nu = c + 16 * j + l16;
@@ -488,8 +606,7 @@ __global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restri
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int mu = mt * 16 + M::row(g, r);
-          if (mu < L.m)
-            atomicAdd(a.K + MD.moff + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
+          if (mu < L.m) atomicAdd(Km + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
         }
       }
     }
@@ -499,10 +616,13 @@ __global__ __launch_bounds__(256) void k_vxc_mfma(BuildArgs a, const S *__restri
 
 // ------------------------------------------------------ split-BF16 path --
 // FP32 values are split exactly into three BF16 terms x = x0 + x1 + x2
-// (8 significant bits each); FP64 values (the density) into the three leading
-// BF16 terms.  A product keeps the six cross terms with i + j <= 2 on
-// v_mfma_f32_16x16x32_bf16 (16 x 16 outputs, K = 32), smallest first, with
-// FP32 accumulation.  Operand layout: lane l supplies A[l%16][8(l/16)+i] and
+// (8 significant bits each).  A product keeps the six cross terms with
+// i + j <= 2 on v_mfma_f32_16x16x32_bf16 (16 x 16 outputs, K = 32), smallest
+// first, with FP32 accumulation.  The density matrix (FP64) is split into
+// four BF16 terms and its product with Phi also keeps x0 d3: the rounding of
+// D is the same at every grid point, so unlike the per-point rounding of Phi
+// and of the sums it does not average out of Exc and N_elec, and it is kept
+// to 2^-32 instead of 2^-24.  Operand layout: lane l supplies A[l%16][8(l/16)+i] and
 // B[8(l/16)+i][l%16], i = 0..7; accumulator register r holds
 // D[4(l/16)+r][l%16] (checked on the device).
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -513,12 +633,12 @@ __device__ __forceinline__ void split3(float x, __bf16 &h, __bf16 &m, __bf16 &l)
   m = static_cast<__bf16>(r1);
   l = static_cast<__bf16>(r1 - static_cast<float>(m));
 }
-__device__ __forceinline__ void split3(double x, __bf16 &h, __bf16 &m, __bf16 &l) {
-  h = static_cast<__bf16>(static_cast<float>(x));
-  const double r1 = x - static_cast<double>(static_cast<float>(h));
-  m = static_cast<__bf16>(static_cast<float>(r1));
-  const double r2 = r1 - static_cast<double>(static_cast<float>(m));
-  l = static_cast<__bf16>(static_cast<float>(r2));
+__device__ __forceinline__ void split4(double x, __bf16 (&o)[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    o[i] = static_cast<__bf16>(static_cast<float>(x));
+    x -= static_cast<double>(static_cast<float>(o[i]));
+  }
 }
 __device__ __forceinline__ void split8(const float (&x)[8], bf8 (&o)[3]) {
 #pragma unroll
@@ -529,6 +649,17 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf8 (&o)[3]) {
     o[1][i] = m;
     o[2][i] = l;
   }
+}
+// Phi (3 terms) times D (4 terms): the six products of mma3 plus x0 d3
+__device__ __forceinline__ f4 mma3d(const bf8 (&a)[3], const bf8 (&b)[4], f4 c) {
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[3], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
+  return c;
 }
 __device__ __forceinline__ f4 mma3(const bf8 (&a)[3], const bf8 (&b)[3], f4 c) {
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
@@ -542,13 +673,13 @@ __device__ __forceinline__ f4 mma3(const bf8 (&a)[3], const bf8 (&b)[3], f4 c) {
 __device__ __forceinline__ bf8 lds8(const __bf16 *p) { return *reinterpret_cast<const bf8 *>(p); }
 
 // T = Phi D_loc with split-BF16 operands: D_loc tiles (nu-major, mu fastest)
-// are split from FP64 into three BF16 planes in LDS; Phi (FP32 AO block) is
+// are split from FP64 into four BF16 planes in LDS; Phi (FP32 AO block) is
 // split in registers.
 __global__ __launch_bounds__(256) void k_density_bf16(BuildArgs a, const float *__restrict__ ao) {
   constexpr int NV = 64, KC = 32, LDK = KC + 8;
   __shared__ double s_q[4][kMaxLeafPts];
   __shared__ double s_red[4];
-  __shared__ __attribute__((aligned(16))) __bf16 s_D[3][NV * LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 s_D[4][NV * LDK];
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
   const MolDev MD = a.mols[L.mol];
@@ -583,11 +714,10 @@ __global__ __launch_bounds__(256) void k_density_bf16(BuildArgs a, const float *
           const int vl = idx & (NV - 1), ml = idx >> 6;
           double v = 0.0;
           if (vl < 16 * ntc) v = a.D2[MD.moff + (long long)map[k + ml] * MD.nbf + map[c + vl]];
-          __bf16 h, m, l;
-          split3(v, h, m, l);
-          s_D[0][vl * LDK + ml] = h;
-          s_D[1][vl * LDK + ml] = m;
-          s_D[2][vl * LDK + ml] = l;
+          __bf16 d[4];
+          split4(v, d);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) s_D[t][vl * LDK + ml] = d[t];
         }
         __syncthreads();
         if (mine) {
@@ -604,9 +734,10 @@ __global__ __launch_bounds__(256) void k_density_bf16(BuildArgs a, const float *
           for (int nt = 0; nt < 4; ++nt) {
             if (nt < ntc) {
               const int o = (16 * nt + l16) * LDK + 8 * g;
-              const bf8 bv[3] = {lds8(&s_D[0][o]), lds8(&s_D[1][o]), lds8(&s_D[2][o])};
-              acc[0][nt] = mma3(av[0], bv, acc[0][nt]);
-              acc[1][nt] = mma3(av[1], bv, acc[1][nt]);
+              const bf8 bv[4] = {lds8(&s_D[0][o]), lds8(&s_D[1][o]), lds8(&s_D[2][o]),
+                                 lds8(&s_D[3][o])};
+              acc[0][nt] = mma3d(av[0], bv, acc[0][nt]);
+              acc[1][nt] = mma3d(av[1], bv, acc[1][nt]);
             }
           }
         }
@@ -662,11 +793,12 @@ __global__ __launch_bounds__(256) void k_vxc_bf16(BuildArgs a, const float *__re
   if (!a.leaf_active[leaf]) return;
   const LeafDev L = a.leaves[leaf];
   const MolDev MD = a.mols[L.mol];
+  double *Km = a.K + MD.moff;
+  const int *map = a.ao_map + L.map_off;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
   const float *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
-  const int *map = a.ao_map + L.map_off;
   const bool deriv = a.deriv != 0;
   const int chw = L.np <= 128 ? CH : CH / 2;
   const int ld = L.np + PADP;
@@ -685,7 +817,7 @@ __global__ __launch_bounds__(256) void k_vxc_bf16(BuildArgs a, const float *__re
         v += s_c[1][p] * (double)phi[cs + o] + s_c[2][p] * (double)phi[2 * cs + o] +
              s_c[3][p] * (double)phi[3 * cs + o];
       __bf16 h, m, l;
-      split3(v, h, m, l);
+      split3(static_cast<float>(v), h, m, l);
       s_A[vl * ld + p] = h;
       s_A[plane + vl * ld + p] = m;
       s_A[2 * plane + vl * ld + p] = l;
@@ -718,8 +850,7 @@ __global__ __launch_bounds__(256) void k_vxc_bf16(BuildArgs a, const float *__re
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int mu = mt * 16 + 4 * g + r;
-          if (mu < L.m)
-            atomicAdd(a.K + MD.moff + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
+          if (mu < L.m) atomicAdd(Km + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
         }
       }
     }
@@ -729,6 +860,154 @@ __global__ __launch_bounds__(256) void k_vxc_bf16(BuildArgs a, const float *__re
 
 static size_t vxc_bf16_smem() {
   return sizeof(double) * 4 * kMaxLeafPts + sizeof(__bf16) * 3 * 64 * (128 + 8);
+}
+
+// ------------------------------------------- FP32 with compensated D --
+// D2 = Dh + Dl with Dh the FP32 rounding of the doubled density and Dl the
+// BF16 rounding of the remainder.  T = Phi Dh on the FP32 matrix cores plus
+// Phi Dl on the BF16 ones, into the same FP32 accumulator: Phi Dl is 2^-24 of
+// T, so Phi rounded to BF16 is enough for it.  The rounding of D is the same
+// at every grid point and would otherwise not average out of Exc and N_elec.
+// Per wavefront a 32-point block, as k_density_wave, in steps of 32 AOs: the
+// FP32 instruction takes mu = 16h + 4g + s at sub-step (h, s) in lane group g,
+// and the BF16 instruction takes the same eight mu as its K slots 4h + s.
+__global__ __launch_bounds__(256, 2) void k_density_f32c(BuildArgs a, const float *__restrict__ ao,
+                                                         const float *__restrict__ Dh,
+                                                         const __bf16 *__restrict__ Dl) {
+  __shared__ double s_q[4][kMaxLeafPts];
+  __shared__ double s_red[4];
+  __shared__ int s_col[kMapCap];
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  const LeafDev L = a.leaves[leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const float *phi = ao + L.ao_off;
+  const float *Dhm = Dh + MD.moff;
+  const __bf16 *Dlm = Dl + MD.moff;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  for (int i = threadIdx.x; i < L.mp; i += blockDim.x) s_col[i] = map[i] * MD.nbf;
+  __syncthreads();
+  const int nblk = L.np >> 5;
+  for (int blk = wave; blk < nblk; blk += 4) {
+    const int p0 = blk * 32;
+    double part[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[i][r][q] = 0.0;
+    for (int c = 0; c < L.mp; c += 32) {
+      f4 acc[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+      const int gn0 = map[c + l16], gn1 = map[c + 16 + l16];
+      float ca[8][2], cb[8][2];
+      auto load = [&](int mc, float (&xa)[8][2], float (&xb)[8][2]) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int mu = mc + 16 * (t >> 2) + 4 * g + (t & 3);
+          const float *row = phi + (long long)mu * L.np + p0 + l16;
+          const int col = s_col[mu];
+          xa[t][0] = row[0];
+          xa[t][1] = row[16];
+          xb[t][0] = Dhm[col + gn0];
+          xb[t][1] = Dhm[col + gn1];
+        }
+      };
+      load(0, ca, cb);
+      for (int mc = 0; mc < L.mp; mc += 32) {
+        // the remainder terms of this step are loaded here and used after the
+        // FP32 products, which cover their latency
+        bf8 bl[2];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int col = s_col[mc + 16 * (t >> 2) + 4 * g + (t & 3)];
+          bl[0][t] = Dlm[col + gn0];
+          bl[1][t] = Dlm[col + gn1];
+        }
+        float na[8][2], nb[8][2];
+        const bool more = mc + 32 < L.mp;
+        if (more) load(mc + 32, na, nb);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[t][0], cb[t][0], acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[t][0], cb[t][1], acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[t][1], cb[t][0], acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(ca[t][1], cb[t][1], acc[1][1], 0, 0, 0);
+        }
+        bf8 al[2];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          al[0][t] = static_cast<__bf16>(ca[t][0]);
+          al[1][t] = static_cast<__bf16>(ca[t][1]);
+        }
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+          for (int nt = 0; nt < 2; ++nt)
+            acc[pt][nt] =
+                __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[pt], bl[nt], acc[pt][nt], 0, 0, 0);
+        if (more) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            ca[t][0] = na[t][0];
+            ca[t][1] = na[t][1];
+            cb[t][0] = nb[t][0];
+            cb[t][1] = nb[t][1];
+          }
+        }
+      }
+      // acc[pt][nt][r] = T(p0 + 16 pt + 4g + r, c + 16 nt + l16)
+#pragma unroll
+      for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+          const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16 + 4 * g;
+          const f4 f = load4(phi + off);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) part[pt][r][0] += (double)f[r] * (double)acc[pt][nt][r];
+          if (deriv) {
+            const f4 fx = load4(phi + cs + off), fy = load4(phi + 2 * cs + off),
+                     fz = load4(phi + 3 * cs + off);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const double t = 2.0 * (double)acc[pt][nt][r];
+              part[pt][r][1] += (double)fx[r] * t;
+              part[pt][r][2] += (double)fy[r] * t;
+              part[pt][r][3] += (double)fz[r] * t;
+            }
+          }
+        }
+    }
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double v = sum16(part[pt][r][q]);
+          if (l16 == 0) s_q[q][p0 + pt * 16 + 4 * g + r] = v;
+        }
+  }
+  __syncthreads();
+  density_epilogue(a, L, leaf, s_q, s_red);
+}
+
+// D2 (FP64) -> Dh = fl32(D2), Dl = bf16(D2 - Dh)
+__global__ void k_split_density(const double *__restrict__ x, float *__restrict__ hi,
+                                __bf16 *__restrict__ lo, long long n) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const float h = static_cast<float>(x[k]);
+    hi[k] = h;
+    lo[k] = static_cast<__bf16>(static_cast<float>(x[k] - static_cast<double>(h)));
+  }
 }
 
 __global__ void k_to_float(const double *__restrict__ x, float *__restrict__ y, long long n) {
@@ -760,14 +1039,16 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
     p.max_np = std::max(p.max_np, L.np);
     p.max_mp = std::max(p.max_mp, L.mp);
   }
-  // density kernel: per-wavefront (0) or LDS-shared D tiles (1); the
+  // density kernel: LDS-staged Phi (0; leaves up to kBlkMaxMp AOs, else the
+  // per-wavefront kernel), LDS-shared D tiles (1) or per-wavefront (2); the
   // per-wavefront kernel keeps the leaf's AO map in LDS and needs mp <= 2048
   const char *dv = std::getenv("OCCXC_DENSITY_VARIANT");
   p.density_variant = dv ? std::atoi(dv) : 0;
   if (p.max_mp > kMapCap) p.density_variant = 1;
   if (precision == 1) {
     p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
-    p.bytes += p.d2f.bytes();
+    p.d2l.alloc((size_t)std::max<long long>(mtot, 1));
+    p.bytes += p.d2f.bytes() + p.d2l.bytes();
   }
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<double, double, 64>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -778,6 +1059,12 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_mfma<float, float, 64>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)vxc_smem(sizeof(float))));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_density_blk<double, double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)density_blk_smem(kBlkMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_density_blk<float, double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)density_blk_smem(kBlkMaxMp)));
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_bf16),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)vxc_bf16_smem()));
   return p;
@@ -786,15 +1073,23 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s) {
   if (a.nleaves == 0) return;
+  const bool blk = plan.density_variant == 0 && plan.max_mp <= kBlkMaxMp;
+  const size_t blk_smem = density_blk_smem(plan.max_mp);
   if (precision == 0) {
-    if (plan.density_variant == 0)
+    if (blk)
+      k_density_blk<double, double>
+          <<<a.nleaves, 256, blk_smem, s>>>(a, static_cast<const double *>(ao), a.D2);
+    else if (plan.density_variant != 1)
       k_density_wave<double, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
     else
       k_density_mfma<double, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
   } else if (precision == 3) {
-    if (plan.density_variant == 0)
+    if (blk)
+      k_density_blk<float, double>
+          <<<a.nleaves, 256, blk_smem, s>>>(a, static_cast<const float *>(ao), a.D2);
+    else if (plan.density_variant != 1)
       k_density_wave<float, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2);
     else
@@ -804,10 +1099,19 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
     k_density_bf16<<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao));
   } else {
     const long long nn = a.mtot;
-    k_to_float<<<(int)std::max<long long>(1, std::min<long long>((nn + 255) / 256, 4096)), 256,
-                 0, s>>>(a.D2, plan.d2f.get(), nn);
+    const int nb = (int)std::max<long long>(1, std::min<long long>((nn + 255) / 256, 4096));
+    if (plan.density_variant == 0) {
+      k_split_density<<<nb, 256, 0, s>>>(a.D2, plan.d2f.get(),
+                                         reinterpret_cast<__bf16 *>(plan.d2l.get()), nn);
+      check_launch("k_split_density");
+      k_density_f32c<<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), plan.d2f.get(),
+                                               reinterpret_cast<const __bf16 *>(plan.d2l.get()));
+      check_launch("k_density_f32c");
+      return;
+    }
+    k_to_float<<<nb, 256, 0, s>>>(a.D2, plan.d2f.get(), nn);
     check_launch("k_to_float");
-    if (plan.density_variant == 0)
+    if (plan.density_variant == 2)
       k_density_wave<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
                                                              plan.d2f.get());
     else

@@ -16,8 +16,10 @@ struct ContractPlan {
   bool use_mfma = false;   // false: the FP64 reference kernels of kernels.hip
   int max_np = 0;          // largest padded leaf (points)
   int max_mp = 0;          // largest padded leaf (AOs)
-  int density_variant = 0; // 0 per-wavefront, 1 LDS-shared D tiles
-  DevBuf<float> d2f;       // FP32 copy of the doubled density (FP32 modes)
+  int density_variant = 0; // 0 per-wavefront, 1 LDS-shared D tiles (FP32 mode:
+                           // 2 per-wavefront without the compensated density)
+  DevBuf<float> d2f;       // FP32 rounding of the doubled density (FP32 mode)
+  DevBuf<uint16_t> d2l;    // BF16 rounding of the remainder (FP32 mode)
   size_t bytes = 0;
 };
 

@@ -40,13 +40,15 @@ constexpr int kNCartMax = (kMaxL + 1) * (kMaxL + 2) / 2;
 // Value (and gradient) of every component of one shell at one point,
 // displacement (dx,dy,dz) from the shell centre.  Coefficients carry
 // common_fac(l).  Cartesian components in CCA order; spherical via c_sph.
-template <int L>
+// Each result goes to sink(q, m, value) (q = 0 value, 1..3 gradient) with
+// compile-time q and m once inlined, so no result array is indexed at run
+// time.
+template <int L, typename Sink>
 __device__ __forceinline__ void shell_point(const ShellDev &sh,
                                             const double *__restrict__ exps,
                                             const double *__restrict__ coefs,
                                             double dx, double dy, double dz,
-                                            bool deriv,
-                                            double (&out)[4][kNCartMax]) {
+                                            bool deriv, Sink &&sink) {
   constexpr int NC = (L + 1) * (L + 2) / 2;
   const double r2 = dx * dx + dy * dy + dz * dz;
   double S0 = 0.0, S1 = 0.0;
@@ -99,33 +101,33 @@ __device__ __forceinline__ void shell_point(const ShellDev &sh,
           vz += t * cz[k];
         }
       }
-      out[0][m] = v;
+      sink(0, m, v);
       if (deriv) {
-        out[1][m] = vx;
-        out[2][m] = vy;
-        out[3][m] = vz;
+        sink(1, m, vx);
+        sink(2, m, vy);
+        sink(3, m, vz);
       }
     }
   } else if (sh.pure && L == 1) {
     // CCA p order is (y, z, x) (gau2grid_transform.c:44-61)
-    const int perm[3] = {1, 2, 0};
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      out[0][m] = cv[perm[m]];
+      const int k = m == 2 ? 0 : m + 1;
+      sink(0, m, cv[k]);
       if (deriv) {
-        out[1][m] = cx[perm[m]];
-        out[2][m] = cy[perm[m]];
-        out[3][m] = cz[perm[m]];
+        sink(1, m, cx[k]);
+        sink(2, m, cy[k]);
+        sink(3, m, cz[k]);
       }
     }
   } else {
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      out[0][k] = cv[k];
+      sink(0, k, cv[k]);
       if (deriv) {
-        out[1][k] = cx[k];
-        out[2][k] = cy[k];
-        out[3][k] = cz[k];
+        sink(1, k, cx[k]);
+        sink(2, k, cy[k]);
+        sink(3, k, cz[k]);
       }
     }
   }
@@ -137,12 +139,13 @@ __device__ __forceinline__ void eval_shell(const ShellDev &sh,
                                            double dx, double dy, double dz,
                                            bool deriv,
                                            double (&out)[4][kNCartMax]) {
+  auto sink = [&](int q, int m, double v) { out[q][m] = v; };
   switch (sh.l) {
-  case 0: shell_point<0>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
-  case 1: shell_point<1>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
-  case 2: shell_point<2>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
-  case 3: shell_point<3>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
-  default: shell_point<4>(sh, exps, coefs, dx, dy, dz, deriv, out); break;
+  case 0: shell_point<0>(sh, exps, coefs, dx, dy, dz, deriv, sink); break;
+  case 1: shell_point<1>(sh, exps, coefs, dx, dy, dz, deriv, sink); break;
+  case 2: shell_point<2>(sh, exps, coefs, dx, dy, dz, deriv, sink); break;
+  case 3: shell_point<3>(sh, exps, coefs, dx, dy, dz, deriv, sink); break;
+  default: shell_point<4>(sh, exps, coefs, dx, dy, dz, deriv, sink); break;
   }
 }
 
@@ -187,7 +190,10 @@ __device__ double block_max_256(double v, double *scratch) {
 // significant shell each thread writes its components.  The AO block of a
 // leaf is [ncomp][mp][np] with points fastest, so each component row is a
 // coalesced run of np values.
-template <typename T>
+// LMAX is the largest angular momentum the kernel handles: the register
+// allocation is that of the largest shell, so bases with l <= 2 run the
+// LMAX = 2 instance.
+template <typename T, int LMAX>
 __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao) {
   const LeafDev L = a.leaves[blockIdx.x];
   const int p = threadIdx.x;
@@ -199,16 +205,17 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
   int mu = 0;
   for (int k = 0; k < L.nsig; ++k) {
     const ShellDev sh = a.shells[a.leaf_shells[L.sh_off + k]];
-    double out[4][kNCartMax];
-    eval_shell(sh, a.exps, a.coefs, x - sh.x, y - sh.y, z - sh.z, deriv, out);
-    for (int c = 0; c < sh.size; ++c) {
-      base[(long long)(mu + c) * L.np + p] = static_cast<T>(out[0][c]);
-      if (deriv) {
-        base[cstride + (long long)(mu + c) * L.np + p] = static_cast<T>(out[1][c]);
-        base[2 * cstride + (long long)(mu + c) * L.np + p] = static_cast<T>(out[2][c]);
-        base[3 * cstride + (long long)(mu + c) * L.np + p] = static_cast<T>(out[3][c]);
-      }
-    }
+    T *row = base + (long long)mu * L.np + p;
+    auto sink = [&](int q, int m, double v) {
+      row[q * cstride + (long long)m * L.np] = static_cast<T>(v);
+    };
+    const double dx = x - sh.x, dy = y - sh.y, dz = z - sh.z;
+    if (sh.l == 0) shell_point<0>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
+    else if (sh.l == 1) shell_point<1>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
+    else if (LMAX == 2 || sh.l == 2)
+      shell_point<2>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
+    else if (sh.l == 3) shell_point<(LMAX > 2 ? 3 : 2)>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
+    else shell_point<LMAX>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
     mu += sh.size;
   }
   for (int r = L.m; r < L.mp; ++r)
@@ -335,6 +342,7 @@ __global__ void k_prepare_density(const double *__restrict__ D, double *__restri
 }
 
 // blockIdx.y = molecule; Vxc = K + K^T
+// V = K + K^T
 __global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V,
                                   const MolDev *__restrict__ mols) {
   const MolDev M = mols[blockIdx.y];
@@ -562,10 +570,14 @@ void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int
 
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s) {
   if (a.nleaves == 0) return;
-  if (fp32)
-    k_ao_eval<float><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
-  else
-    k_ao_eval<double><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
+  const bool low = a.lmax <= 2;
+  if (fp32) {
+    if (low) k_ao_eval<float, 2><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
+    else k_ao_eval<float, kMaxL><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
+  } else {
+    if (low) k_ao_eval<double, 2><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
+    else k_ao_eval<double, kMaxL><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
+  }
   check_launch();
 }
 

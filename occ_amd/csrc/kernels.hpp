@@ -56,6 +56,7 @@ struct BuildArgs {
   int deriv;
   int ncomp;
   int nfunc;
+  int lmax;       // largest shell angular momentum
   int func_ids[kMaxFunc];
   double dthr;
   long long ptot;  // total padded points
@@ -84,8 +85,7 @@ void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32,
                                hipStream_t s);
 void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s);
 void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol,
-                     int maxnbf, const double *leaf_part, double *result,
-                     hipStream_t s);
+                     int maxnbf, const double *leaf_part, double *result, hipStream_t s);
 void launch_eval_basis_flat(const ShellDev *shells, int nshell,
                             const double *exps, const double *coefs,
                             const double *xyz, long long npts, int deriv,

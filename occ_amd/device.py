@@ -71,3 +71,11 @@ def copy_device(dst, src, nbytes):
 
 def synchronize():
     _check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def memory_info():
+    """(free, total) device bytes of the current device."""
+    free = ctypes.c_size_t()
+    total = ctypes.c_size_t()
+    _check(hip().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)), "hipMemGetInfo")
+    return free.value, total.value

@@ -110,6 +110,10 @@ class XCContext:
     def set_timing(self, enable=True):
         _lib.check(_lib.lib().occxc_context_set_timing(self._h, int(bool(enable))))
 
+    def set_graph(self, enable=True):
+        """Replay builds from a captured HIP graph (see occxc_context_set_graph)."""
+        _lib.check(_lib.lib().occxc_context_set_graph(self._h, int(bool(enable))))
+
     def timing(self):
         """{stage: (total ms, launches)} of the timed builds since set_timing."""
         ms = np.zeros(len(self.STAGES))
@@ -133,6 +137,32 @@ class XCContext:
             self.close()
         except Exception:
             pass
+
+
+def estimate_batch_bytes(bases, grids, funcs, precision="fp64", screen=True, device=0):
+    """Device bytes an XCBatch over these molecules would hold
+    (occxc_batch_estimate_bytes): hierarchy and screening only."""
+    n = len(bases)
+    keep = []
+    cb = (_lib.CBasis * n)()
+    cg = (_lib.CGrid * n)()
+    for i, (b, (pts, w)) in enumerate(zip(bases, grids)):
+        st, k = b.c_struct()
+        pts = _lib.f64(pts).reshape(-1)
+        w = _lib.f64(w)
+        keep += [k, pts, w]
+        cb[i] = st
+        cg[i] = _lib.CGrid(len(w), _lib.dptr(pts), _lib.dptr(w))
+    ids = _lib.i32(funcs)
+    func = _lib.CFunctional(len(ids), _lib.iptr(ids))
+    o = default_options()
+    o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    o.screen = int(bool(screen))
+    o.device = device
+    out = ctypes.c_double()
+    _lib.check(_lib.lib().occxc_batch_estimate_bytes(n, cb, cg, ctypes.byref(func),
+                                                     ctypes.byref(o), ctypes.byref(out)))
+    return out.value
 
 
 class XCBatch:
@@ -199,6 +229,8 @@ class XCBatch:
         return {f: getattr(i, f) for f, _ in i._fields_}
 
     set_timing = XCContext.set_timing
+    set_graph = XCContext.set_graph
+    evaluate_ao = XCContext.evaluate_ao
     timing = XCContext.timing
     STAGES = XCContext.STAGES
     synchronize = XCContext.synchronize

@@ -10,6 +10,11 @@ conversion factor is occ::units::ANGSTROM_TO_BOHR, include/occ/core/units.h:7-8)
   U(-0.3, 0.3) A per axis; each molecule rotated by a uniform random rotation
   (unit quaternion from four normal deviates).
 - alkane(n): all-trans C_nH_(2n+2), C-C 1.54 A, C-H 1.09 A, tetrahedral angles.
+- batch_set(count, seed): the mixed batch of the batched-launch workload:
+  count/2 water clusters with n ~ U{1..16} and count/2 alkanes with
+  n ~ U{1..24}, each with 6-31G or def2-SVP drawn with equal odds, shuffled.
+- sized_molecule(natoms, rng): a water cluster or an alkane (equal odds) with
+  about natoms atoms, for request streams with log-uniform molecule sizes.
 """
 import math
 
@@ -99,3 +104,29 @@ def alkane(n):
 
 def nuclear_charge(atoms):
     return sum(int(a[0]) for a in atoms)
+
+
+def batch_set(count=512, seed=0):
+    """[(name, atoms, basis_name)] for the batched-launch workload."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        basis = "6-31g" if rng.integers(2) == 0 else "def2-svp"
+        if i < count // 2:
+            n = int(rng.integers(1, 17))
+            out.append((f"(H2O){n}", water_cluster(n, seed=int(rng.integers(1 << 31))), basis))
+        else:
+            n = int(rng.integers(1, 25))
+            out.append((f"C{n}H{2 * n + 2}", alkane(n), basis))
+    order = rng.permutation(count)
+    return [out[i] for i in order]
+
+
+def sized_molecule(natoms, rng):
+    """(name, atoms) of a water cluster (3n atoms) or an alkane (3n + 2 atoms)
+    with about natoms atoms."""
+    if rng.integers(2) == 0:
+        n = max(1, int(round(natoms / 3.0)))
+        return f"(H2O){n}", water_cluster(n, seed=int(rng.integers(1 << 31)))
+    n = max(1, int(round((natoms - 2) / 3.0)))
+    return f"C{n}H{2 * n + 2}", alkane(n)

@@ -91,3 +91,15 @@ def test_radial_grid_rejects_unknown_method():
     n = np.zeros(1, dtype=np.int32)
     st = _lib.lib().occxc_radial_grid(42, 4, 1.0, _lib.dptr(r), _lib.dptr(r), _lib.iptr(n))
     assert st == 2
+
+
+def test_server_rejects_bad_options_without_gpu():
+    L = _lib.lib()
+    o = _lib.CServerOptions()
+    L.occxc_server_default_options(ctypes.byref(o))
+    assert o.slots == 4 and o.memory_budget == 0.0
+    o.slots = 0
+    h = ctypes.c_void_p()
+    assert L.occxc_server_create(ctypes.byref(o), ctypes.byref(h)) == 1
+    assert L.occxc_server_submit(None, 1, None, None, None, None, None) == 1
+    assert L.occxc_server_destroy(None) == 0
