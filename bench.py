@@ -29,6 +29,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, mode, stage):
+    """HBM bytes per launch of a build stage, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary committed under profiles/ (None when not measured)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.txt")
+    try:
+        for line in open(path):
+            f = line.split("#")[0].split()
+            if len(f) == 4 and f[:3] == [workload, mode, stage]:
+                return float(f[3])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -186,7 +200,12 @@ def main():
         results[mode] = r
         ctx.close()
 
-    ok = [m for m in results if ref is None or results[m]["dexc"] <= 1e-6]
+    # headline: the fastest mode whose Exc is within 1e-6 Ha of the FP64 oracle;
+    # without the oracle only FP64 (the reference precision) qualifies
+    if ref is None:
+        ok = [m for m in results if m == "fp64"]
+    else:
+        ok = [m for m in results if results[m]["dexc"] <= 1e-6]
     head = min(ok or list(results), key=lambda m: results[m]["ms_per_step"])
     R = results[head]
     tim = R["timing"]
@@ -197,7 +216,7 @@ def main():
     achieved = flops_half / (kms * 1e-3) / 1e12
     roof = {"bound": "mfma", "kernel": kname, "achieved": achieved,
             "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
-            "traffic": None, "kernel_ms": kms,
+            "traffic": measured_traffic(args.workload, head, kname), "kernel_ms": kms,
             "work": "sum over leaves of 2 n_b m_b^2 (one GEMM of the build, screened m_b)"}
     ao_gbs = R["info"]["ao_write_bytes"] / (R["ao_ms"] * 1e-3) / 1e9 if R["ao_ms"] else None
     out = {

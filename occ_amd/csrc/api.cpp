@@ -367,6 +367,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
       for (int q = m; q < L.mp; ++q) ao_map.push_back(0);
       L.ao_off = ao_off;
       ao_off += (long long)c->ncomp * L.mp * L.np;
+      L.kl_off = c->kloc_elems;
+      c->kloc_elems += (long long)L.mp * L.mp;
       const long long o = h.off[li];
       for (int p = 0; p < L.np; ++p) {
         const long long src = o + (p < L.n ? p : 0);
@@ -410,8 +412,10 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
   // device memory the context will hold: AO values, the nbf^2 matrices
   // (D, D2, K, V and the FP32 copies), per-point data and the leaf tables
+  const bool two_pass = vxc_variant_setting() == 2;
   const size_t planned = ao_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
-                         c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes();
+                         c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
+                         (two_pass ? (size_t)c->kloc_elems * 8 : 0);
   c->device_bytes = planned;
   OCCXC_REQUIRE(byte_limit == 0 || planned <= byte_limit, OCCXC_ERR_OUT_OF_MEMORY,
                 "context needs " + std::to_string(planned >> 20) + " MiB, limit " +
@@ -463,9 +467,53 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.leaf_active = c->d_active.get();
 
   c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream);
+  c->plan.two_pass = c->plan.two_pass && c->plan.use_mfma;
+  if (c->plan.two_pass) {
+    // 32 x 32 AO tile pairs (TI <= TJ) per molecule and, for each, the leaves
+    // whose AOs touch both tiles (in leaf order); a leaf's AOs in one tile are
+    // a run of its (sorted) local AOs, recorded as a start and a bit mask
+    std::vector<TilePairDev> tpairs;
+    std::vector<TileEntry> tentries;
+    struct Touch { int tile, start; unsigned mask; };
+    std::vector<Touch> touch;
+    for (int mol = 0; mol < nmol; ++mol) {
+      const int nt = (mols[mol].nbf + 31) / 32;
+      std::vector<std::vector<TileEntry>> plist((size_t)nt * nt);
+      for (int li = mols[mol].leaf_begin; li < mols[mol].leaf_end; ++li) {
+        const LeafDev &L = c->leaves[li];
+        touch.clear();
+        for (int mu = 0; mu < L.m; ++mu) {
+          const int g = ao_map[L.map_off + mu];
+          if (touch.empty() || touch.back().tile != g / 32) touch.push_back({g / 32, mu, 0u});
+          touch.back().mask |= 1u << (g % 32);
+        }
+        for (size_t x = 0; x < touch.size(); ++x)
+          for (size_t y = x; y < touch.size(); ++y)
+            plist[(size_t)touch[x].tile * nt + touch[y].tile].push_back(
+                TileEntry{L.kl_off, L.mp, li, touch[x].start, touch[y].start, touch[x].mask,
+                          touch[y].mask});
+      }
+      for (int ti = 0; ti < nt; ++ti)
+        for (int tj = ti; tj < nt; ++tj) {
+          auto &v = plist[(size_t)ti * nt + tj];
+          tpairs.push_back(TilePairDev{(long long)tentries.size(), (int)v.size(), mol, ti, tj});
+          tentries.insert(tentries.end(), v.begin(), v.end());
+        }
+    }
+    c->d_kloc.alloc((size_t)std::max<long long>(c->kloc_elems, 1));
+    c->d_tpairs.alloc(std::max<size_t>(tpairs.size(), 1));
+    c->d_tpairs.upload(tpairs.data(), tpairs.size(), c->stream);
+    c->d_tentries.alloc(std::max<size_t>(tentries.size(), 1));
+    c->d_tentries.upload(tentries.data(), tentries.size(), c->stream);
+    a.kloc = c->d_kloc.get();
+    a.tpairs = c->d_tpairs.get();
+    a.ntpairs = (int)tpairs.size();
+    a.tentries = c->d_tentries.get();
+  }
   c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
                     c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
-                    c->plan.bytes;
+                    c->plan.bytes + c->d_kloc.bytes() + c->d_tpairs.bytes() +
+                    c->d_tentries.bytes();
 
   launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
   OCCXC_HIP(hipStreamSynchronize(c->stream));
@@ -515,7 +563,7 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
   StageTimer t{c, s, {}};
   t.run(OCCXC_STAGE_PREPARE, [&] {
     launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
-    launch_zero(c->d_K.get(), c->mtot, s);
+    if (!c->plan.two_pass) launch_zero(c->d_K.get(), c->mtot, s);
   });
   if (!c->plan.use_mfma) {
     const bool fp32 = c->opt.precision != OCCXC_PREC_FP64;
@@ -526,8 +574,13 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
     t.run(OCCXC_STAGE_VXC, [&] { run_vxc(c->plan, a, c->d_ao.get(), c->opt.precision, s); });
   }
   t.run(OCCXC_STAGE_FINALIZE, [&] {
-    launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
-                    d_result, s);
+    if (c->plan.two_pass) {
+      run_vxc_reduce(a, d_V, s);
+      launch_finalize_scalars(c->d_mols.get(), c->nmol, c->d_part.get(), d_result, s);
+    } else {
+      launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
+                      d_result, s);
+    }
   });
 }
 

@@ -38,6 +38,11 @@ struct occxc_context {
   occxc::DevBuf<char> d_ao;
   occxc::DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
   occxc::ContractPlan plan;
+  // two-pass Vxc: per-leaf K_loc blocks and the tile-pair leaf lists
+  occxc::DevBuf<double> d_kloc;
+  occxc::DevBuf<occxc::TilePairDev> d_tpairs;
+  occxc::DevBuf<occxc::TileEntry> d_tentries;
+  long long kloc_elems = 0;
   occxc::BuildArgs args{};
   // stage timing: pending event sets, summed on read-out
   bool timing = false;

@@ -287,10 +287,11 @@ __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__re
 // operands of the next 16-mu step are loaded before the MFMAs of the current
 // one are issued.
 constexpr int kMapCap = 2048;
+constexpr int kDefaultDensityVariant = 2;
 
 template <typename S, typename T>
 __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *__restrict__ ao,
-                                                      const T *__restrict__ D2t) {
+                                                      const T *__restrict__ D2t, int min_mp) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   __shared__ double s_q[4][kMaxLeafPts];
@@ -298,6 +299,7 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
   __shared__ int s_col[kMapCap];
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
+  if (L.mp < min_mp) return;  // k_density_blk handles it
   const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -425,6 +427,7 @@ __global__ __launch_bounds__(256, 2) void k_density_blk(BuildArgs a, const S *__
   T *s_phi = reinterpret_cast<T *>(s_col + kBlkMaxMp);  // [mp][kBlkLd]
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
+  if (L.mp > kBlkMaxMp) return;  // k_density_wave handles it
   const MolDev MD = a.mols[L.mol];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -517,7 +520,8 @@ __global__ __launch_bounds__(256, 2) void k_density_blk(BuildArgs a, const S *__
 // block) and multiplies them with the CH-wide A chunk.  K_loc^T is scattered
 // into the global accumulator (Vxc = K + K^T afterwards).
 template <typename S, typename T, int CH>
-__global__ __launch_bounds__(256, 2) void k_vxc_mfma(BuildArgs a, const S *__restrict__ ao) {
+__global__ __launch_bounds__(256, 2) void k_vxc_mfma(BuildArgs a, const S *__restrict__ ao,
+                                                     int min_mp) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   using vec_t = typename M::vec_t;
@@ -530,6 +534,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mfma(BuildArgs a, const S *__res
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   if (!a.leaf_active[leaf]) return;
   const LeafDev L = a.leaves[leaf];
+  if (L.mp < min_mp) return;  // k_vxc_sym handles it
   const MolDev MD = a.mols[L.mol];
   double *Km = a.K + MD.moff;
   const int *map = a.ao_map + L.map_off;
@@ -606,11 +611,132 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mfma(BuildArgs a, const S *__res
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int mu = mt * 16 + M::row(g, r);
-          if (mu < L.m) atomicAdd(Km + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
+          if (mu >= L.m) continue;
+          if (a.kloc) a.kloc[L.kl_off + (long long)mu * L.mp + nu] = (double)acc[j][r];
+          else atomicAdd(Km + gn + (long long)map[mu] * MD.nbf, (double)acc[j][r]);
         }
       }
     }
     __syncthreads();
+  }
+}
+
+// Symmetric form for leaves of up to kSymMaxMp AOs: the block
+//   V_loc = Phi^T A + A^T Phi
+// is symmetric, so only its 16 x 16 tiles (I, J) with I <= J are formed,
+// V_IJ = Phi_I^T A_J + A_I^T Phi_J (I < J) and K_II = Phi_I^T A_I, and only
+// they are scattered: half the atomics of the full K_loc, the same flops.
+// The leaf's points are streamed 16 at a time through LDS, where Phi and A
+// (built from the four AO components and the per-point coefficients) are
+// staged once and shared by the four wavefronts; each wavefront keeps its
+// tiles' accumulators in registers for the whole leaf (tiles w, w + 4, ...
+// of the upper triangle in row-major order).
+// Scatter: V_IJ(mu, nu) (I < J, so map[mu] < map[nu]) goes to K(map[nu],
+// map[mu]) only and K_II(mu, nu) to K(map[nu], map[mu]) as before; the
+// finalisation V = K + K^T then gives every entry once.
+constexpr int kSymPts = 16;
+constexpr int kSymMaxMp = 192;
+constexpr int kSymMaxTiles = kSymMaxMp / 16;
+constexpr int kSymPerWave = (kSymMaxTiles * (kSymMaxTiles + 1) / 2 + 3) / 4;
+
+template <typename T> constexpr int sym_ld() { return kSymPts + 16 / (int)sizeof(T); }
+
+static size_t vxc_sym_smem(size_t elem, int max_mp) {
+  const size_t ld = kSymPts + 16 / elem;
+  return sizeof(double) * 4 * kMaxLeafPts + 2 * elem * ld * (size_t)max_mp;
+}
+
+// upper-triangle tile pair p (row-major over I <= J of an nt x nt grid)
+__device__ __forceinline__ void sym_pair(int p, int nt, int &I, int &J) {
+  int i = 0, rem = p;
+  while (rem >= nt - i) {
+    rem -= nt - i;
+    ++i;
+  }
+  I = i;
+  J = i + rem;
+}
+
+template <typename S, typename T>
+__global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__restrict__ ao) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  using vec_t = typename M::vec_t;
+  constexpr int LD = sym_ld<T>();
+  extern __shared__ double smem[];
+  double(*s_c)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(smem);
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  if (!a.leaf_active[leaf]) return;
+  const LeafDev L = a.leaves[leaf];
+  if (L.mp > kSymMaxMp) return;  // k_vxc_mfma handles it
+  T *s_phi = reinterpret_cast<T *>(smem + 4 * kMaxLeafPts);
+  T *s_A = s_phi + L.mp * LD;
+  const MolDev MD = a.mols[L.mol];
+  double *Km = a.K + MD.moff;
+  const int *map = a.ao_map + L.map_off;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const S *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const bool deriv = a.deriv != 0;
+  for (int i = t; i < L.np; i += blockDim.x)
+    for (int k = 0; k < 4; ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+  const int nt = L.mp >> 4;
+  const int npair = nt * (nt + 1) / 2;
+  acc_t acc[kSymPerWave];
+#pragma unroll
+  for (int k = 0; k < kSymPerWave; ++k) acc[k] = M::zero();
+  for (int p0 = 0; p0 < L.np; p0 += kSymPts) {
+    __syncthreads();
+    for (int idx = t; idx < L.mp * kSymPts; idx += blockDim.x) {
+      const int mu = idx / kSymPts, p = idx - mu * kSymPts;
+      const long long o = (long long)mu * L.np + p0 + p;
+      const double f0 = (double)phi[o];
+      double v = s_c[0][p0 + p] * f0;
+      if (deriv)
+        v += s_c[1][p0 + p] * (double)phi[cs + o] + s_c[2][p0 + p] * (double)phi[2 * cs + o] +
+             s_c[3][p0 + p] * (double)phi[3 * cs + o];
+      s_phi[mu * LD + p] = static_cast<T>(f0);
+      s_A[mu * LD + p] = static_cast<T>(v);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSymPerWave; ++k) {
+      const int pr = wave + 4 * k;
+      if (pr < npair) {
+        int I, J;
+        sym_pair(pr, nt, I, J);
+        const int oi = (I * 16 + l16) * LD + 4 * g, oj = (J * 16 + l16) * LD + 4 * g;
+        const vec_t fi = *reinterpret_cast<const vec_t *>(s_phi + oi);
+        const vec_t aj = *reinterpret_cast<const vec_t *>(s_A + oj);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) acc[k] = M::mma(fi[s4], aj[s4], acc[k]);
+        if (I != J) {
+          const vec_t ai = *reinterpret_cast<const vec_t *>(s_A + oi);
+          const vec_t fj = *reinterpret_cast<const vec_t *>(s_phi + oj);
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) acc[k] = M::mma(ai[s4], fj[s4], acc[k]);
+        }
+      }
+    }
+  }
+  // acc[k][r] = tile (I, J) element (mu = 16 I + row(g, r), nu = 16 J + l16)
+#pragma unroll
+  for (int k = 0; k < kSymPerWave; ++k) {
+    const int pr = wave + 4 * k;
+    if (pr >= npair) continue;
+    int I, J;
+    sym_pair(pr, nt, I, J);
+    const int nu = J * 16 + l16;
+    if (nu >= L.m) continue;
+    const long long gn = map[nu];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mu = I * 16 + M::row(g, r);
+      if (mu >= L.m) continue;
+      if (a.kloc) a.kloc[L.kl_off + (long long)mu * L.mp + nu] = (double)acc[k][r];
+      else atomicAdd(Km + gn + (long long)map[mu] * MD.nbf, (double)acc[k][r]);
+    }
   }
 }
 
@@ -1016,6 +1142,60 @@ __global__ void k_to_float(const double *__restrict__ x, float *__restrict__ y, 
     y[k] = static_cast<float>(x[k]);
 }
 
+// ------------------------------------------------ two-pass Vxc reduction --
+// With a.kloc set, the Vxc kernels store their per-leaf blocks instead of
+// scattering them with atomics: the general kernel K_loc(mu, nu) in full, the
+// symmetric kernel V_loc(mu, nu) = K_loc(mu, nu) + K_loc(nu, mu) for local
+// 16-tiles t(mu) < t(nu) and K_loc(mu, nu) within a 16-tile.  One workgroup
+// per global 32 x 32 AO tile pair (TI <= TJ) then sums, leaf by leaf in list
+// order (deterministic), the stored entries that make up
+//   V(i, j) = sum over leaves of K_loc(li, lj) + K_loc(lj, li)
+// (dft.h:361-369 and the symmetrisation of xc_potential_matrix.cpp:35-40),
+// and writes V(i, j) and V(j, i).
+__device__ __forceinline__ double kloc_pair(const double *S, int mp, bool sym, int li, int lj) {
+  if (!sym) return S[(long long)li * mp + lj] + S[(long long)lj * mp + li];
+  const int ti = li >> 4, tj = lj >> 4;
+  double v = 0.0;
+  if (ti <= tj) v += S[(long long)li * mp + lj];
+  if (tj <= ti) v += S[(long long)lj * mp + li];
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_vxc_reduce(BuildArgs a, double *__restrict__ V,
+                                                    int sym_max_mp) {
+  const TilePairDev P = a.tpairs[blockIdx.x];
+  const MolDev MD = a.mols[P.mol];
+  const int t = threadIdx.x;
+  const int j = t & 31, i0 = t >> 5;  // elements (i0 + 8 k, j), k = 0..3
+  const TileEntry *list = a.tentries + P.list_off;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = 0; e < P.nlist; ++e) {
+    const TileEntry E = list[e];
+    if (!a.leaf_active[E.leaf]) continue;
+    if (!((E.mask_b >> j) & 1u)) continue;
+    const double *S = a.kloc + E.kl_off;
+    const bool sym = E.mp <= sym_max_mp;
+    const int lj = E.start_b + __popc(E.mask_b & ((1u << j) - 1u));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = i0 + 8 * k;
+      if ((E.mask_a >> i) & 1u) {
+        const int li = E.start_a + __popc(E.mask_a & ((1u << i) - 1u));
+        acc[k] += kloc_pair(S, E.mp, sym, li, lj);
+      }
+    }
+  }
+  const long long gj = 32LL * P.tj + j;
+  if (gj >= MD.nbf) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const long long gi = 32LL * P.ti + i0 + 8 * k;
+    if (gi >= MD.nbf) continue;
+    V[MD.moff + gi + gj * MD.nbf] = acc[k];
+    if (P.ti != P.tj) V[MD.moff + gj + gi * MD.nbf] = acc[k];
+  }
+}
+
 void check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -1029,6 +1209,11 @@ static size_t vxc_smem(size_t elem) {
   return sizeof(double) * 4 * kMaxLeafPts + elem * 64 * (128 + pad);
 }
 
+int vxc_variant_setting() {
+  const char *vv = std::getenv("OCCXC_VXC_VARIANT");
+  return vv ? std::atoi(vv) : 0;
+}
+
 ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
                                 int precision, hipStream_t s) {
   (void)deriv;
@@ -1040,11 +1225,17 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
     p.max_mp = std::max(p.max_mp, L.mp);
   }
   // density kernel: LDS-staged Phi (0; leaves up to kBlkMaxMp AOs, else the
-  // per-wavefront kernel), LDS-shared D tiles (1) or per-wavefront (2); the
-  // per-wavefront kernel keeps the leaf's AO map in LDS and needs mp <= 2048
+  // per-wavefront kernel), LDS-shared D tiles (1) or per-wavefront (2, the
+  // default: fastest measured); the per-wavefront kernel keeps the leaf's AO
+  // map in LDS and needs mp <= 2048.  In FP32 mode 0 and 2 select the
+  // compensated-density kernel and 3 the plain per-wavefront one.
   const char *dv = std::getenv("OCCXC_DENSITY_VARIANT");
-  p.density_variant = dv ? std::atoi(dv) : 0;
+  p.density_variant = dv ? std::atoi(dv) : kDefaultDensityVariant;
   if (p.max_mp > kMapCap) p.density_variant = 1;
+  // Vxc kernel: symmetric tile pairs for small leaves (0), general only (1),
+  // or (2) as 0 with the K_loc blocks stored and reduced in a second pass
+  p.vxc_variant = vxc_variant_setting();
+  p.two_pass = p.vxc_variant == 2 && precision != 2;  // not with the split-BF16 Vxc
   if (precision == 1) {
     p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
     p.d2l.alloc((size_t)std::max<long long>(mtot, 1));
@@ -1065,6 +1256,15 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_density_blk<float, double>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)density_blk_smem(kBlkMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<double, double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, float>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(float), kSymMaxMp)));
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_bf16),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)vxc_bf16_smem()));
   return p;
@@ -1073,26 +1273,33 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s) {
   if (a.nleaves == 0) return;
-  const bool blk = plan.density_variant == 0 && plan.max_mp <= kBlkMaxMp;
-  const size_t blk_smem = density_blk_smem(plan.max_mp);
+  // variant 0: LDS-staged kernel for leaves of up to kBlkMaxMp AOs, the
+  // per-wavefront kernel for the rest (each skips the other's leaves)
+  const bool blk = plan.density_variant == 0;
+  const int wave_min = blk ? kBlkMaxMp + 1 : 0;
+  const bool wave_too = !blk || plan.max_mp > kBlkMaxMp;
+  const size_t blk_smem = density_blk_smem(std::min(plan.max_mp, kBlkMaxMp));
   if (precision == 0) {
-    if (blk)
-      k_density_blk<double, double>
-          <<<a.nleaves, 256, blk_smem, s>>>(a, static_cast<const double *>(ao), a.D2);
-    else if (plan.density_variant != 1)
-      k_density_wave<double, double>
-          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
-    else
+    if (plan.density_variant == 1) {
       k_density_mfma<double, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
+    } else {
+      if (blk)
+        k_density_blk<double, double>
+            <<<a.nleaves, 256, blk_smem, s>>>(a, static_cast<const double *>(ao), a.D2);
+      if (wave_too)
+        k_density_wave<double, double>
+            <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, wave_min);
+    }
   } else if (precision == 3) {
-    if (blk)
-      k_density_blk<float, double>
-          <<<a.nleaves, 256, blk_smem, s>>>(a, static_cast<const float *>(ao), a.D2);
-    else if (plan.density_variant != 1)
-      k_density_wave<float, double>
-          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2);
-    else
+    if (plan.density_variant != 1) {
+      if (blk)
+        k_density_blk<float, double>
+            <<<a.nleaves, 256, blk_smem, s>>>(a, static_cast<const float *>(ao), a.D2);
+      if (wave_too)
+        k_density_wave<float, double>
+            <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, wave_min);
+    } else
       k_density_mfma<float, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2);
   } else if (precision == 2) {
@@ -1100,7 +1307,9 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
   } else {
     const long long nn = a.mtot;
     const int nb = (int)std::max<long long>(1, std::min<long long>((nn + 255) / 256, 4096));
-    if (plan.density_variant == 0) {
+    // FP32 mode: the compensated-density kernel unless the plain per-wavefront
+    // (variant 3) or LDS-shared (variant 1) kernel is asked for
+    if (plan.density_variant == 0 || plan.density_variant == 2) {
       k_split_density<<<nb, 256, 0, s>>>(a.D2, plan.d2f.get(),
                                          reinterpret_cast<__bf16 *>(plan.d2l.get()), nn);
       check_launch("k_split_density");
@@ -1111,9 +1320,9 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
     }
     k_to_float<<<nb, 256, 0, s>>>(a.D2, plan.d2f.get(), nn);
     check_launch("k_to_float");
-    if (plan.density_variant == 2)
+    if (plan.density_variant == 3)
       k_density_wave<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
-                                                             plan.d2f.get());
+                                                             plan.d2f.get(), 0);
     else
       k_density_mfma<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
                                                              plan.d2f.get());
@@ -1121,21 +1330,47 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
   check_launch("k_density_mfma");
 }
 
+void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s) {
+  if (a.ntpairs == 0) return;
+  k_vxc_reduce<<<a.ntpairs, 256, 0, s>>>(a, V, kSymMaxMp);
+  check_launch("k_vxc_reduce");
+}
+
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s) {
   if (a.nleaves == 0) return;
-  (void)plan;
-  if (precision == 0)
-    k_vxc_mfma<double, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
-        a, static_cast<const double *>(ao));
-  else if (precision == 3)
-    k_vxc_mfma<float, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
-        a, static_cast<const float *>(ao));
-  else if (precision == 2)
+  if (precision == 2) {
     k_vxc_bf16<<<a.nleaves, 256, vxc_bf16_smem(), s>>>(a, static_cast<const float *>(ao));
-  else
-    k_vxc_mfma<float, float, 64><<<a.nleaves, 256, vxc_smem(sizeof(float)), s>>>(
-        a, static_cast<const float *>(ao));
+    check_launch("k_vxc_bf16");
+    return;
+  }
+  // symmetric tile-pair kernel for leaves of up to kSymMaxMp AOs, the general
+  // kernel for the rest (each skips the other's leaves)
+  const bool sym = plan.vxc_variant == 0 || plan.vxc_variant == 2;
+  const int min_mp = sym ? kSymMaxMp + 1 : 0;
+  const int smp = std::min(plan.max_mp, kSymMaxMp);
+  if (precision == 0) {
+    if (sym)
+      k_vxc_sym<double, double><<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp), s>>>(
+          a, static_cast<const double *>(ao));
+    if (!sym || plan.max_mp > kSymMaxMp)
+      k_vxc_mfma<double, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
+          a, static_cast<const double *>(ao), min_mp);
+  } else if (precision == 3) {
+    if (sym)
+      k_vxc_sym<float, double><<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp), s>>>(
+          a, static_cast<const float *>(ao));
+    if (!sym || plan.max_mp > kSymMaxMp)
+      k_vxc_mfma<float, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
+          a, static_cast<const float *>(ao), min_mp);
+  } else {
+    if (sym)
+      k_vxc_sym<float, float><<<a.nleaves, 256, vxc_sym_smem(sizeof(float), smp), s>>>(
+          a, static_cast<const float *>(ao));
+    if (!sym || plan.max_mp > kSymMaxMp)
+      k_vxc_mfma<float, float, 64><<<a.nleaves, 256, vxc_smem(sizeof(float)), s>>>(
+          a, static_cast<const float *>(ao), min_mp);
+  }
   check_launch("k_vxc_mfma");
 }
 

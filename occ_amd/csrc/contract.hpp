@@ -16,12 +16,18 @@ struct ContractPlan {
   bool use_mfma = false;   // false: the FP64 reference kernels of kernels.hip
   int max_np = 0;          // largest padded leaf (points)
   int max_mp = 0;          // largest padded leaf (AOs)
-  int density_variant = 0; // 0 per-wavefront, 1 LDS-shared D tiles (FP32 mode:
-                           // 2 per-wavefront without the compensated density)
+  int density_variant = 2; // 0 LDS-staged Phi, 1 LDS-shared D tiles, 2 per-wavefront
+                           // (FP32 mode: 0/2 compensated density, 3 plain per-wavefront)
+  int vxc_variant = 0;     // 0 symmetric tile pairs (small leaves), 1 general kernel,
+                           // 2 as 0 but two-pass (stores + tile-pair reduction)
+  bool two_pass = false;
   DevBuf<float> d2f;       // FP32 rounding of the doubled density (FP32 mode)
   DevBuf<uint16_t> d2l;    // BF16 rounding of the remainder (FP32 mode)
   size_t bytes = 0;
 };
+
+// Vxc kernel selection (OCCXC_VXC_VARIANT, default 0)
+int vxc_variant_setting();
 
 ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
                                 int precision, hipStream_t s);
@@ -29,5 +35,8 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
                  hipStream_t s);
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s);
+// two-pass Vxc: V = sum over leaves of the stored K_loc blocks (+ transposes),
+// one workgroup per 32 x 32 tile pair, fixed summation order
+void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s);
 
 } // namespace occxc

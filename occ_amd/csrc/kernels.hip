@@ -599,6 +599,13 @@ void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s) {
   check_launch();
 }
 
+void launch_finalize_scalars(const MolDev *mols, int nmol, const double *leaf_part,
+                             double *result, hipStream_t s) {
+  if (nmol == 0) return;
+  k_finalize_scalars<<<nmol, 256, 0, s>>>(leaf_part, mols, result);
+  check_launch();
+}
+
 void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol, int maxnbf,
                      const double *leaf_part, double *result, hipStream_t s) {
   if (nmol == 0) return;

@@ -35,6 +35,27 @@ struct LeafDev {
   int map_off;  // offset into ao_map (mp entries; padding entries map to AO 0)
   int mol;      // molecule of this leaf (index into BuildArgs::mols)
   long long ao_off;  // element offset of this leaf's AO block [ncomp][mp][np]
+  long long kl_off;  // element offset of this leaf's K_loc block (two-pass Vxc)
+};
+
+// Two-pass Vxc: one global 32 x 32 AO tile pair (TI <= TJ) of a molecule and
+// the leaves whose AOs touch both tiles.
+struct TilePairDev {
+  long long list_off;
+  int nlist;
+  int mol;
+  int ti, tj;
+};
+
+// One leaf of a tile pair's list.  The leaf's AOs in tile TI are a sorted
+// subset of its 32 AOs: bit k of mask_a is set when AO 32 TI + k is present,
+// and the present ones sit at local indices start_a, start_a + 1, ...
+struct TileEntry {
+  long long kl_off;
+  int mp;
+  int leaf;
+  int start_a, start_b;
+  unsigned mask_a, mask_b;
 };
 
 // One molecule of a context: its nbf x nbf blocks (D2, K, Vxc) start at moff
@@ -73,6 +94,10 @@ struct BuildArgs {
   double *coef;         // [4][ptot] per-point Vxc coefficients
   double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
   int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
+  double *kloc;         // two-pass Vxc: per-leaf K_loc blocks (null: atomics into K)
+  const TilePairDev *tpairs;
+  int ntpairs;
+  const TileEntry *tentries;
 };
 
 // host-callable launchers (kernels.hip)
@@ -86,6 +111,9 @@ void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32,
 void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s);
 void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol,
                      int maxnbf, const double *leaf_part, double *result, hipStream_t s);
+// {Exc, N_elec} per molecule only (the two-pass Vxc writes Vxc itself)
+void launch_finalize_scalars(const MolDev *mols, int nmol, const double *leaf_part,
+                             double *result, hipStream_t s);
 void launch_eval_basis_flat(const ShellDev *shells, int nshell,
                             const double *exps, const double *coefs,
                             const double *xyz, long long npts, int deriv,

@@ -290,3 +290,51 @@ def test_graph_replay_matches_direct_builds():
             assert np.abs(V - direct[k][0]).max() <= 1e-13
             assert abs(r[0] - direct[k][1][0]) <= 1e-12 and abs(r[1] - direct[k][1][1]) <= 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_vxc_variants_agree(monkeypatch, precision):
+    """The symmetric tile-pair Vxc kernel (upper tiles only), the general one
+    and the two-pass form (stored blocks + tile-pair reduction) agree, and the
+    FP64 result matches the oracle."""
+    atoms = molecules.water_cluster(3, seed=8)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 12, 71)
+    out = []
+    for v in ("0", "1", "2"):
+        monkeypatch.setenv("OCCXC_VXC_VARIANT", v)
+        dft = DFT("pbe", basis, pts, wts, precision=precision)
+        out.append((dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density()))
+        if v == "2":
+            # the two-pass form sums in a fixed order: builds repeat bit for bit
+            assert np.array_equal(dft.compute_vxc(D), out[-1][0])
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    assert np.abs(out[0][0] - out[1][0]).max() <= tol
+    assert np.abs(out[0][0] - out[2][0]).max() <= tol
+    assert np.abs(out[2][0] - out[2][0].T).max() == 0.0
+    assert np.abs(out[0][0] - out[0][0].T).max() == 0.0
+    if precision == "fp64":
+        K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(basis.flat()), pts, wts,
+                                      [101, 130], D)
+        assert np.abs(out[0][0] - K).max() <= TOL_V
+        assert abs(out[0][1] - exc) <= TOL_E and abs(out[0][2] - nel) <= TOL_N
+
+
+def test_two_pass_vxc_batch_and_large_leaves(monkeypatch):
+    """Two-pass Vxc on a batch context (several molecules, one reduction
+    launch) and on a system whose leaves exceed the symmetric kernel's size
+    (general-kernel blocks in the same reduction)."""
+    from occ_amd.dft import XCBatch
+    monkeypatch.setenv("OCCXC_VXC_VARIANT", "2")
+    mols = [molecules.water(), molecules.water_cluster(10, seed=2)]
+    bases = [AOBasis.load(m, "def2-svp", pure=True) for m in mols]
+    grids = [_grid(m, 21, 50) for m in mols]
+    Ds = [random_density(b.nbf, 6, 80 + i) for i, b in enumerate(bases)]
+    batch = XCBatch(bases, grids, [101, 130])
+    Vs, excs, nels = batch.build(Ds)
+    for i, (b, (pts, w)) in enumerate(zip(bases, grids)):
+        K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(b.flat()), pts, w,
+                                      [101, 130], Ds[i])
+        assert abs(excs[i] - exc) <= TOL_E and abs(nels[i] - nel) <= TOL_N
+        assert np.abs(Vs[i] - K).max() <= TOL_V
