@@ -474,6 +474,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     // a run of its (sorted) local AOs, recorded as a start and a bit mask
     std::vector<TilePairDev> tpairs;
     std::vector<TileEntry> tentries;
+    std::vector<TileChunkDev> tchunks;
     struct Touch { int tile, start; unsigned mask; };
     std::vector<Touch> touch;
     for (int mol = 0; mol < nmol; ++mol) {
@@ -496,7 +497,15 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
       for (int ti = 0; ti < nt; ++ti)
         for (int tj = ti; tj < nt; ++tj) {
           auto &v = plist[(size_t)ti * nt + tj];
-          tpairs.push_back(TilePairDev{(long long)tentries.size(), (int)v.size(), mol, ti, tj});
+          const long long off = (long long)tentries.size();
+          const int nchunk = ((int)v.size() + kTileChunk - 1) / kTileChunk;
+          for (int q = 0; q < nchunk; ++q)
+            tchunks.push_back(TileChunkDev{off + (long long)q * kTileChunk,
+                                           off + std::min<long long>((long long)(q + 1) * kTileChunk,
+                                                                     (long long)v.size()),
+                                           (int)tpairs.size(), 0});
+          tpairs.push_back(TilePairDev{off, (int)v.size(), mol, ti, tj,
+                                       (int)tchunks.size() - nchunk, nchunk});
           tentries.insert(tentries.end(), v.begin(), v.end());
         }
     }
@@ -509,11 +518,17 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     a.tpairs = c->d_tpairs.get();
     a.ntpairs = (int)tpairs.size();
     a.tentries = c->d_tentries.get();
+    c->d_tchunks.alloc(std::max<size_t>(tchunks.size(), 1));
+    c->d_tchunks.upload(tchunks.data(), tchunks.size(), c->stream);
+    c->d_tpartial.alloc(std::max<size_t>(tchunks.size(), 1) * 1024);
+    a.tchunks = c->d_tchunks.get();
+    a.ntchunks = (int)tchunks.size();
+    a.tpartial = c->d_tpartial.get();
   }
   c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
                     c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                     c->plan.bytes + c->d_kloc.bytes() + c->d_tpairs.bytes() +
-                    c->d_tentries.bytes();
+                    c->d_tentries.bytes() + c->d_tchunks.bytes() + c->d_tpartial.bytes();
 
   launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
   OCCXC_HIP(hipStreamSynchronize(c->stream));

@@ -42,6 +42,8 @@ struct occxc_context {
   occxc::DevBuf<double> d_kloc;
   occxc::DevBuf<occxc::TilePairDev> d_tpairs;
   occxc::DevBuf<occxc::TileEntry> d_tentries;
+  occxc::DevBuf<occxc::TileChunkDev> d_tchunks;
+  occxc::DevBuf<double> d_tpartial;
   long long kloc_elems = 0;
   occxc::BuildArgs args{};
   // stage timing: pending event sets, summed on read-out

@@ -1161,38 +1161,55 @@ __device__ __forceinline__ double kloc_pair(const double *S, int mp, bool sym, i
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_vxc_reduce(BuildArgs a, double *__restrict__ V,
-                                                    int sym_max_mp) {
-  const TilePairDev P = a.tpairs[blockIdx.x];
-  const MolDev MD = a.mols[P.mol];
-  const int t = threadIdx.x;
-  const int j = t & 31, i0 = t >> 5;  // elements (i0 + 8 k, j), k = 0..3
-  const TileEntry *list = a.tentries + P.list_off;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int e = 0; e < P.nlist; ++e) {
-    const TileEntry E = list[e];
+// Pass 2a: one workgroup per chunk of a tile pair's list; wavefront w sums
+// the chunk's entries w, w + 4, ... (lanes over the 32 x 32 block, 16
+// elements each), and the four wavefront sums are added in a fixed order into
+// the chunk's partial block.
+__global__ __launch_bounds__(256) void k_vxc_reduce_chunk(BuildArgs a, int sym_max_mp) {
+  __shared__ double s_acc[4][1024];
+  const TileChunkDev C = a.tchunks[blockIdx.x];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int j = lane & 31, i0 = lane >> 5;  // elements (i0 + 2 k, j), k = 0..15
+  double acc[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+  for (long long e = C.e0 + wave; e < C.e1; e += 4) {
+    const TileEntry E = a.tentries[e];
     if (!a.leaf_active[E.leaf]) continue;
     if (!((E.mask_b >> j) & 1u)) continue;
     const double *S = a.kloc + E.kl_off;
     const bool sym = E.mp <= sym_max_mp;
     const int lj = E.start_b + __popc(E.mask_b & ((1u << j) - 1u));
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int i = i0 + 8 * k;
+    for (int k = 0; k < 16; ++k) {
+      const int i = i0 + 2 * k;
       if ((E.mask_a >> i) & 1u) {
         const int li = E.start_a + __popc(E.mask_a & ((1u << i) - 1u));
         acc[k] += kloc_pair(S, E.mp, sym, li, lj);
       }
     }
   }
-  const long long gj = 32LL * P.tj + j;
-  if (gj >= MD.nbf) return;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const long long gi = 32LL * P.ti + i0 + 8 * k;
-    if (gi >= MD.nbf) continue;
-    V[MD.moff + gi + gj * MD.nbf] = acc[k];
-    if (P.ti != P.tj) V[MD.moff + gj + gi * MD.nbf] = acc[k];
+  for (int k = 0; k < 16; ++k) s_acc[wave][(i0 + 2 * k) * 32 + j] = acc[k];
+  __syncthreads();
+  double *out = a.tpartial + (long long)blockIdx.x * 1024;
+  for (int u = t; u < 1024; u += 256)
+    out[u] = ((s_acc[0][u] + s_acc[1][u]) + s_acc[2][u]) + s_acc[3][u];
+}
+
+// Pass 2b: one workgroup per tile pair sums its chunks' partial blocks in
+// chunk order and writes V(i, j) and V(j, i).
+__global__ __launch_bounds__(256) void k_vxc_reduce_final(BuildArgs a, double *__restrict__ V) {
+  const TilePairDev P = a.tpairs[blockIdx.x];
+  const MolDev MD = a.mols[P.mol];
+  for (int u = threadIdx.x; u < 1024; u += 256) {
+    double v = 0.0;
+    for (int q = 0; q < P.nchunk; ++q) v += a.tpartial[(long long)(P.chunk_off + q) * 1024 + u];
+    const long long gi = 32LL * P.ti + (u >> 5), gj = 32LL * P.tj + (u & 31);
+    if (gi >= MD.nbf || gj >= MD.nbf) continue;
+    V[MD.moff + gi + gj * MD.nbf] = v;
+    if (P.ti != P.tj) V[MD.moff + gj + gi * MD.nbf] = v;
   }
 }
 
@@ -1332,8 +1349,12 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
 
 void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s) {
   if (a.ntpairs == 0) return;
-  k_vxc_reduce<<<a.ntpairs, 256, 0, s>>>(a, V, kSymMaxMp);
-  check_launch("k_vxc_reduce");
+  if (a.ntchunks > 0) {
+    k_vxc_reduce_chunk<<<a.ntchunks, 256, 0, s>>>(a, kSymMaxMp);
+    check_launch("k_vxc_reduce_chunk");
+  }
+  k_vxc_reduce_final<<<a.ntpairs, 256, 0, s>>>(a, V);
+  check_launch("k_vxc_reduce_final");
 }
 
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,

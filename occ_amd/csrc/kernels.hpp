@@ -45,7 +45,18 @@ struct TilePairDev {
   int nlist;
   int mol;
   int ti, tj;
+  int chunk_off;  // its chunks of at most kTileChunk list entries
+  int nchunk;
 };
+
+// A run of at most kTileChunk entries of one tile pair's list, summed by one
+// workgroup into a 32 x 32 partial block.
+struct TileChunkDev {
+  long long e0, e1;
+  int pair;
+  int pad_;
+};
+constexpr int kTileChunk = 64;
 
 // One leaf of a tile pair's list.  The leaf's AOs in tile TI are a sorted
 // subset of its 32 AOs: bit k of mask_a is set when AO 32 TI + k is present,
@@ -98,6 +109,9 @@ struct BuildArgs {
   const TilePairDev *tpairs;
   int ntpairs;
   const TileEntry *tentries;
+  const TileChunkDev *tchunks;
+  int ntchunks;
+  double *tpartial;     // [ntchunks][32 * 32] partial blocks
 };
 
 // host-callable launchers (kernels.hip)
