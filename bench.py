@@ -284,7 +284,7 @@ def run_batch(args):
     results = {}
     for mode in [m for m in args.modes.split(",") if m]:
         t0 = time.time()
-        est = [estimate_batch_bytes([b], [g], ids, precision=mode) for b, g in zip(bases, grids)]
+        est = list(estimate_batch_bytes(bases, grids, ids, precision=mode))
         t_est = time.time() - t0
         chunks, cur, cur_b = [], [], 0.0
         for i, e in enumerate(est):

@@ -150,10 +150,11 @@ occxc_status occxc_batch_create(int nmol, const occxc_basis *bases,
 occxc_status occxc_batch_build(occxc_context *ctx, const double *const *D,
                                double *const *Vxc, double *exc, double *nelec);
 
-/* Device bytes a batch context over these molecules would hold (AO values,
- * matrices, per-point data), from the same hierarchy and screening as
- * occxc_batch_create but without the large allocations: for packing many
- * molecules into contexts that fit device memory. */
+/* Device bytes each molecule adds to a batch context (its AO values,
+ * matrices, per-point data and leaf tables): bytes[m] for m < nmol, from the
+ * same hierarchy and screening as occxc_batch_create but without the large
+ * allocations.  For packing many molecules into contexts that fit device
+ * memory; molecules are set up on several host threads. */
 occxc_status occxc_batch_estimate_bytes(int nmol, const occxc_basis *bases,
                                         const occxc_grid *grids, const occxc_functional *func,
                                         const occxc_options *opt, double *bytes);

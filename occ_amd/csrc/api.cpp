@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -35,11 +36,15 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 void clear_last_error() { g_last_error.clear(); }
 
+// Set on threads that already run one task of an outer parallel loop: their
+// inner loops stay serial.
+static thread_local bool t_serial = false;
+
 // Split [0, n) over host threads (set-up work only).
 static void parallel_for(long long n, const std::function<void(long long, long long)> &fn) {
   unsigned hw = std::thread::hardware_concurrency();
   int nt = (int)std::min<unsigned>(hw ? hw : 1, 16);
-  if (n < 4096 || nt <= 1) {
+  if (n < 4096 || nt <= 1 || t_serial) {
     fn(0, n);
     return;
   }
@@ -51,6 +56,31 @@ static void parallel_for(long long n, const std::function<void(long long, long l
     th.emplace_back(fn, lo, hi);
   }
   for (auto &t : th) t.join();
+}
+
+// Run fn(i) for i in [0, n) on up to 16 host threads, tasks taken in order.
+static void parallel_each(int n, const std::function<void(int)> &fn) {
+  unsigned hw = std::thread::hardware_concurrency();
+  const int nt = (int)std::min<unsigned>(std::min<unsigned>(hw ? hw : 1, 16), (unsigned)n);
+  if (nt <= 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  std::vector<std::exception_ptr> err(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      t_serial = true;
+      try {
+        for (int i = next++; i < n; i = next++) fn(i);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  for (auto &x : th) x.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 static int shell_size(int l, int pure) { return pure ? 2 * l + 1 : (l + 1) * (l + 2) / 2; }
@@ -302,12 +332,16 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   std::vector<double> px, py, pz, pw;
   long long pt_off = 0, ao_off = 0;
   double msum = 0.0;
-  for (int mol = 0; mol < nmol; ++mol) {
+  // per molecule: leaves (Morton hierarchy when screened, else 128-point
+  // chunks in input order) and their significant shells (dft.h:314-320);
+  // molecules are independent, so a batch builds them on several threads
+  std::vector<Hierarchy> hier(nmol);
+  std::vector<std::vector<std::vector<int>>> sigs(nmol);
+  parallel_each(nmol, [&](int mol) {
     const occxc_grid *grid = &grids[mol];
     const HostBasis &hb = hbs[mol];
     const int sb = shell_base[mol];
-    // leaves: Morton hierarchy (screened) or 128-point chunks in input order
-    Hierarchy h;
+    Hierarchy &h = hier[mol];
     const long long n = grid->npts;
     if (screened) {
       h = build_hierarchy(n, grid->points, grid->weights);
@@ -322,8 +356,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
       }
     }
     const long long nl_all = (long long)h.off.size();
-    // significant shells per leaf (dft.h:314-320)
-    std::vector<std::vector<int>> sig(nl_all);
+    auto &sig = sigs[mol];
+    sig.assign(nl_all, {});
     parallel_for(nl_all, [&](long long lo, long long hi) {
       for (long long li = lo; li < hi; ++li) {
         if (h.cnt[li] == 0) continue;
@@ -341,6 +375,13 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
         }
       }
     });
+  });
+  for (int mol = 0; mol < nmol; ++mol) {
+    const HostBasis &hb = hbs[mol];
+    const int sb = shell_base[mol];
+    const Hierarchy &h = hier[mol];
+    const auto &sig = sigs[mol];
+    const long long nl_all = (long long)h.off.size();
     mols[mol].leaf_begin = (int)c->leaves.size();
     for (long long li = 0; li < nl_all; ++li) {
       if (sig[li].empty()) continue;
@@ -413,6 +454,12 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   // device memory the context will hold: AO values, the nbf^2 matrices
   // (D, D2, K, V and the FP32 copies), per-point data and the leaf tables
   const bool two_pass = vxc_variant_setting() == 2;
+  c->mol_bytes.assign(nmol, 0.0);
+  for (int m = 0; m < nmol; ++m) c->mol_bytes[m] = 6.0 * 8.0 * c->mol_nbf[m] * c->mol_nbf[m];
+  for (const LeafDev &L : c->leaves)
+    c->mol_bytes[L.mol] += (double)c->ncomp * L.mp * L.np * c->ao_elem_size + 64.0 * L.np +
+                           4.0 * (L.mp + L.nsig) + sizeof(LeafDev) +
+                           (two_pass ? 8.0 * L.mp * L.mp : 0.0);
   const size_t planned = ao_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
                          c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                          (two_pass ? (size_t)c->kloc_elems * 8 : 0);
@@ -709,7 +756,7 @@ occxc_status occxc_batch_estimate_bytes(int nmol, const occxc_basis *bases,
       }
       if (e.status != OCCXC_ERR_OUT_OF_MEMORY || c->device_bytes == 0) throw;
     }
-    *bytes = (double)c->device_bytes;
+    for (int m = 0; m < nmol; ++m) bytes[m] = c->mol_bytes[m];
   });
 }
 

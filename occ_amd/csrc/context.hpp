@@ -53,6 +53,7 @@ struct occxc_context {
   long long stage_n[OCCXC_NSTAGES] = {0};
   bool ao_ready = false;
   size_t device_bytes = 0;
+  std::vector<double> mol_bytes;  // device bytes each molecule adds
   // captured build graph and the (D, Vxc, result, stream) it was captured on
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;

@@ -398,6 +398,108 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
   density_epilogue(a, L, leaf, s_q, s_red);
 }
 
+// Register-lean per-wavefront variant: as k_density_wave, but each 32-AO
+// chunk's contributions to rho and grad rho are reduced and added into LDS at
+// once instead of being kept in registers across chunks (three waves per
+// SIMD instead of two).
+template <typename S, typename T>
+__global__ __launch_bounds__(256, 3) void k_density_lean(BuildArgs a, const S *__restrict__ ao,
+                                                      const T *__restrict__ D2t, int min_mp) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  __shared__ double s_q[4][kMaxLeafPts];
+  __shared__ double s_red[4];
+  __shared__ int s_col[kMapCap];
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  const LeafDev L = a.leaves[leaf];
+  (void)min_mp;
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const S *phi = ao + L.ao_off;
+  const T *Dm = D2t + MD.moff;
+  const long long cs = (long long)L.mp * L.np;
+  const int *map = a.ao_map + L.map_off;
+  const bool deriv = a.deriv != 0;
+  for (int i = threadIdx.x; i < L.mp; i += blockDim.x) s_col[i] = map[i] * MD.nbf;
+  for (int i = threadIdx.x; i < 4 * kMaxLeafPts; i += blockDim.x) (&s_q[0][0])[i] = 0.0;
+  __syncthreads();
+  const int nblk = L.np >> 5;
+  for (int blk = wave; blk < nblk; blk += 4) {
+    const int p0 = blk * 32;
+    for (int c = 0; c < L.mp; c += 32) {
+      acc_t acc[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = M::zero();
+      const int gn0 = map[c + l16], gn1 = map[c + 16 + l16];
+      T ca[4][2], cb[4][2];
+      auto load = [&](int mc, T (&xa)[4][2], T (&xb)[4][2]) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int mu = mc + 4 * g + s4;
+          const S *row = phi + (long long)mu * L.np + p0 + l16;
+          const int col = s_col[mu];
+          xa[s4][0] = static_cast<T>(row[0]);
+          xa[s4][1] = static_cast<T>(row[16]);
+          xb[s4][0] = Dm[col + gn0];
+          xb[s4][1] = Dm[col + gn1];
+        }
+      };
+      // no explicit prefetch: three waves per SIMD cover the load latency
+      for (int mc = 0; mc < L.mp; mc += 16) {
+        load(mc, ca, cb);
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          acc[0][0] = M::mma(ca[s4][0], cb[s4][0], acc[0][0]);
+          acc[0][1] = M::mma(ca[s4][0], cb[s4][1], acc[0][1]);
+          acc[1][0] = M::mma(ca[s4][1], cb[s4][0], acc[1][0]);
+          acc[1][1] = M::mma(ca[s4][1], cb[s4][1], acc[1][1]);
+        }
+      }
+      // this chunk's contributions, reduced over the 16 nu lanes at once and
+      // added into the wave's own points (no register accumulators kept)
+#pragma unroll
+      for (int pt = 0; pt < 2; ++pt) {
+        double q4[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) q4[r][q] = 0.0;
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+          const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16;
+          const auto f = M::template rows4<S>(phi + off, g);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) q4[r][0] += (double)f[r] * (double)acc[pt][nt][r];
+          if (deriv) {
+            const auto fx = M::template rows4<S>(phi + cs + off, g);
+            const auto fy = M::template rows4<S>(phi + 2 * cs + off, g);
+            const auto fz = M::template rows4<S>(phi + 3 * cs + off, g);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const double t = 2.0 * (double)acc[pt][nt][r];
+              q4[r][1] += (double)fx[r] * t;
+              q4[r][2] += (double)fy[r] * t;
+              q4[r][3] += (double)fz[r] * t;
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double v = sum16(q4[r][q]);
+            if (l16 == 0) s_q[q][p0 + pt * 16 + M::row(g, r)] += v;
+          }
+      }
+    }
+  }
+  __syncthreads();
+  density_epilogue(a, L, leaf, s_q, s_red);
+}
+
 // LDS-staged variant: the leaf's Phi is staged in LDS kBlkPts points at a
 // time (all mp AO rows, converted to T), so Phi is read from HBM once; the
 // four wavefronts share it as the A operand and split the 16-wide nu tiles
@@ -1296,6 +1398,16 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
   const int wave_min = blk ? kBlkMaxMp + 1 : 0;
   const bool wave_too = !blk || plan.max_mp > kBlkMaxMp;
   const size_t blk_smem = density_blk_smem(std::min(plan.max_mp, kBlkMaxMp));
+  if (plan.density_variant == 4 && (precision == 0 || precision == 3)) {
+    if (precision == 0)
+      k_density_lean<double, double>
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, 0);
+    else
+      k_density_lean<float, double>
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, 0);
+    check_launch("k_density_lean");
+    return;
+  }
   if (precision == 0) {
     if (plan.density_variant == 1) {
       k_density_mfma<double, double>

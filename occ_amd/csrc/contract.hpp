@@ -16,7 +16,8 @@ struct ContractPlan {
   bool use_mfma = false;   // false: the FP64 reference kernels of kernels.hip
   int max_np = 0;          // largest padded leaf (points)
   int max_mp = 0;          // largest padded leaf (AOs)
-  int density_variant = 2; // 0 LDS-staged Phi, 1 LDS-shared D tiles, 2 per-wavefront
+  int density_variant = 2; // 0 LDS-staged Phi, 1 LDS-shared D tiles, 2 per-wavefront,
+                           // 4 per-wavefront with per-chunk LDS reduction (FP64 data)
                            // (FP32 mode: 0/2 compensated density, 3 plain per-wavefront)
   int vxc_variant = 0;     // 0 symmetric tile pairs (small leaves), 1 general kernel,
                            // 2 as 0 but two-pass (stores + tile-pair reduction)

@@ -140,8 +140,8 @@ class XCContext:
 
 
 def estimate_batch_bytes(bases, grids, funcs, precision="fp64", screen=True, device=0):
-    """Device bytes an XCBatch over these molecules would hold
-    (occxc_batch_estimate_bytes): hierarchy and screening only."""
+    """Device bytes each molecule adds to an XCBatch (occxc_batch_estimate_bytes):
+    hierarchy and screening only.  Returns an array, one entry per molecule."""
     n = len(bases)
     keep = []
     cb = (_lib.CBasis * n)()
@@ -159,10 +159,10 @@ def estimate_batch_bytes(bases, grids, funcs, precision="fp64", screen=True, dev
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.screen = int(bool(screen))
     o.device = device
-    out = ctypes.c_double()
+    out = np.zeros(n)
     _lib.check(_lib.lib().occxc_batch_estimate_bytes(n, cb, cg, ctypes.byref(func),
-                                                     ctypes.byref(o), ctypes.byref(out)))
-    return out.value
+                                                     ctypes.byref(o), _lib.dptr(out)))
+    return out
 
 
 class XCBatch:

@@ -249,11 +249,11 @@ def test_density_variants_agree(monkeypatch):
     pts, wts = _grid(atoms)
     D = random_density(basis.nbf, 10, 51)
     out = []
-    for v in ("0", "1", "2"):
+    for v in ("0", "1", "2", "4"):
         monkeypatch.setenv("OCCXC_DENSITY_VARIANT", v)
         dft = DFT("pbe", basis, pts, wts)
         out.append((dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density()))
-    for k in (1, 2):
+    for k in (1, 2, 3):
         assert abs(out[0][1] - out[k][1]) <= 1e-11 and abs(out[0][2] - out[k][2]) <= 1e-11
         assert np.abs(out[0][0] - out[k][0]).max() <= 1e-12
 
@@ -338,3 +338,23 @@ def test_two_pass_vxc_batch_and_large_leaves(monkeypatch):
                                       [101, 130], Ds[i])
         assert abs(excs[i] - exc) <= TOL_E and abs(nels[i] - nel) <= TOL_N
         assert np.abs(Vs[i] - K).max() <= TOL_V
+
+
+def test_batch_estimate_bytes_tracks_context_footprint():
+    """Per-molecule footprints from occxc_batch_estimate_bytes (no large
+    allocations) against the device bytes the contexts really hold."""
+    from occ_amd.dft import XCBatch, XCContext, estimate_batch_bytes
+    mols = [molecules.water(), molecules.alkane(4), molecules.water_cluster(3, seed=5)]
+    bases = [AOBasis.load(m, "def2-svp", pure=True) for m in mols]
+    grids = [_grid(m, 21, 50) for m in mols]
+    est = estimate_batch_bytes(bases, grids, [101, 130])
+    assert est.shape == (3,) and np.all(est > 0)
+    for b, (pts, w), e in zip(bases, grids, est):
+        ctx = XCContext(b, pts, w, [101, 130])
+        real = ctx.info()["device_bytes"]
+        ctx.close()
+        assert 0.5 * real <= e <= 1.5 * real, (e, real)
+    batch = XCBatch(bases, grids, [101, 130])
+    real = batch.info()["device_bytes"]
+    batch.close()
+    assert 0.5 * real <= est.sum() <= 1.5 * real
