@@ -303,6 +303,11 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
   const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
+  // A-operand point of this lane within a 16-point tile: the f64 instruction
+  // returns rows g, g+4, g+8, g+12 to lane group g, so feeding point
+  // 4(i%4) + i/4 as row i gives each lane group 4 consecutive points (one
+  // vector load in the epilogue); the f32 instruction needs no permutation
+  const int pa = sizeof(T) == 8 ? 4 * (l16 & 3) + (l16 >> 2) : l16;
   const S *phi = ao + L.ao_off;
   const T *Dm = D2t + MD.moff;
   const long long cs = (long long)L.mp * L.np;
@@ -332,7 +337,7 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
           const int mu = mc + 4 * g + s4;
-          const S *row = phi + (long long)mu * L.np + p0 + l16;
+          const S *row = phi + (long long)mu * L.np + p0 + pa;
           const int col = s_col[mu];
           xa[s4][0] = static_cast<T>(row[0]);
           xa[s4][1] = static_cast<T>(row[16]);
@@ -367,13 +372,13 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
           const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16;
-          const auto f = M::template rows4<S>(phi + off, g);
+          const auto f = load4_as<T>(phi + off + 4 * g);
 #pragma unroll
           for (int r = 0; r < 4; ++r) part[pt][r][0] += (double)f[r] * (double)acc[pt][nt][r];
           if (deriv) {
-            const auto fx = M::template rows4<S>(phi + cs + off, g);
-            const auto fy = M::template rows4<S>(phi + 2 * cs + off, g);
-            const auto fz = M::template rows4<S>(phi + 3 * cs + off, g);
+            const auto fx = load4_as<T>(phi + cs + off + 4 * g);
+            const auto fy = load4_as<T>(phi + 2 * cs + off + 4 * g);
+            const auto fz = load4_as<T>(phi + 3 * cs + off + 4 * g);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const double t = 2.0 * (double)acc[pt][nt][r];
@@ -391,7 +396,7 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const double v = sum16(part[pt][r][q]);
-          if (l16 == 0) s_q[q][p0 + pt * 16 + M::row(g, r)] = v;
+          if (l16 == 0) s_q[q][p0 + pt * 16 + 4 * g + r] = v;
         }
   }
   __syncthreads();
@@ -416,6 +421,11 @@ __global__ __launch_bounds__(256, 3) void k_density_lean(BuildArgs a, const S *_
   const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
+  // A-operand point of this lane within a 16-point tile: the f64 instruction
+  // returns rows g, g+4, g+8, g+12 to lane group g, so feeding point
+  // 4(i%4) + i/4 as row i gives each lane group 4 consecutive points (one
+  // vector load in the epilogue); the f32 instruction needs no permutation
+  const int pa = sizeof(T) == 8 ? 4 * (l16 & 3) + (l16 >> 2) : l16;
   const S *phi = ao + L.ao_off;
   const T *Dm = D2t + MD.moff;
   const long long cs = (long long)L.mp * L.np;
@@ -439,7 +449,7 @@ __global__ __launch_bounds__(256, 3) void k_density_lean(BuildArgs a, const S *_
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
           const int mu = mc + 4 * g + s4;
-          const S *row = phi + (long long)mu * L.np + p0 + l16;
+          const S *row = phi + (long long)mu * L.np + p0 + pa;
           const int col = s_col[mu];
           xa[s4][0] = static_cast<T>(row[0]);
           xa[s4][1] = static_cast<T>(row[16]);
@@ -470,13 +480,13 @@ __global__ __launch_bounds__(256, 3) void k_density_lean(BuildArgs a, const S *_
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
           const long long off = (long long)(c + nt * 16 + l16) * L.np + p0 + pt * 16;
-          const auto f = M::template rows4<S>(phi + off, g);
+          const auto f = load4_as<T>(phi + off + 4 * g);
 #pragma unroll
           for (int r = 0; r < 4; ++r) q4[r][0] += (double)f[r] * (double)acc[pt][nt][r];
           if (deriv) {
-            const auto fx = M::template rows4<S>(phi + cs + off, g);
-            const auto fy = M::template rows4<S>(phi + 2 * cs + off, g);
-            const auto fz = M::template rows4<S>(phi + 3 * cs + off, g);
+            const auto fx = load4_as<T>(phi + cs + off + 4 * g);
+            const auto fy = load4_as<T>(phi + 2 * cs + off + 4 * g);
+            const auto fz = load4_as<T>(phi + 3 * cs + off + 4 * g);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const double t = 2.0 * (double)acc[pt][nt][r];
@@ -491,7 +501,7 @@ __global__ __launch_bounds__(256, 3) void k_density_lean(BuildArgs a, const S *_
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const double v = sum16(q4[r][q]);
-            if (l16 == 0) s_q[q][p0 + pt * 16 + M::row(g, r)] += v;
+            if (l16 == 0) s_q[q][p0 + pt * 16 + 4 * g + r] += v;
           }
       }
     }
