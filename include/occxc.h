@@ -136,6 +136,22 @@ occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc,
 occxc_status occxc_build_device(occxc_context *ctx, const double *d_D,
                                 double *d_Vxc, double *d_result, void *stream);
 
+/* XC contribution to the nuclear gradient, replacing
+ * DFT::compute_xc_gradient (include/occ/dft/dft.h:531-532) for a restricted
+ * density: compute_xc_gradient_impl<d, Restricted> (dft.h:598-749) with
+ * kernels::process_grid_block_gradient (src/dft/dft_gradient_kernels.cpp:
+ * 473-552).  D as occxc_build (nbf*nbf column-major, no factor 2; symmetric:
+ * the reference's density reads its upper triangle and its distribution both,
+ * which agree only for a symmetric D).  shell_atom[nshell] is the atom of
+ * each shell (AOBasis::shell_to_atom), natoms the atom count; grad receives
+ * the 3 x natoms column-major Mat3N (x, y, z of atom 0, then atom 1, ...).
+ * The gradient is that of Exc with the grid and weights held fixed, as in the
+ * reference.  The first call evaluates and keeps the AO derivative components
+ * the build does not need (the Hessian for GGA, the gradient for LDA).
+ * Single-molecule contexts only; repeats bit for bit. */
+occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
+                               const int *shell_atom, double *grad);
+
 /* --- batches: many molecules in one context, one launch per stage ----- */
 
 /* A context over nmol molecules (each its own basis and grid; shared
@@ -185,7 +201,8 @@ enum {
   OCCXC_STAGE_VXC = 2,       /* phi^T A per leaf into K */
   OCCXC_STAGE_FINALIZE = 3,  /* K + K^T, Exc / N_elec reduction */
   OCCXC_STAGE_AO = 4,        /* AO values of every leaf */
-  OCCXC_NSTAGES = 5
+  OCCXC_STAGE_GRADIENT = 5,  /* XC gradient contraction and per-atom reduction */
+  OCCXC_NSTAGES = 6
 };
 occxc_status occxc_context_set_timing(occxc_context *ctx, int enable);
 occxc_status occxc_context_get_timing(occxc_context *ctx, double *ms,

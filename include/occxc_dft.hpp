@@ -4,13 +4,17 @@
 //   Mat DFT::compute_vxc(const MolecularOrbitals &mo, const Mat &Schwarz)
 //                                             (include/occ/dft/dft.h:395-429)
 //   double DFT::exchange_correlation_energy() const        (dft.h:83)
+//   Mat3N DFT::compute_xc_gradient(const MolecularOrbitals &mo, ...)
+//                                                          (dft.h:531-532)
 //
 // XCBuild owns one occxc context (basis + grid + functionals resident on one
 // GPU).  compute_vxc takes the restricted density D (nbf x nbf, column-major,
 // no factor 2, OCC's convention of include/occ/qm/orb.h:64-66) and returns the
 // matrix OCC's compute_vxc returns; the Exc it computed is then available from
 // exchange_correlation_energy(), and N_elec (only logged by OCC) from
-// total_density().  Errors arrive as std::runtime_error, as in OCC.
+// total_density().  compute_xc_gradient returns the 3 x natoms column-major
+// XC gradient for the same D, given each shell's atom.  Errors arrive as
+// std::runtime_error, as in OCC.
 #pragma once
 
 #include <stdexcept>
@@ -44,6 +48,14 @@ public:
     std::vector<double> V(static_cast<size_t>(nbf_) * nbf_);
     check(occxc_build(ctx_, D, V.data(), &exc_, &nelec_));
     return V;
+  }
+
+  // 3 x natoms, column-major (x, y, z of each atom); shell_atom[nshell].
+  std::vector<double> compute_xc_gradient(const double *D, int natoms,
+                                          const std::vector<int> &shell_atom) {
+    std::vector<double> G(3 * static_cast<size_t>(natoms));
+    check(occxc_xc_gradient(ctx_, D, natoms, shell_atom.data(), G.data()));
+    return G;
   }
 
   double exchange_correlation_energy() const { return exc_; }

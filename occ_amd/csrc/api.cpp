@@ -294,6 +294,9 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   c->nmol = nmol;
   c->nsh = (int)shells.size();
   c->nbf = hbs[0].nbf;
+  if (nmol == 1)
+    for (int k = 0; k < hbs[0].nsh; ++k)
+      for (int q = 0; q < hbs[0].shells[k].size; ++q) c->bf_shell.push_back(k);
   c->npts = npts_total;
   std::vector<MolDev> mols(nmol);
   long long moff = 0;
@@ -646,6 +649,65 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
   });
 }
 
+// XC gradient set-up: the AO derivative components the cache lacks (once
+// per context) and the per-atom lists of (leaf, local AO) entries for this
+// shell-to-atom map (rebuilt when the map changes).
+void gradient_setup(occxc_context *c, int natoms, const int *shell_atom) {
+  if (!c->ao2_ready) {
+    const size_t n = c->ao_elems / c->ncomp * (c->deriv ? 6 : 3);
+    c->d_ao2.alloc(std::max<size_t>(n, 1));
+    c->d_gpart.alloc(std::max<size_t>(3 * c->d_ao_map.size(), 1));
+    c->device_bytes += c->d_ao2.bytes() + c->d_gpart.bytes();
+    StageTimer t{c, c->stream, {}};
+    t.run(OCCXC_STAGE_AO,
+          [&] { launch_ao_extra(c->args, c->d_ao2.get(), c->deriv != 0, c->stream); });
+    c->ao2_ready = true;
+  }
+  const bool same = c->grad_natoms == natoms && (int)c->grad_shell_atom.size() == c->nsh &&
+                    std::equal(shell_atom, shell_atom + c->nsh, c->grad_shell_atom.begin());
+  if (same) return;
+  std::vector<int> map(c->d_ao_map.size());
+  c->d_ao_map.download(map.data(), map.size(), c->stream);
+  OCCXC_HIP(hipStreamSynchronize(c->stream));
+  std::vector<int> off(natoms + 1, 0);
+  for (const LeafDev &L : c->leaves)
+    for (int mu = 0; mu < L.m; ++mu) ++off[shell_atom[c->bf_shell[map[L.map_off + mu]]] + 1];
+  for (int A = 0; A < natoms; ++A) off[A + 1] += off[A];
+  std::vector<int> ent(std::max(off[natoms], 1)), fill(off.begin(), off.end() - 1);
+  for (const LeafDev &L : c->leaves)
+    for (int mu = 0; mu < L.m; ++mu)
+      ent[fill[shell_atom[c->bf_shell[map[L.map_off + mu]]]]++] = L.map_off + mu;
+  c->d_gatom_off.alloc(off.size());
+  c->d_gatom_off.upload(off.data(), off.size(), c->stream);
+  c->d_gatom_ent.alloc(ent.size());
+  c->d_gatom_ent.upload(ent.data(), ent.size(), c->stream);
+  c->d_grad.alloc(3 * (size_t)natoms);
+  OCCXC_HIP(hipStreamSynchronize(c->stream));
+  c->grad_shell_atom.assign(shell_atom, shell_atom + c->nsh);
+  c->grad_natoms = natoms;
+}
+
+// One XC gradient on stream s: D2, the density stage (coefficients, leaf
+// skip), then the gradient contraction and the per-atom reduction.
+void run_gradient(occxc_context *c, const double *d_D, hipStream_t s) {
+  const BuildArgs &a = c->args;
+  StageTimer t{c, s, {}};
+  t.run(OCCXC_STAGE_PREPARE, [&] {
+    launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
+  });
+  t.run(OCCXC_STAGE_DENSITY, [&] {
+    if (c->plan.use_mfma)
+      run_density(c->plan, a, c->d_ao.get(), c->opt.precision, s);
+    else
+      launch_density_functional(a, c->d_ao.get(), c->opt.precision != OCCXC_PREC_FP64, s);
+  });
+  t.run(OCCXC_STAGE_GRADIENT, [&] {
+    launch_xc_gradient(a, c->d_ao.get(), c->opt.precision != OCCXC_PREC_FP64, c->d_ao2.get(),
+                       c->d_gpart.get(), c->grad_natoms, c->d_gatom_off.get(),
+                       c->d_gatom_ent.get(), c->d_grad.get(), s);
+  });
+}
+
 void drop_graph(occxc_context *c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   c->graph_exec = nullptr;
@@ -843,6 +905,29 @@ occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc, doubl
   const double *Ds[1] = {D};
   double *Vs[1] = {Vxc};
   return occxc_batch_build(ctx, Ds, Vs, exc, nelec);
+}
+
+occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
+                               const int *shell_atom, double *grad) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "occxc_xc_gradient needs a single-molecule context");
+    OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
+    for (int k = 0; k < ctx->nsh; ++k)
+      OCCXC_REQUIRE(shell_atom[k] >= 0 && shell_atom[k] < natoms, OCCXC_ERR_INVALID_ARGUMENT,
+                    "shell_atom: atom index out of range");
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    for (size_t i = 0; i < nn; ++i)
+      OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
+    DeviceGuard g(ctx->device);
+    gradient_setup(ctx, natoms, shell_atom);
+    OCCXC_HIP(hipMemcpyAsync(ctx->d_D.get(), D, nn * sizeof(double), hipMemcpyHostToDevice,
+                             ctx->stream));
+    run_gradient(ctx, ctx->d_D.get(), ctx->stream);
+    ctx->d_grad.download(grad, 3 * (size_t)natoms, ctx->stream);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+  });
 }
 
 occxc_status occxc_batch_layout(const occxc_context *ctx, int *nmol, int *nbf,

@@ -58,6 +58,16 @@ struct occxc_context {
   bool use_graph = false;
   hipGraphExec_t graph_exec = nullptr;
   const void *graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  // XC gradient (single-molecule contexts): shell of each basis function, the
+  // extra AO derivative components (evaluated on the first gradient call),
+  // per-AO partial sums, and the per-atom entry lists built for the
+  // shell-to-atom map of the last call
+  std::vector<int> bf_shell;
+  occxc::DevBuf<double> d_ao2, d_gpart, d_grad;
+  occxc::DevBuf<int> d_gatom_off, d_gatom_ent;
+  bool ao2_ready = false;
+  std::vector<int> grad_shell_atom;
+  int grad_natoms = 0;
 };
 
 namespace occxc {

@@ -133,6 +133,107 @@ __device__ __forceinline__ void shell_point(const ShellDev &sh,
   }
 }
 
+// x^a y^b z^c from the power tables, 0 when an exponent is negative (the
+// derivative of a monomial that does not contain that coordinate).
+template <int L>
+__device__ __forceinline__ double mono(const double (&xp)[L + 1], const double (&yp)[L + 1],
+                                       const double (&zp)[L + 1], int a, int b, int c) {
+  return (a < 0 || b < 0 || c < 0) ? 0.0 : xp[a < 0 ? 0 : a] * yp[b < 0 ? 0 : b] * zp[c < 0 ? 0 : c];
+}
+
+// Gradient and Hessian of every component of one shell at one point
+// (gau2grid gg_collocation_deriv2, as evaluate_basis_subset calls it with
+// max_derivative 2, gto.cpp:245-285).  For a Cartesian monomial P times the
+// contraction, with S0 = sum c e^{-a r2}, S1 = sum -2a c e^{-a r2} and
+// S2 = sum 4a^2 c e^{-a r2}:
+//   d_i f  = (d_i P) S0 + P x_i S1
+//   d_ij f = (d_ij P) S0 + ((d_i P) x_j + (d_j P) x_i + delta_ij P) S1 + P x_i x_j S2.
+// Only the components q in [Q0, Q1) are formed (q = 1..3 x,y,z; 4..9 xx, xy,
+// xz, yy, yz, zz) and each goes to sink(q, m, value).
+template <int L, int Q0, int Q1, typename Sink>
+__device__ __forceinline__ void shell_point_d2(const ShellDev &sh,
+                                               const double *__restrict__ exps,
+                                               const double *__restrict__ coefs,
+                                               double dx, double dy, double dz, Sink &&sink) {
+  constexpr int NC = (L + 1) * (L + 2) / 2;
+  constexpr int NQ = Q1 - Q0;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+  for (int k = 0; k < sh.nprim; ++k) {
+    const double a = exps[sh.poff + k];
+    const double c = coefs[sh.poff + k] * exp(-a * r2);
+    S0 += c;
+    S1 += -2.0 * a * c;
+    S2 += 4.0 * a * a * c;
+  }
+  double xp[L + 1], yp[L + 1], zp[L + 1];
+  xp[0] = yp[0] = zp[0] = 1.0;
+#pragma unroll
+  for (int i = 1; i <= L; ++i) {
+    xp[i] = xp[i - 1] * dx;
+    yp[i] = yp[i - 1] * dy;
+    zp[i] = zp[i - 1] * dz;
+  }
+  const double d[3] = {dx, dy, dz};
+  double cq[NQ][NC];
+  int idx = 0;
+#pragma unroll
+  for (int a = L; a >= 0; --a) {
+#pragma unroll
+    for (int b = L - a; b >= 0; --b) {
+      const int c = L - a - b;
+      const double P = mono<L>(xp, yp, zp, a, b, c);
+      const double dP[3] = {a * mono<L>(xp, yp, zp, a - 1, b, c),
+                            b * mono<L>(xp, yp, zp, a, b - 1, c),
+                            c * mono<L>(xp, yp, zp, a, b, c - 1)};
+#pragma unroll
+      for (int q = Q0; q < Q1; ++q) {
+        double v;
+        if (q <= 3) {
+          const int i = q - 1;
+          v = dP[i] * S0 + P * d[i] * S1;
+        } else {
+          constexpr int hi[6] = {0, 0, 0, 1, 1, 2}, hj[6] = {0, 1, 2, 1, 2, 2};
+          const int i = hi[q - 4], j = hj[q - 4];
+          const int ea[3] = {a, b, c};
+          // d_ij P: lower exponent i, then j
+          const int ci = ea[i];
+          const int cj = ea[j] - (i == j ? 1 : 0);
+          const int e0 = a - (i == 0) - (j == 0), e1 = b - (i == 1) - (j == 1),
+                    e2 = c - (i == 2) - (j == 2);
+          const double ddP = (double)(ci * cj) * mono<L>(xp, yp, zp, e0, e1, e2);
+          v = ddP * S0 + (dP[i] * d[j] + dP[j] * d[i] + (i == j ? P : 0.0)) * S1 +
+              P * d[i] * d[j] * S2;
+        }
+        cq[q - Q0][idx] = v;
+      }
+      ++idx;
+    }
+  }
+  if (sh.pure && L >= 2) {
+    const double *T = c_sph + c_sph_off[L];
+#pragma unroll
+    for (int m = 0; m < 2 * L + 1; ++m)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) v += T[m * NC + k] * cq[q][k];
+        sink(Q0 + q, m, v);
+      }
+  } else if (sh.pure && L == 1) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sink(Q0 + q, m, cq[q][m == 2 ? 0 : m + 1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) sink(Q0 + q, k, cq[q][k]);
+  }
+}
+
 __device__ __forceinline__ void eval_shell(const ShellDev &sh,
                                            const double *__restrict__ exps,
                                            const double *__restrict__ coefs,
@@ -221,6 +322,37 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
   for (int r = L.m; r < L.mp; ++r)
     for (int c = 0; c < a.ncomp; ++c)
       base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
+}
+
+// Derivative components the resident AO cache does not hold, for the XC
+// gradient: q in [Q0, Q1) of shell_point_d2 (the gradient for an LDA context,
+// the Hessian for a GGA one), FP64, each leaf's block [Q1-Q0][mp][np] at
+// element offset ao_off / ncomp * (Q1 - Q0).
+template <int LMAX, int Q0, int Q1>
+__global__ __launch_bounds__(256) void k_ao_extra(BuildArgs a, double *__restrict__ out) {
+  constexpr int NQ = Q1 - Q0;
+  const LeafDev L = a.leaves[blockIdx.x];
+  const int p = threadIdx.x;
+  if (p >= L.np) return;
+  const double x = a.px[L.pt_off + p], y = a.py[L.pt_off + p], z = a.pz[L.pt_off + p];
+  double *base = out + L.ao_off / a.ncomp * NQ;
+  const long long cstride = (long long)L.mp * L.np;
+  int mu = 0;
+  for (int k = 0; k < L.nsig; ++k) {
+    const ShellDev sh = a.shells[a.leaf_shells[L.sh_off + k]];
+    double *row = base + (long long)mu * L.np + p;
+    auto sink = [&](int q, int m, double v) { row[(q - Q0) * cstride + (long long)m * L.np] = v; };
+    const double dx = x - sh.x, dy = y - sh.y, dz = z - sh.z;
+    if (sh.l == 0) shell_point_d2<0, Q0, Q1>(sh, a.exps, a.coefs, dx, dy, dz, sink);
+    else if (sh.l == 1) shell_point_d2<1, Q0, Q1>(sh, a.exps, a.coefs, dx, dy, dz, sink);
+    else if (LMAX == 2 || sh.l == 2) shell_point_d2<2, Q0, Q1>(sh, a.exps, a.coefs, dx, dy, dz, sink);
+    else if (sh.l == 3)
+      shell_point_d2<(LMAX > 2 ? 3 : 2), Q0, Q1>(sh, a.exps, a.coefs, dx, dy, dz, sink);
+    else shell_point_d2<LMAX, Q0, Q1>(sh, a.exps, a.coefs, dx, dy, dz, sink);
+    mu += sh.size;
+  }
+  for (int r = L.m; r < L.mp; ++r)
+    for (int c = 0; c < NQ; ++c) base[c * cstride + (long long)r * L.np + p] = 0.0;
 }
 
 // ------------------------------------------------- density + functional --
@@ -577,6 +709,19 @@ void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s) {
   } else {
     if (low) k_ao_eval<double, 2><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
     else k_ao_eval<double, kMaxL><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
+  }
+  check_launch();
+}
+
+void launch_ao_extra(const BuildArgs &a, double *out, bool hessian, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  const bool low = a.lmax <= 2;
+  if (hessian) {
+    if (low) k_ao_extra<2, 4, 10><<<a.nleaves, 256, 0, s>>>(a, out);
+    else k_ao_extra<kMaxL, 4, 10><<<a.nleaves, 256, 0, s>>>(a, out);
+  } else {
+    if (low) k_ao_extra<2, 1, 4><<<a.nleaves, 256, 0, s>>>(a, out);
+    else k_ao_extra<kMaxL, 1, 4><<<a.nleaves, 256, 0, s>>>(a, out);
   }
   check_launch();
 }

@@ -120,6 +120,16 @@ void launch_zero(double *p, long long n, hipStream_t s);
 void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int nmol,
                             int maxnbf, hipStream_t s);
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s);
+// XC gradient: the AO derivative components the cache lacks (FP64): the
+// Hessian (6 components) for a GGA context, the gradient (3) for an LDA one.
+void launch_ao_extra(const BuildArgs &a, double *out, bool hessian, hipStream_t s);
+// XC gradient contraction (grad.hip): per leaf and local AO mu the sum over
+// the leaf's points of the gradient integrand into gpart[3 (map_off + mu) + k],
+// then grad[3 A + k] = -2 sum over the entries of atom A (CSR atom_off /
+// atom_ent), in a fixed order.
+void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const double *ao2,
+                        double *gpart, int natoms, const int *atom_off, const int *atom_ent,
+                        double *grad, hipStream_t s);
 void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32,
                                hipStream_t s);
 void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s);

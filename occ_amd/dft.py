@@ -105,7 +105,20 @@ class XCContext:
                                                  ctypes.c_void_p(d_result),
                                                  ctypes.c_void_p(stream or 0)))
 
-    STAGES = ("prepare", "density", "vxc", "finalize", "ao")
+    def xc_gradient(self, D, shell_atom, natoms=None):
+        """XC contribution to the nuclear gradient (occxc_xc_gradient):
+        3 x natoms, column per atom."""
+        D = np.asfortranarray(D, dtype=np.float64)
+        if D.shape != (self.nbf, self.nbf):
+            raise ValueError(f"D must be {self.nbf}x{self.nbf}")
+        sa = _lib.i32(shell_atom)
+        natoms = int(natoms if natoms is not None else (sa.max() + 1 if len(sa) else 0))
+        G = np.zeros((3, max(natoms, 1)), order="F")
+        _lib.check(_lib.lib().occxc_xc_gradient(self._h, D.ctypes.data_as(_lib._dp), natoms,
+                                                _lib.iptr(sa), G.ctypes.data_as(_lib._dp)))
+        return G
+
+    STAGES = ("prepare", "density", "vxc", "finalize", "ao", "gradient")
 
     def set_timing(self, enable=True):
         _lib.check(_lib.lib().occxc_context_set_timing(self._h, int(bool(enable))))
@@ -258,6 +271,10 @@ class DFT:
 
     def exchange_correlation_energy(self):
         return self._exc
+
+    def compute_xc_gradient(self, D):
+        """The 3 x natoms matrix OCC's compute_xc_gradient returns (dft.h:531)."""
+        return self.ctx.xc_gradient(D, self.basis.shell_atom, len(self.basis.atoms))
 
     def total_density(self):
         return self._nel

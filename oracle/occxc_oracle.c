@@ -212,6 +212,84 @@ void oracle_eval_shell(int l, int pure, int nprim, const double *exps,
   }
 }
 
+/* Value, gradient and Hessian of one shell times common_fac: gto.cpp:245-285
+ * (evaluate_basis_subset, max_derivative 2) with gau2grid
+ * gg_collocation_deriv2.  With S0 = sum c e^{-a r2}, S1 = sum -2a c e^{-a r2},
+ * S2 = sum 4a^2 c e^{-a r2} and a Cartesian monomial P:
+ *   d_i f    = (d_i P) S0 + P x_i S1
+ *   d_ij f   = (d_ij P) S0 + ((d_i P) x_j + (d_j P) x_i + delta_ij P) S1
+ *              + P x_i x_j S2.
+ * out[(q*nout + o)*ld + p], q = 0 value, 1..3 x,y,z, 4..9 xx,xy,xz,yy,yz,zz. */
+void oracle_eval_shell2(int l, int pure, int nprim, const double *exps,
+                        const double *coefs, const double *center, long npts,
+                        const double *xyz, double *out, long ld) {
+  enum { NCM = (ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2 };
+  int ncart = (l + 1) * (l + 2) / 2;
+  int nout = oracle_shell_size(l, pure);
+  double T[(2 * ORACLE_MAX_L + 1) * NCM];
+  if (pure) oracle_cart_to_sph(l, T);
+  double fac = oracle_common_fac(l, pure);
+  double cc[10][NCM];
+  for (long p = 0; p < npts; p++) {
+    double d[3] = {xyz[3 * p] - center[0], xyz[3 * p + 1] - center[1],
+                   xyz[3 * p + 2] - center[2]};
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+    for (int k = 0; k < nprim; k++) {
+      double e = coefs[k] * exp(-exps[k] * r2);
+      S0 += e;
+      S1 += -2.0 * exps[k] * e;
+      S2 += 4.0 * exps[k] * exps[k] * e;
+    }
+    int idx = 0;
+    for (int a = l; a >= 0; a--)
+      for (int b = l - a; b >= 0; b--) {
+        int c = l - a - b;
+        int pw[3] = {a, b, c};
+        double P = ipow(d[0], a) * ipow(d[1], b) * ipow(d[2], c);
+        double dP[3], ddP[3][3];
+        for (int i = 0; i < 3; i++) {
+          int q[3] = {a, b, c};
+          dP[i] = pw[i] ? pw[i] * ipow(d[i], pw[i] - 1) : 0.0;
+          for (int j = 0; j < 3; j++)
+            if (j != i) dP[i] *= ipow(d[j], q[j]);
+        }
+        for (int i = 0; i < 3; i++)
+          for (int j = 0; j < 3; j++) {
+            int q[3] = {a, b, c};
+            double coef = 1.0;
+            if (q[i] == 0) { ddP[i][j] = 0.0; continue; }
+            coef *= q[i];
+            q[i] -= 1;
+            if (q[j] == 0) { ddP[i][j] = 0.0; continue; }
+            coef *= q[j];
+            q[j] -= 1;
+            ddP[i][j] = coef * ipow(d[0], q[0]) * ipow(d[1], q[1]) * ipow(d[2], q[2]);
+          }
+        cc[0][idx] = P * S0;
+        for (int i = 0; i < 3; i++) cc[1 + i][idx] = dP[i] * S0 + P * d[i] * S1;
+        static const int hi[6] = {0, 0, 0, 1, 1, 2}, hj[6] = {0, 1, 2, 1, 2, 2};
+        for (int h = 0; h < 6; h++) {
+          int i = hi[h], j = hj[h];
+          cc[4 + h][idx] = ddP[i][j] * S0 +
+                           (dP[i] * d[j] + dP[j] * d[i] + (i == j ? P : 0.0)) * S1 +
+                           P * d[i] * d[j] * S2;
+        }
+        idx++;
+      }
+    for (int o = 0; o < nout; o++)
+      for (int q = 0; q < 10; q++) {
+        double v = 0.0;
+        if (pure) {
+          for (int k = 0; k < ncart; k++) v += T[o * ncart + k] * cc[q][k];
+        } else {
+          v = cc[q][o];
+        }
+        out[((long)q * nout + o) * ld + p] = fac * v;
+      }
+  }
+}
+
 void oracle_decay_cutoff(const oracle_basis *b, double threshold,
                          double *extents) {
   /* gto.cpp:18-67: r = 0.5 .. 30 Bohr in 0.1 steps along +x from the shell
@@ -770,6 +848,293 @@ int oracle_xc_build(const oracle_basis *b, long npts, const double *xyz,
   }
   *exc_out = energy;
   *nelec_out = nel;
+  free(first_bf);
+  free(D2);
+  free(sxyz);
+  free(sw);
+  free(perm);
+  free(loff);
+  free(lcnt);
+  free(lcen);
+  free(lrad);
+  free(ext);
+  return 0;
+}
+
+/* kernels::process_grid_block_gradient<d, Restricted>
+ * (dft_gradient_kernels.cpp:473-552) with accumulate_gradient_contribution_R
+ * <0> and <1> (dft_gradient_kernels.cpp:26-90).  phi is [10][m][n] (value,
+ * gradient, Hessian xx,xy,xz,yy,yz,zz); Dloc is the gathered block of 2 D
+ * (full, both triangles); g is 3 x natoms column-major. */
+static void process_block_gradient(const block_cfg *cfg, int m, long n,
+                                   const double *phi, const double *Dloc,
+                                   const double *wts, const int *bf_atom,
+                                   double *g, double *work) {
+  const long mn = (long)m * n;
+  const double *p0 = phi, *px = phi + mn, *py = phi + 2 * mn, *pz = phi + 3 * mn;
+  const double *pxx = phi + 4 * mn, *pxy = phi + 5 * mn, *pxz = phi + 6 * mn;
+  const double *pyy = phi + 7 * mn, *pyz = phi + 8 * mn, *pzz = phi + 9 * mn;
+  double *T = work;                /* [m][n] */
+  double *rho = T + mn;            /* [4][n] */
+  double *exc = rho + 4 * n, *vrho = exc + n, *vsig = vrho + n, *sig = vsig + n;
+  double *A = sig + n;             /* [m][n] */
+  double *V = A + mn;              /* 3 x [m x m], column-major */
+
+  /* evaluate_density<d,R>: T = phi * Dloc.selfadjointView<Upper>() */
+  memset(T, 0, sizeof(double) * mn);
+  for (int nu = 0; nu < m; nu++) {
+    double *t = T + (long)nu * n;
+    for (int mu = 0; mu < m; mu++) {
+      int r = mu < nu ? mu : nu, c = mu < nu ? nu : mu;
+      double dm = Dloc[r + (long)c * m];
+      const double *f = p0 + (long)mu * n;
+      for (long p = 0; p < n; p++) t[p] += dm * f[p];
+    }
+  }
+  memset(rho, 0, sizeof(double) * 4 * n);
+  for (int nu = 0; nu < m; nu++) {
+    const double *t = T + (long)nu * n, *f = p0 + (long)nu * n;
+    for (long p = 0; p < n; p++) rho[p] += f[p] * t[p];
+    if (cfg->deriv) {
+      const double *fx = px + (long)nu * n, *fy = py + (long)nu * n, *fz = pz + (long)nu * n;
+      for (long p = 0; p < n; p++) {
+        rho[n + p] += 2.0 * fx[p] * t[p];
+        rho[2 * n + p] += 2.0 * fy[p] * t[p];
+        rho[3 * n + p] += 2.0 * fz[p] * t[p];
+      }
+    }
+  }
+  /* dft_gradient_kernels.cpp:488-491: skip when max rho < threshold */
+  double rmax = -1e300;
+  for (long p = 0; p < n; p++)
+    if (rho[p] > rmax) rmax = rho[p];
+  if (rmax < cfg->dthr) return;
+  for (long p = 0; p < n; p++) {
+    exc[p] = vrho[p] = vsig[p] = 0.0;
+    sig[p] = rho[n + p] * rho[n + p] + rho[2 * n + p] * rho[2 * n + p] +
+             rho[3 * n + p] * rho[3 * n + p];
+  }
+  for (int f = 0; f < cfg->nfunc; f++)
+    oracle_functional_eval(cfg->ids[f], n, rho, sig, exc, vrho, vsig);
+  for (long p = 0; p < n; p++) { /* Result::weight_by */
+    vrho[p] *= wts[p];
+    vsig[p] *= wts[p];
+  }
+  double *Vx = V, *Vy = V + (long)m * m, *Vz = V + 2L * m * m;
+  memset(V, 0, sizeof(double) * 3 * m * m);
+  if (!cfg->deriv) {
+    /* <0>: V_k += d_k phi^T (phi o vrho) */
+    for (int nu = 0; nu < m; nu++) {
+      const double *f = p0 + (long)nu * n;
+      double *a = A + (long)nu * n;
+      for (long p = 0; p < n; p++) a[p] = f[p] * vrho[p];
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *a = A + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double *fx = px + (long)mu * n, *fy = py + (long)mu * n, *fz = pz + (long)mu * n;
+        double sx = 0, sy = 0, sz = 0;
+        for (long p = 0; p < n; p++) {
+          sx += fx[p] * a[p];
+          sy += fy[p] * a[p];
+          sz += fz[p] * a[p];
+        }
+        Vx[mu + (long)nu * m] += sx;
+        Vy[mu + (long)nu * m] += sy;
+        Vz[mu + (long)nu * m] += sz;
+      }
+    }
+  } else {
+    /* <1>: aow = 0.5 vrho phi + 2 vsigma grad rho . grad phi;
+     * V_k += d_k phi^T aow, then (phi^T aow_k)^T with
+     * aow_k = 0.5 vrho d_k phi + 2 vsigma sum_j d_j rho d_kj phi */
+    for (int nu = 0; nu < m; nu++) {
+      const long o = (long)nu * n;
+      double *a = A + o;
+      for (long p = 0; p < n; p++)
+        a[p] = 0.5 * vrho[p] * p0[o + p] +
+               2.0 * vsig[p] * (rho[n + p] * px[o + p] + rho[2 * n + p] * py[o + p] +
+                                rho[3 * n + p] * pz[o + p]);
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *a = A + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double *fx = px + (long)mu * n, *fy = py + (long)mu * n, *fz = pz + (long)mu * n;
+        double sx = 0, sy = 0, sz = 0;
+        for (long p = 0; p < n; p++) {
+          sx += fx[p] * a[p];
+          sy += fy[p] * a[p];
+          sz += fz[p] * a[p];
+        }
+        Vx[mu + (long)nu * m] += sx;
+        Vy[mu + (long)nu * m] += sy;
+        Vz[mu + (long)nu * m] += sz;
+      }
+    }
+    const double *dk[3] = {px, py, pz};
+    const double *hk[3][3] = {{pxx, pxy, pxz}, {pxy, pyy, pyz}, {pxz, pyz, pzz}};
+    double *Vk[3] = {Vx, Vy, Vz};
+    for (int k = 0; k < 3; k++) {
+      /* aow_k in A, then V_k(mu, nu) += sum_p aow_k(mu) phi(nu) */
+      for (int mu = 0; mu < m; mu++) {
+        const long o = (long)mu * n;
+        double *a = A + o;
+        for (long p = 0; p < n; p++)
+          a[p] = 0.5 * vrho[p] * dk[k][o + p] +
+                 2.0 * vsig[p] * (rho[n + p] * hk[k][0][o + p] + rho[2 * n + p] * hk[k][1][o + p] +
+                                  rho[3 * n + p] * hk[k][2][o + p]);
+      }
+      for (int nu = 0; nu < m; nu++) {
+        const double *f = p0 + (long)nu * n;
+        for (int mu = 0; mu < m; mu++) {
+          const double *a = A + (long)mu * n;
+          double s = 0;
+          for (long p = 0; p < n; p++) s += a[p] * f[p];
+          Vk[k][mu + (long)nu * m] += s;
+        }
+      }
+    }
+  }
+  /* dft_gradient_kernels.cpp:533-550: distribute with 2 D(mu, nu), skipping
+   * |2 D| < 1e-12 */
+  for (int mu = 0; mu < m; mu++) {
+    const int at = bf_atom[mu];
+    for (int nu = 0; nu < m; nu++) {
+      const double dv = 2.0 * Dloc[mu + (long)nu * m];
+      if (fabs(dv) < 1e-12) continue;
+      g[3 * at] -= Vx[mu + (long)nu * m] * dv;
+      g[3 * at + 1] -= Vy[mu + (long)nu * m] * dv;
+      g[3 * at + 2] -= Vz[mu + (long)nu * m] * dv;
+    }
+  }
+}
+
+int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
+                       long npts, const double *xyz, const double *w, int nfunc,
+                       const int *func_ids, const double *D,
+                       double density_threshold, double screening_threshold,
+                       int screened, int nthreads, double *grad) {
+  if (!b || !shell_atom || natoms <= 0 || npts < 0 || nfunc <= 0) return -1;
+  int deriv = 0;
+  for (int f = 0; f < nfunc; f++) {
+    int fam = oracle_functional_family(func_ids[f]);
+    if (fam < 0) return -1;
+    if (fam > deriv) deriv = fam;
+  }
+  const int nsh = b->nshell;
+  for (int s = 0; s < nsh; s++)
+    if (shell_atom[s] < 0 || shell_atom[s] >= natoms) return -1;
+  const int nbf = oracle_nbf(b);
+  int *first_bf = malloc(sizeof(int) * (nsh + 1));
+  first_bf[0] = 0;
+  for (int s = 0; s < nsh; s++)
+    first_bf[s + 1] = first_bf[s] + oracle_shell_size(b->l[s], b->pure[s]);
+  /* dft.h:606: D = 2 mo.D (not symmetrised; the density reads its upper
+   * triangle, the distribution both) */
+  double *D2 = malloc(sizeof(double) * nbf * nbf);
+  for (long i = 0; i < (long)nbf * nbf; i++) D2[i] = 2.0 * D[i];
+  memset(grad, 0, sizeof(double) * 3 * natoms);
+  block_cfg cfg = {deriv, nfunc, func_ids, density_threshold};
+
+  const double *pxyz = xyz, *pw = w;
+  double *sxyz = 0, *sw = 0, *lcen = 0, *lrad = 0, *ext = 0;
+  long *perm = 0, *loff = 0, *lcnt = 0;
+  long nblk;
+  if (screened) {
+    sxyz = malloc(sizeof(double) * 3 * (npts + 1));
+    sw = malloc(sizeof(double) * (npts + 1));
+    perm = malloc(sizeof(long) * (npts + 1));
+    long cap = npts / 32 + 2;
+    loff = malloc(sizeof(long) * cap);
+    lcnt = malloc(sizeof(long) * cap);
+    lcen = malloc(sizeof(double) * 3 * cap);
+    lrad = malloc(sizeof(double) * cap);
+    nblk = oracle_build_hierarchy(npts, xyz, w, sxyz, sw, perm, loff, lcnt, lcen, lrad);
+    pxyz = sxyz;
+    pw = sw;
+    ext = malloc(sizeof(double) * nsh);
+    oracle_decay_cutoff(b, screening_threshold, ext);
+  } else {
+    /* the dense path (dft.h:631-667) cuts TBB-sized blocks; 128-point blocks
+     * here (only the max-rho skip depends on the cut) */
+    nblk = (npts + 127) / 128;
+    loff = malloc(sizeof(long) * (nblk + 1));
+    lcnt = malloc(sizeof(long) * (nblk + 1));
+    for (long i = 0; i < nblk; i++) {
+      loff[i] = 128 * i;
+      lcnt[i] = (npts - 128 * i < 128) ? npts - 128 * i : 128;
+    }
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    double *gloc = calloc((size_t)3 * natoms, sizeof(double));
+    int *sig = malloc(sizeof(int) * (nsh + 1));
+    int *gidx = malloc(sizeof(int) * (nbf + 1));
+    int *batom = malloc(sizeof(int) * (nbf + 1));
+    long maxn = 256;
+    double *phi = malloc(sizeof(double) * 10 * (size_t)nbf * maxn);
+    double *tmp = malloc(sizeof(double) * 10 * (ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2 * maxn);
+    double *Dloc = malloc(sizeof(double) * (size_t)nbf * nbf);
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 8 * maxn +
+                                            3 * (size_t)nbf * nbf));
+#pragma omp for schedule(dynamic, 4)
+    for (long li = 0; li < nblk; li++) {
+      long n = lcnt[li];
+      if (n == 0) continue;
+      int nsig = 0;
+      if (screened) {
+        /* dft.h:697-702 */
+        for (int s = 0; s < nsh; s++) {
+          const double *o = b->origin + 3 * s;
+          double dx = o[0] - lcen[3 * li], dy = o[1] - lcen[3 * li + 1],
+                 dz = o[2] - lcen[3 * li + 2];
+          if (sqrt(dx * dx + dy * dy + dz * dz) - ext[s] < lrad[li]) sig[nsig++] = s;
+        }
+        if (nsig == 0) continue;
+      } else {
+        for (int s = 0; s < nsh; s++) sig[nsig++] = s;
+      }
+      int m = 0;
+      for (int k = 0; k < nsig; k++) {
+        int s = sig[k];
+        for (int q = first_bf[s]; q < first_bf[s + 1]; q++) {
+          batom[m] = shell_atom[s];
+          gidx[m++] = q;
+        }
+      }
+      for (int j = 0; j < m; j++)
+        for (int i = 0; i < m; i++) Dloc[i + (long)j * m] = D2[gidx[i] + (long)gidx[j] * nbf];
+      /* evaluate_basis_subset(..., max_deriv = 2, sig) */
+      const double *bx = pxyz + 3 * loff[li];
+      int off = 0;
+      for (int k = 0; k < nsig; k++) {
+        int s = sig[k], sz = oracle_shell_size(b->l[s], b->pure[s]);
+        oracle_eval_shell2(b->l[s], b->pure[s], b->nprim[s], b->exps + b->prim_offset[s],
+                           b->coefs + b->prim_offset[s], b->origin + 3 * s, n, bx, tmp, n);
+        for (int q = 0; q < 10; q++)
+          for (int o = 0; o < sz; o++)
+            memcpy(phi + ((long)q * m + off + o) * n, tmp + ((long)q * sz + o) * n,
+                   sizeof(double) * n);
+        off += sz;
+      }
+      process_block_gradient(&cfg, m, n, phi, Dloc, pw + loff[li], batom, gloc, work);
+    }
+#pragma omp critical
+    {
+      for (int i = 0; i < 3 * natoms; i++) grad[i] += gloc[i];
+    }
+    free(gloc);
+    free(sig);
+    free(gidx);
+    free(batom);
+    free(phi);
+    free(tmp);
+    free(Dloc);
+    free(work);
+  }
   free(first_bf);
   free(D2);
   free(sxyz);

@@ -52,6 +52,13 @@ void oracle_eval_shell(int l, int pure, int nprim, const double *exps,
                        const double *xyz, int deriv, double *phi, double *phx,
                        double *phy, double *phz, long ld);
 
+/* Value, gradient and Hessian of one shell, times common_fac (gto.cpp:245-285
+ * with gau2grid gg_collocation_deriv2).  out[(q*nout + o)*ld + p] with q = 0
+ * value, 1..3 x,y,z, 4..9 xx,xy,xz,yy,yz,zz. */
+void oracle_eval_shell2(int l, int pure, int nprim, const double *exps,
+                        const double *coefs, const double *center, long npts,
+                        const double *xyz, double *out, long ld);
+
 /* gto.cpp:18-67 */
 void oracle_decay_cutoff(const oracle_basis *b, double threshold,
                          double *extents);
@@ -101,6 +108,18 @@ int oracle_xc_build(const oracle_basis *b, long npts, const double *xyz,
                     const double *D, double density_threshold,
                     double screening_threshold, int screened, int nthreads,
                     double *K, double *exc, double *nelec);
+
+/* DFT::compute_xc_gradient_impl<d, Restricted> (include/occ/dft/dft.h:598-749)
+ * with kernels::process_grid_block_gradient (src/dft/dft_gradient_kernels.cpp:
+ * 473-552): the XC contribution to the nuclear gradient, 3 x natoms
+ * column-major (Mat3N).  shell_atom[s] is the atom of shell s
+ * (AOBasis::shell_to_atom).  Other arguments as oracle_xc_build.  Returns 0,
+ * or -1 on invalid input. */
+int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
+                       long npts, const double *xyz, const double *w, int nfunc,
+                       const int *func_ids, const double *D,
+                       double density_threshold, double screening_threshold,
+                       int screened, int nthreads, double *grad);
 
 /* Algorithmic work of the screened build: per-leaf (n_b, m_b) with the
  * reference screening rule.  Writes up to maxleaves entries; returns the

@@ -53,6 +53,12 @@ def lib():
         L.oracle_xc_build.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp,
                                       ctypes.c_int, _ip, _dp, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_eval_shell2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                         _dp, ctypes.c_long, _dp, _dp, ctypes.c_long]
+        L.oracle_xc_gradient.argtypes = [ctypes.POINTER(_Basis), _ip, ctypes.c_int,
+                                         ctypes.c_long, _dp, _dp, ctypes.c_int, _ip, _dp,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, _dp]
         L.oracle_screened_work.restype = ctypes.c_long
         L.oracle_screened_work.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp,
                                            ctypes.c_double, ctypes.c_long, _lp, _lp]
@@ -135,6 +141,20 @@ def eval_shell(l, pure, exps, coefs, center, xyz, deriv=1):
     return out if deriv else out[:1]
 
 
+def eval_shell2(l, pure, exps, coefs, center, xyz):
+    """(10, ncomp, npts): value, gradient, Hessian xx,xy,xz,yy,yz,zz."""
+    exps = np.ascontiguousarray(exps, dtype=np.float64)
+    coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    center = np.ascontiguousarray(center, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    nc = (2 * l + 1) if pure else (l + 1) * (l + 2) // 2
+    out = np.zeros((10, nc, n))
+    lib().oracle_eval_shell2(l, int(pure), len(exps), _d(exps), _d(coefs), _d(center), n,
+                             _d(xyz), _d(out), n)
+    return out
+
+
 def decay_cutoff(basis, threshold=1e-10):
     ext = np.zeros(len(basis.l))
     lib().oracle_decay_cutoff(basis.ptr, threshold, _d(ext))
@@ -196,6 +216,27 @@ def xc_build(basis, xyz, w, func_ids, D, density_threshold=1e-10,
     return K, exc.value, nel.value
 
 
+def xc_gradient(basis, shell_atom, natoms, xyz, w, func_ids, D, density_threshold=1e-10,
+                screening_threshold=1e-10, screened=True, nthreads=0):
+    """compute_xc_gradient restated on the CPU: returns the 3 x natoms XC
+    contribution to the nuclear gradient."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    ids = np.ascontiguousarray(func_ids, dtype=np.int32)
+    sa = np.ascontiguousarray(shell_atom, dtype=np.int32)
+    D = np.asfortranarray(D, dtype=np.float64)
+    nbf = basis.nbf
+    assert D.shape == (nbf, nbf) and len(sa) == len(basis.l)
+    G = np.zeros((3, natoms), order="F")
+    rc = lib().oracle_xc_gradient(basis.ptr, _i(sa), int(natoms), len(w), _d(xyz), _d(w),
+                                  len(ids), _i(ids), D.ctypes.data_as(_dp), density_threshold,
+                                  screening_threshold, int(screened), int(nthreads),
+                                  G.ctypes.data_as(_dp))
+    if rc != 0:
+        raise ValueError("oracle_xc_gradient rejected its input")
+    return G
+
+
 def screened_work(basis, xyz, w, screening_threshold=1e-10):
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
     w = np.ascontiguousarray(w, dtype=np.float64)
@@ -227,6 +268,8 @@ def gau2grid():
                                             _dp, _dp, _dp, _dp]
         L.gg_collocation.argtypes = [ctypes.c_int, ctypes.c_ulong, _dp, ctypes.c_ulong,
                                      ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.gg_collocation_deriv2.argtypes = [ctypes.c_int, ctypes.c_ulong, _dp, ctypes.c_ulong,
+                                            ctypes.c_int, _dp, _dp, _dp, ctypes.c_int] + [_dp] * 10
         _gg = L
     return _gg
 
@@ -249,3 +292,19 @@ def gau2grid_deriv1(l, pure, exps, coefs, center, xyz):
 
 
 os.environ.setdefault("OMP_PROC_BIND", "false")
+
+
+def gau2grid_deriv2(l, pure, exps, coefs, center, xyz):
+    """gg_collocation_deriv2 as gto.cpp:267-270 calls it: (10, ncomp, npts) =
+    value, gradient, Hessian xx,xy,xz,yy,yz,zz, WITHOUT common_fac."""
+    exps = np.ascontiguousarray(exps, dtype=np.float64)
+    coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+    center = np.ascontiguousarray(center, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    nc = (2 * l + 1) if pure else (l + 1) * (l + 2) // 2
+    out = np.zeros((10, nc, n))
+    order = GG_SPHERICAL_CCA if pure else GG_CARTESIAN_CCA
+    gau2grid().gg_collocation_deriv2(l, n, _d(xyz), 3, len(exps), _d(coefs), _d(exps),
+                                     _d(center), order, *[_d(out[q]) for q in range(10)])
+    return out

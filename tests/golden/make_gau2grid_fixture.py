@@ -1,4 +1,6 @@
-"""Writes gau2grid_collocation.npz: phi, phi_x, phi_y, phi_z from the
+"""Writes gau2grid_collocation.npz: phi, phi_x, phi_y, phi_z (and, under
+hess<k>, phi_xx, phi_xy, phi_xz, phi_yy, phi_yz, phi_zz from
+gg_collocation_deriv2) from the
 reference's gau2grid (src/3rdparty/gau2grid, compiled from its own sources
 into oracle/_ref by `make -C oracle ref`) for L = 0..4, spherical CCA and
 Cartesian, nprim 1/3/6, points at the centre, mid-range and in the tails.
@@ -30,6 +32,7 @@ def main():
         out[f"center{k}"] = cen
         out[f"xyz{k}"] = xyz
         out[f"phi{k}"] = oracle.gau2grid_deriv1(l, pure, exps, coefs, cen, xyz)
+        out[f"hess{k}"] = oracle.gau2grid_deriv2(l, pure, exps, coefs, cen, xyz)[4:]
     out["ncase"] = len(cases)
     np.savez_compressed(HERE / "gau2grid_collocation.npz", **out)
     print(len(cases), "cases")

@@ -103,3 +103,12 @@ def test_server_rejects_bad_options_without_gpu():
     assert L.occxc_server_create(ctypes.byref(o), ctypes.byref(h)) == 1
     assert L.occxc_server_submit(None, 1, None, None, None, None, None) == 1
     assert L.occxc_server_destroy(None) == 0
+
+
+def test_xc_gradient_rejects_null_context_without_gpu():
+    L = _lib.lib()
+    G = np.zeros(3)
+    sa = np.zeros(1, dtype=np.int32)
+    st = L.occxc_xc_gradient(None, None, 1, _lib.iptr(sa), G.ctypes.data_as(_lib._dp))
+    assert st == 1
+    assert b"null" in L.occxc_last_error_message()

@@ -1,0 +1,152 @@
+"""GPU parity of the XC nuclear gradient (occxc_xc_gradient, replacing
+DFT::compute_xc_gradient, include/occ/dft/dft.h:531-532,598-749) against the
+CPU oracle, through the C ABI.  Tolerance: max|dG| <= 1e-8 Ha/Bohr per
+component for the FP64 path (the FP32-storage modes are reported with a
+looser bound).  Also the gradient as the derivative of the GPU's own Exc over
+shifted shell centres (grid held fixed), bit-for-bit repeats, and the
+argument checks."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_density
+from occ_amd import AOBasis, _lib, molecules
+from occ_amd.dft import DFT, XCBatch, XCContext, functional_ids
+from occ_amd.grid import GridSettings, MolecularGrid
+
+pytestmark = pytest.mark.gpu
+
+TOL_G = 1e-8
+
+
+def _grid(atoms, nrad=31, nang=110):
+    g = MolecularGrid(atoms, GridSettings(radial_points=nrad, angular_points=nang))
+    return g.build(device=0)
+
+
+def _case(atoms, basis_name, method, seed, pure, precision="fp64", screen=True, nrad=31,
+          nang=110):
+    basis = AOBasis.load(atoms, basis_name, pure=pure)
+    pts, wts = _grid(atoms, nrad, nang)
+    nocc = max(1, molecules.nuclear_charge(atoms) // 2)
+    D = random_density(basis.nbf, min(nocc, basis.nbf), seed)
+    print(f"gradient case {basis_name} {method} nbf={basis.nbf} npts={len(wts)} {precision}",
+          flush=True)
+    dft = DFT(method, basis, pts, wts, precision=precision, screen=screen)
+    G = dft.compute_xc_gradient(D)
+    ob = oracle.OracleBasis.from_flat(basis.flat())
+    G0 = oracle.xc_gradient(ob, basis.shell_atom, len(atoms), pts, wts, dft.ids, D,
+                            screened=screen)
+    return G, G0, dft, D
+
+
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+def test_gradient_water_631g(method):
+    G, G0, _, _ = _case(molecules.water(), "6-31g", method, 21, pure=False)
+    err = np.abs(G - G0).max()
+    print(f"water/6-31G {method}: max|dG|={err:.2e} max|G|={np.abs(G0).max():.3f}")
+    assert err <= TOL_G
+
+
+@pytest.mark.parametrize("method", ["pbe", "blyp", "svwn5"])
+def test_gradient_water_dimer_def2svp(method):
+    G, G0, _, _ = _case(molecules.water_cluster(2, seed=1), "def2-svp", method, 22, pure=True)
+    err = np.abs(G - G0).max()
+    print(f"(H2O)2/def2-SVP {method}: max|dG|={err:.2e}")
+    assert err <= TOL_G
+
+
+def test_gradient_cluster_and_f_shells():
+    G, G0, _, _ = _case(molecules.water_cluster(4, seed=2), "def2-svp", "pbe", 23, pure=True)
+    assert np.abs(G - G0).max() <= TOL_G
+    # def2-TZVP: O carries f shells (the l <= 4 instance of the AO kernels)
+    G, G0, _, _ = _case(molecules.water(), "def2-tzvp", "blyp", 24, pure=True)
+    assert np.abs(G - G0).max() <= TOL_G
+
+
+def test_gradient_dense_path():
+    G, G0, _, _ = _case(molecules.water(), "6-31g", "pbe", 25, pure=False, screen=False)
+    assert np.abs(G - G0).max() <= TOL_G
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32cache", 1e-5), ("fp32", 1e-5),
+                                           ("bf16x3", 1e-5)])
+def test_gradient_reduced_precision(precision, tol):
+    G, G0, _, _ = _case(molecules.water_cluster(2, seed=1), "def2-svp", "pbe", 26, pure=True,
+                        precision=precision)
+    err = np.abs(G - G0).max()
+    print(f"(H2O)2/def2-SVP pbe {precision}: max|dG|={err:.2e}")
+    assert err <= tol
+
+
+def test_gradient_is_the_derivative_of_the_gpu_exc():
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 5, 27)
+    ids = functional_ids("pbe")
+    G = XCContext(basis, pts, wts, ids).xc_gradient(D, basis.shell_atom, 3)
+    h = 1e-4
+    sa = np.array(basis.shell_atom)
+    for A in range(3):
+        for k in range(3):
+            e = []
+            for sgn in (1.0, -1.0):
+                b2 = AOBasis.load(atoms, "6-31g", pure=False)
+                for s, sh in enumerate(b2.shells):
+                    if sa[s] == A:
+                        sh.origin = np.array(sh.origin, dtype=float)
+                        sh.origin[k] += sgn * h
+                _, ex, _ = XCContext(b2, pts, wts, ids).build(D)
+                e.append(ex)
+            assert G[k, A] == pytest.approx((e[0] - e[1]) / (2 * h), abs=1e-7)
+
+
+def test_gradient_repeats_and_leaves_the_build_alone():
+    atoms = molecules.water_cluster(2, seed=1)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 10, 28)
+    ctx = XCContext(basis, pts, wts, functional_ids("blyp"))
+    V1, e1, n1 = ctx.build(D)
+    G1 = ctx.xc_gradient(D, basis.shell_atom, len(atoms))
+    V2, e2, n2 = ctx.build(D)
+    G2 = ctx.xc_gradient(D, basis.shell_atom, len(atoms))
+    assert np.array_equal(G1, G2)
+    assert e1 == e2 and n1 == n2
+    assert np.abs(V1 - V2).max() <= 1e-14
+    ctx.set_graph(True)
+    ctx.build(D)
+    assert np.array_equal(ctx.xc_gradient(D, basis.shell_atom, len(atoms)), G1)
+    # a different shell-to-atom map (all shells on atom 0) sums to the total
+    G3 = ctx.xc_gradient(D, np.zeros(len(basis.shell_atom), dtype=np.int32), 1)
+    np.testing.assert_allclose(G3[:, 0], G1.sum(axis=1), rtol=0, atol=1e-12)
+
+
+def test_gradient_argument_checks():
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, wts = _grid(atoms, 11, 26)
+    D = random_density(basis.nbf, 5, 29)
+    ctx = XCContext(basis, pts, wts, functional_ids("svwn5"))
+    L = _lib.lib()
+    sa = np.array(basis.shell_atom, dtype=np.int32)
+    G = np.zeros(9)
+    Df = np.asfortranarray(D)
+    st = L.occxc_xc_gradient(ctx.handle, Df.ctypes.data_as(_lib._dp), 2, _lib.iptr(sa),
+                             G.ctypes.data_as(_lib._dp))
+    assert st == 1 and b"out of range" in L.occxc_last_error_message()
+    st = L.occxc_xc_gradient(ctx.handle, Df.ctypes.data_as(_lib._dp), 0, _lib.iptr(sa),
+                             G.ctypes.data_as(_lib._dp))
+    assert st == 1
+    bad = Df.copy()
+    bad[0, 0] = np.nan
+    st = L.occxc_xc_gradient(ctx.handle, bad.ctypes.data_as(_lib._dp), 3, _lib.iptr(sa),
+                             G.ctypes.data_as(_lib._dp))
+    assert st == 1
+    batch = XCBatch([basis, basis], [(pts, wts), (pts, wts)], functional_ids("svwn5"))
+    st = L.occxc_xc_gradient(batch.handle, Df.ctypes.data_as(_lib._dp), 3, _lib.iptr(sa),
+                             G.ctypes.data_as(_lib._dp))
+    assert st == 1 and b"single-molecule" in L.occxc_last_error_message()

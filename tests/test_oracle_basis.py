@@ -100,6 +100,51 @@ def test_collocation_matches_golden_fixture():
         assert np.abs(mine - ref).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300)
 
 
+def test_hessian_matches_gau2grid_library():
+    if not oracle.gau2grid_available():
+        pytest.skip("reference gau2grid not built here (no OCC sources)")
+    for l, pure, exps, coefs, cen, xyz in _collocation_cases():
+        mine = oracle.eval_shell2(l, pure, exps, coefs, cen, xyz)
+        ref = oracle.gau2grid_deriv2(l, pure, exps, coefs, cen, xyz) * oracle.common_fac(l, pure)
+        assert np.abs(mine - ref).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300)
+
+
+def test_hessian_matches_golden_fixture():
+    """phi_xx..phi_zz of gg_collocation_deriv2 (gau2grid_deriv2.c), written by
+    tests/golden/make_gau2grid_fixture.py."""
+    z = np.load(GOLDEN)
+    for k in range(int(z["ncase"])):
+        l, pure = int(z[f"l{k}"]), int(z[f"pure{k}"])
+        mine = oracle.eval_shell2(l, pure, z[f"exps{k}"], z[f"coefs{k}"], z[f"center{k}"],
+                                  z[f"xyz{k}"])
+        fac = oracle.common_fac(l, pure)
+        ref1 = z[f"phi{k}"] * fac
+        ref2 = z[f"hess{k}"] * fac
+        assert np.abs(mine[:4] - ref1).max() <= 1e-13 * max(np.abs(ref1).max(), 1e-300)
+        assert np.abs(mine[4:] - ref2).max() <= 1e-13 * max(np.abs(ref2).max(), 1e-300)
+
+
+def test_hessian_finite_difference(rng):
+    """d_j of the gradient component i equals the (i, j) Hessian entry."""
+    pairs = {(0, 0): 4, (0, 1): 5, (0, 2): 6, (1, 1): 7, (1, 2): 8, (2, 2): 9}
+    for l in range(5):
+        for pure in (0, 1):
+            exps = np.array([0.8, 2.5])
+            coefs = np.array([0.6, 0.4])
+            cen = np.array([0.1, -0.2, 0.3])
+            xyz = rng.normal(size=(20, 3))
+            v = oracle.eval_shell2(l, pure, exps, coefs, cen, xyz)
+            np.testing.assert_allclose(v[:4], oracle.eval_shell(l, pure, exps, coefs, cen, xyz),
+                                       rtol=1e-14, atol=1e-15)
+            h = 1e-5
+            for (i, j), q in pairs.items():
+                e = np.zeros(3)
+                e[j] = h
+                fd = (oracle.eval_shell(l, pure, exps, coefs, cen, xyz + e)[1 + i] -
+                      oracle.eval_shell(l, pure, exps, coefs, cen, xyz - e)[1 + i]) / (2 * h)
+                np.testing.assert_allclose(v[q], fd, atol=1e-7)
+
+
 def test_collocation_gradient_finite_difference(rng):
     for l in range(5):
         exps = np.array([0.8, 2.5])

@@ -1,0 +1,78 @@
+"""Pins the oracle's XC nuclear gradient (compute_xc_gradient_impl,
+include/occ/dft/dft.h:598-749; process_grid_block_gradient,
+src/dft/dft_gradient_kernels.cpp:473-552) by what the mathematics fixes.
+
+The reference's gradient is the derivative of Exc with respect to the basis
+function centres of each atom with D, the grid points and the weights held
+fixed (no grid-weight derivative terms), so a central difference of the
+oracle's own Exc over shifted shell origins pins every factor and sign of the
+restated formula; the Hessian collocation it uses is pinned separately
+against gau2grid in test_oracle_basis.py.  Also: screened == dense
+(tests/dft_tests.cpp:904-931 restated for the gradient) and the input checks.
+"""
+import numpy as np
+import pytest
+
+import grid_ref
+import oracle
+from conftest import random_density
+from occ_amd import AOBasis, molecules
+from occ_amd.grid import GridSettings, MolecularGrid
+
+METHODS = {"svwn5": [1, 7], "pbe": [101, 130], "blyp": [106, 131]}
+
+
+@pytest.fixture(scope="module", params=[("6-31g", False), ("def2-svp", True)])
+def water(request):
+    name, pure = request.param
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, name, pure=pure)
+    g = MolecularGrid(atoms, GridSettings(radial_points=41, angular_points=146))
+    pts, w = grid_ref.molecular_grid(g)
+    return basis, pts, w
+
+
+def _shifted(flat, shell_atom, atom, k, h):
+    f = dict(flat)
+    o = np.array(flat["origin"], dtype=float).reshape(-1, 3).copy()
+    o[np.asarray(shell_atom) == atom, k] += h
+    f["origin"] = o.reshape(-1)
+    return oracle.OracleBasis.from_flat(f)
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+def test_gradient_is_the_derivative_of_exc(water, method):
+    basis, pts, w = water
+    flat = basis.flat()
+    ob = oracle.OracleBasis.from_flat(flat)
+    D = random_density(basis.nbf, 5, 4)
+    sa = basis.shell_atom
+    G = oracle.xc_gradient(ob, sa, 3, pts, w, METHODS[method], D, screened=False)
+    h = 1e-4
+    for A in range(3):
+        for k in range(3):
+            _, ep, _ = oracle.xc_build(_shifted(flat, sa, A, k, h), pts, w, METHODS[method], D,
+                                       screened=False)
+            _, em, _ = oracle.xc_build(_shifted(flat, sa, A, k, -h), pts, w, METHODS[method], D,
+                                       screened=False)
+            assert G[k, A] == pytest.approx((ep - em) / (2 * h), abs=1e-7)
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+def test_gradient_screened_equals_dense(water, method):
+    basis, pts, w = water
+    ob = oracle.OracleBasis.from_flat(basis.flat())
+    D = random_density(basis.nbf, 5, 7)
+    G1 = oracle.xc_gradient(ob, basis.shell_atom, 3, pts, w, METHODS[method], D, screened=True)
+    G2 = oracle.xc_gradient(ob, basis.shell_atom, 3, pts, w, METHODS[method], D, screened=False)
+    assert np.abs(G1 - G2).max() < 1e-8
+
+
+def test_gradient_rejects_bad_atoms(water):
+    basis, pts, w = water
+    ob = oracle.OracleBasis.from_flat(basis.flat())
+    D = random_density(basis.nbf, 5, 7)
+    bad = np.array(basis.shell_atom)
+    bad[0] = 3
+    with pytest.raises(ValueError):
+        oracle.xc_gradient(ob, bad, 3, pts, w, [1, 7], D)
