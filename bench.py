@@ -113,6 +113,8 @@ def main():
 
     if args.workload == "batch512":
         return run_batch(args)
+    if args.workload == "grad32":
+        return run_gradient(args)
     if args.workload == "serve":
         return run_serving(args)
     atoms, bname, pure, method, nocc, desc = workload(args.workload)
@@ -243,6 +245,99 @@ def main():
         "cold_build_ms": R["cold_ms"],
         "accuracy": {m: {k: results[m].get(k) for k in ("dexc", "dvxc", "dnelec")}
                      for m in results},
+        "modes_ms": {m: results[m]["ms_per_step"] for m in results},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_gradient(args):
+    """XC nuclear gradient (compute_xc_gradient) of the (H2O)32 / def2-SVP /
+    PBE workload: a step is one gradient from a resident context (D in ->
+    3 x 96 gradient out: D2, the density stage and the gradient kernels); the
+    first call, which evaluates the AO Hessian once, is a warm-up."""
+    from occ_amd import AOBasis
+    from occ_amd.dft import XCContext, functional_ids
+    from occ_amd.device import synchronize
+    from occ_amd.grid import GridSettings, MolecularGrid
+    atoms, bname, pure, method, nocc, desc = workload("h2o32")
+    basis = AOBasis.load(atoms, bname, pure=pure)
+    nbf = basis.nbf
+    grid = MolecularGrid(atoms, GridSettings(radial_points=76, angular_points=302))
+    pts, wts = grid.build(device=0)
+    rng = np.random.default_rng(0)
+    C, _ = np.linalg.qr(rng.normal(size=(nbf, nocc)))
+    D = np.asfortranarray(C @ C.T)
+    ids = functional_ids(method)
+    sa, nat = basis.shell_atom, len(atoms)
+    ref, cpu = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        ob = oracle.OracleBasis.from_flat(basis.flat())
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        times = []
+        for _ in range(max(1, min(args.cpu_repeats, 3))):
+            t0 = time.time()
+            ref = oracle.xc_gradient(ob, sa, nat, pts, wts, ids, D, nthreads=threads)
+            times.append(time.time() - t0)
+        dt = float(np.median(times))
+        cpu = {"value": 1.0 / dt, "unit": "XC gradients/s", "cores": threads, "kind": "port",
+               "sample": f"full FP64 screened XC gradients of the same workload on the CPU "
+                         f"oracle (oracle/occxc_oracle.c: the reference's per-leaf V_x, V_y, "
+                         f"V_z matrices and distribution, OpenMP over leaves); median of "
+                         f"{len(times)} on {threads} threads = {dt:.2f} s",
+               "cpu_model": cpu_model()}
+        log(f"cpu oracle gradient: {dt:.2f}s on {threads} threads")
+    from occ_amd.device import DeviceArray
+    d_D = DeviceArray(nbf * nbf)
+    d_D.upload(D)
+    d_G = DeviceArray(3 * nat)
+    results = {}
+    for mode in [m for m in args.modes.split(",") if m]:
+        ctx = XCContext(basis, pts, wts, ids, precision=mode)
+        info = ctx.info()
+        for _ in range(max(args.warmup, 1)):
+            ctx.xc_gradient_device(d_D.ptr, sa, nat, d_G.ptr)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.xc_gradient_device(d_D.ptr, sa, nat, d_G.ptr)
+        ctx.synchronize()
+        synchronize()
+        wall = time.perf_counter() - t0
+        G = d_G.download((3, nat))
+        tim = ctx.timing()
+        ms = wall / args.steps * 1e3
+        r = {"ms_per_step": ms, "per_s": 1e3 / ms, "info": info, "timing": tim}
+        if ref is not None:
+            r["dgrad"] = float(np.abs(G - ref).max())
+        log(f"{mode}: {ms:.3f} ms/gradient, stages {tim}, max|dG|={r.get('dgrad')}")
+        results[mode] = r
+        ctx.close()
+    # headline: FP64, the reference's precision (the other modes are reported
+    # with their accuracy)
+    head = "fp64" if "fp64" in results else min(results, key=lambda m: results[m]["ms_per_step"])
+    R = results[head]
+    tim = R["timing"]
+    gms = tim["gradient"][0] / max(tim["gradient"][1], 1)
+    flops = R["info"]["contraction_flops"]   # T = Phi P and U = Aow P: 2 x 2 n m^2
+    achieved = flops / (gms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": "gradient", "achieved": achieved,
+            "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
+            "traffic": measured_traffic("grad32", head, "gradient"), "kernel_ms": gms,
+            "work": "sum over leaves of 4 n_b m_b^2 (T = Phi P and U = Aow P, screened m_b)"}
+    out = {
+        "metric": "XC nuclear gradients/s (FP64)",
+        "value": R["per_s"], "unit": "XC gradients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[head], "data": "synthetic",
+        "config": {"workload": "XC gradient of the " + desc, "npts": len(wts), "nbf": nbf,
+                   "natoms": nat, "leaves": R["info"]["nleaves"], "precision": head},
+        "roofline": roof,
+        "accuracy": {m: results[m].get("dgrad") for m in results},
         "modes_ms": {m: results[m]["ms_per_step"] for m in results},
         "cpu_baseline": cpu,
     }

@@ -152,6 +152,14 @@ occxc_status occxc_build_device(occxc_context *ctx, const double *d_D,
 occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
                                const int *shell_atom, double *grad);
 
+/* Same gradient on device-resident buffers (d_D nbf*nbf, d_grad 3*natoms),
+ * enqueued on `stream` (NULL: the context's own stream); shell_atom stays a
+ * host array.  The first call for a given shell_atom map waits for its
+ * set-up; later calls do not synchronise the host.  D is not checked for
+ * non-finite entries on this path. */
+occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int natoms,
+                                      const int *shell_atom, double *d_grad, void *stream);
+
 /* --- batches: many molecules in one context, one launch per stage ----- */
 
 /* A context over nmol molecules (each its own basis and grid; shared

@@ -77,6 +77,8 @@ SIGNATURES = {
     "occxc_context_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CContextInfo)]),
     "occxc_build": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "occxc_xc_gradient": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _ip, _dp]),
+    "occxc_xc_gradient_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                _ip, ctypes.c_void_p, ctypes.c_void_p]),
     "occxc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CBasis),
                                           ctypes.POINTER(CGrid), ctypes.POINTER(CFunctional),
                                           ctypes.POINTER(COptions),

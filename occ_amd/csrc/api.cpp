@@ -661,6 +661,7 @@ void gradient_setup(occxc_context *c, int natoms, const int *shell_atom) {
     StageTimer t{c, c->stream, {}};
     t.run(OCCXC_STAGE_AO,
           [&] { launch_ao_extra(c->args, c->d_ao2.get(), c->deriv != 0, c->stream); });
+    OCCXC_HIP(hipStreamSynchronize(c->stream));
     c->ao2_ready = true;
   }
   const bool same = c->grad_natoms == natoms && (int)c->grad_shell_atom.size() == c->nsh &&
@@ -688,8 +689,8 @@ void gradient_setup(occxc_context *c, int natoms, const int *shell_atom) {
 }
 
 // One XC gradient on stream s: D2, the density stage (coefficients, leaf
-// skip), then the gradient contraction and the per-atom reduction.
-void run_gradient(occxc_context *c, const double *d_D, hipStream_t s) {
+// skip), then the gradient contraction and the per-atom reduction into d_grad.
+void run_gradient(occxc_context *c, const double *d_D, double *d_grad, hipStream_t s) {
   const BuildArgs &a = c->args;
   StageTimer t{c, s, {}};
   t.run(OCCXC_STAGE_PREPARE, [&] {
@@ -704,7 +705,7 @@ void run_gradient(occxc_context *c, const double *d_D, hipStream_t s) {
   t.run(OCCXC_STAGE_GRADIENT, [&] {
     launch_xc_gradient(a, c->d_ao.get(), c->opt.precision != OCCXC_PREC_FP64, c->d_ao2.get(),
                        c->d_gpart.get(), c->grad_natoms, c->d_gatom_off.get(),
-                       c->d_gatom_ent.get(), c->d_grad.get(), s);
+                       c->d_gatom_ent.get(), d_grad, s);
   });
 }
 
@@ -924,9 +925,27 @@ occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
     gradient_setup(ctx, natoms, shell_atom);
     OCCXC_HIP(hipMemcpyAsync(ctx->d_D.get(), D, nn * sizeof(double), hipMemcpyHostToDevice,
                              ctx->stream));
-    run_gradient(ctx, ctx->d_D.get(), ctx->stream);
+    run_gradient(ctx, ctx->d_D.get(), ctx->d_grad.get(), ctx->stream);
     ctx->d_grad.download(grad, 3 * (size_t)natoms, ctx->stream);
     OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int natoms,
+                                      const int *shell_atom, double *d_grad, void *stream) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && d_D && shell_atom && d_grad, OCCXC_ERR_INVALID_ARGUMENT,
+                  "null pointer");
+    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "occxc_xc_gradient needs a single-molecule context");
+    OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
+    for (int k = 0; k < ctx->nsh; ++k)
+      OCCXC_REQUIRE(shell_atom[k] >= 0 && shell_atom[k] < natoms, OCCXC_ERR_INVALID_ARGUMENT,
+                    "shell_atom: atom index out of range");
+    DeviceGuard g(ctx->device);
+    gradient_setup(ctx, natoms, shell_atom);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    run_gradient(ctx, d_D, d_grad, s);
   });
 }
 

@@ -118,6 +118,15 @@ class XCContext:
                                                 _lib.iptr(sa), G.ctypes.data_as(_lib._dp)))
         return G
 
+    def xc_gradient_device(self, d_D, shell_atom, natoms, d_grad, stream=None):
+        """occxc_xc_gradient_device: D and the 3 x natoms gradient in device
+        memory (pointers), enqueued without host synchronisation."""
+        self._sa = _lib.i32(shell_atom)
+        _lib.check(_lib.lib().occxc_xc_gradient_device(self._h, ctypes.c_void_p(d_D),
+                                                       int(natoms), _lib.iptr(self._sa),
+                                                       ctypes.c_void_p(d_grad),
+                                                       ctypes.c_void_p(stream or 0)))
+
     STAGES = ("prepare", "density", "vxc", "finalize", "ao", "gradient")
 
     def set_timing(self, enable=True):

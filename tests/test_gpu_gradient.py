@@ -120,6 +120,14 @@ def test_gradient_repeats_and_leaves_the_build_alone():
     ctx.set_graph(True)
     ctx.build(D)
     assert np.array_equal(ctx.xc_gradient(D, basis.shell_atom, len(atoms)), G1)
+    # the device-resident entry point gives the same numbers
+    from occ_amd.device import DeviceArray
+    d_D = DeviceArray(basis.nbf ** 2)
+    d_D.upload(D)
+    d_G = DeviceArray(3 * len(atoms))
+    ctx.xc_gradient_device(d_D.ptr, basis.shell_atom, len(atoms), d_G.ptr)
+    ctx.synchronize()
+    assert np.array_equal(d_G.download((3, len(atoms))), G1)
     # a different shell-to-atom map (all shells on atom 0) sums to the total
     G3 = ctx.xc_gradient(D, np.zeros(len(basis.shell_atom), dtype=np.int32), 1)
     np.testing.assert_allclose(G3[:, 0], G1.sum(axis=1), rtol=0, atol=1e-12)
