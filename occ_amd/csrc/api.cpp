@@ -234,6 +234,131 @@ static Hierarchy build_hierarchy(long long n, const double *xyz, const double *w
 
 static long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
+// Temporaries of the device set-up, alive until the context is set up.
+struct DeviceSetup {
+  DevBuf<double> xyz, w, center, radius, part, box;
+  DevBuf<int> perm, nsig, nao, idx;
+  DevBuf<unsigned long long> keys_in, keys_out;
+  DevBuf<char> temp;
+  DevBuf<FillLeaf> fill;
+};
+
+// The screened leaf layout of every molecule built on the device (setup.hip):
+// raw points uploaded once, per molecule the Morton sort, leaf spheres and
+// per-leaf screening counts; the host turns the counts into the LeafDev
+// table (prefix sums over the active leaves), and one kernel writes the
+// leaf-shell lists, the AO maps and the padded points.  Gives the same
+// layout as the host path below.
+static void setup_leaves_device(occxc_context *c, int nmol, const occxc_grid *grids,
+                                const std::vector<HostBasis> &hbs,
+                                const std::vector<int> &shell_base, std::vector<MolDev> &mols,
+                                const double *d_ext, DeviceSetup &ds, long long &pt_off,
+                                long long &ao_off, double &msum) {
+  hipStream_t s = c->stream;
+  std::vector<long long> pbase(nmol + 1, 0);
+  std::vector<int> lbase(nmol + 1, 0);
+  long long nmax = 0;
+  for (int m = 0; m < nmol; ++m) {
+    pbase[m + 1] = pbase[m] + grids[m].npts;
+    lbase[m + 1] = lbase[m] + hierarchy_leaf_count(grids[m].npts);
+    nmax = std::max<long long>(nmax, grids[m].npts);
+  }
+  const long long N = pbase[nmol];
+  const int Lall = lbase[nmol];
+  ds.xyz.alloc(std::max<long long>(3 * N, 1));
+  ds.w.alloc(std::max<long long>(N, 1));
+  for (int m = 0; m < nmol; ++m) {
+    if (grids[m].npts) {
+      OCCXC_HIP(hipMemcpyAsync(ds.xyz.get() + 3 * pbase[m], grids[m].points,
+                               3 * grids[m].npts * sizeof(double), hipMemcpyHostToDevice, s));
+      OCCXC_HIP(hipMemcpyAsync(ds.w.get() + pbase[m], grids[m].weights,
+                               grids[m].npts * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+  }
+  ds.perm.alloc(std::max<long long>(N, 1));
+  ds.center.alloc(std::max(3 * Lall, 1));
+  ds.radius.alloc(std::max(Lall, 1));
+  ds.nsig.alloc(std::max(Lall, 1));
+  ds.nao.alloc(std::max(Lall, 1));
+  ds.keys_in.alloc(std::max<long long>(nmax, 1));
+  ds.keys_out.alloc(std::max<long long>(nmax, 1));
+  ds.idx.alloc(std::max<long long>(nmax, 1));
+  const size_t tb = std::max<size_t>(sort_temp_bytes(nmax), 1);
+  ds.temp.alloc(tb);
+  ds.part.alloc(256 * 6);
+  ds.box.alloc(6);
+  for (int m = 0; m < nmol; ++m) {
+    const long long n = grids[m].npts;
+    const int nl = lbase[m + 1] - lbase[m];
+    if (n == 0) continue;
+    launch_hierarchy(ds.xyz.get() + 3 * pbase[m], n, nl, ds.part.get(), ds.box.get(),
+                     ds.keys_in.get(), ds.keys_out.get(), ds.idx.get(), ds.perm.get() + pbase[m],
+                     ds.temp.get(), tb, ds.center.get() + 3 * lbase[m], ds.radius.get() + lbase[m],
+                     s);
+    launch_screen_count(ds.center.get() + 3 * lbase[m], ds.radius.get() + lbase[m], nl,
+                        c->d_shells.get() + shell_base[m], d_ext + shell_base[m], hbs[m].nsh,
+                        ds.nsig.get() + lbase[m], ds.nao.get() + lbase[m], s);
+  }
+  std::vector<int> nsig(Lall), nao(Lall);
+  ds.nsig.download(nsig.data(), Lall, s);
+  ds.nao.download(nao.data(), Lall, s);
+  OCCXC_HIP(hipStreamSynchronize(s));
+  std::vector<FillLeaf> fill;
+  long long sh_total = 0, map_total = 0;
+  for (int m = 0; m < nmol; ++m) {
+    const long long n = grids[m].npts;
+    const int nl = lbase[m + 1] - lbase[m];
+    mols[m].leaf_begin = (int)c->leaves.size();
+    for (int li = 0; li < nl; ++li) {
+      const int g = lbase[m] + li;
+      if (nsig[g] == 0) continue;
+      LeafDev L{};
+      L.n = (int)(li < nl - 1 ? 128 : n - 128LL * li);
+      L.np = (int)round_up(L.n, kPtPad);
+      L.pt_off = (int)pt_off;
+      L.sh_off = (int)sh_total;
+      L.nsig = nsig[g];
+      L.map_off = (int)map_total;
+      L.mol = m;
+      L.m = nao[g];
+      OCCXC_REQUIRE(L.m <= kMaxLocalAo, OCCXC_ERR_UNSUPPORTED,
+                    "a grid leaf has more than " + std::to_string(kMaxLocalAo) +
+                        " significant basis functions");
+      L.mp = (int)round_up(L.m, kAoPad);
+      L.ao_off = ao_off;
+      ao_off += (long long)c->ncomp * L.mp * L.np;
+      L.kl_off = c->kloc_elems;
+      c->kloc_elems += (long long)L.mp * L.mp;
+      sh_total += L.nsig;
+      map_total += L.mp;
+      pt_off += L.np;
+      msum += L.m;
+      c->flops += 4.0 * L.n * (double)L.m * L.m;
+      c->ao_write_bytes +=
+          (double)L.n * L.m * c->ncomp * (c->opt.precision == 0 ? 8.0 : 4.0) + 32.0 * L.n;
+      c->leaves.push_back(L);
+      fill.push_back(FillLeaf{g, shell_base[m], hbs[m].nsh, 0, pbase[m] + 128LL * li, pbase[m]});
+    }
+    mols[m].leaf_end = (int)c->leaves.size();
+  }
+  OCCXC_REQUIRE(map_total < (1LL << 31) && pt_off < (1LL << 31), OCCXC_ERR_UNSUPPORTED,
+                "context too large for 32-bit leaf offsets");
+  c->d_leaves.alloc(c->leaves.size());
+  c->d_leaves.upload(c->leaves.data(), c->leaves.size(), s);
+  ds.fill.alloc(fill.size());
+  ds.fill.upload(fill.data(), fill.size(), s);
+  c->d_leaf_shells.alloc(sh_total);
+  c->d_ao_map.alloc(map_total);
+  c->d_px.alloc(pt_off);
+  c->d_py.alloc(pt_off);
+  c->d_pz.alloc(pt_off);
+  c->d_pw.alloc(pt_off);
+  launch_fill_leaves(c->d_leaves.get(), ds.fill.get(), (int)fill.size(), ds.center.get(),
+                     ds.radius.get(), c->d_shells.get(), d_ext, ds.xyz.get(), ds.w.get(),
+                     ds.perm.get(), c->d_leaf_shells.get(), c->d_ao_map.get(), c->d_px.get(),
+                     c->d_py.get(), c->d_pz.get(), c->d_pw.get(), s);
+}
+
 } // namespace occxc
 
 using namespace occxc;
@@ -323,18 +448,29 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   // shell extents on the device (gto.cpp:18-67)
   std::vector<double> ext(shells.size(), 30.0);
   const bool screened = opt->screen != 0 && opt->screening_threshold > 0.0;
+  // the screened layout is built on the device unless OCCXC_HOST_SETUP is set
+  // (the host restatement, kept to check that both give the same layout)
+  const bool device_setup = screened && std::getenv("OCCXC_HOST_SETUP") == nullptr;
+  DevBuf<double> d_ext;
   if (screened) {
-    DevBuf<double> d_ext(shells.size());
+    d_ext.alloc(shells.size());
     launch_decay_cutoff(c->d_shells.get(), (int)shells.size(), c->d_exps.get(), c->d_coefs.get(),
                         opt->screening_threshold, d_ext.get(), c->stream);
-    d_ext.download(ext.data(), shells.size(), c->stream);
-    OCCXC_HIP(hipStreamSynchronize(c->stream));
+    if (!device_setup) {
+      d_ext.download(ext.data(), shells.size(), c->stream);
+      OCCXC_HIP(hipStreamSynchronize(c->stream));
+    }
   }
 
   std::vector<int> leaf_shells, ao_map;
   std::vector<double> px, py, pz, pw;
   long long pt_off = 0, ao_off = 0;
   double msum = 0.0;
+  DeviceSetup dsetup;
+  if (device_setup) {
+    setup_leaves_device(c, nmol, grids, hbs, shell_base, mols, d_ext.get(), dsetup, pt_off,
+                        ao_off, msum);
+  } else {
   // per molecule: leaves (Morton hierarchy when screened, else 128-point
   // chunks in input order) and their significant shells (dft.h:314-320);
   // molecules are independent, so a batch builds them on several threads
@@ -430,15 +566,6 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     }
     mols[mol].leaf_end = (int)c->leaves.size();
   }
-  c->nleaves = (int)c->leaves.size();
-  c->ptot = pt_off;
-  c->mean_m = c->nleaves ? msum / c->nleaves : 0.0;
-  c->ao_elems = (size_t)ao_off;
-  c->ao_elem_size = opt->precision == 0 ? 8 : 4;
-
-  // device layout
-  c->d_mols.alloc(mols.size());
-  c->d_mols.upload(mols.data(), mols.size(), c->stream);
   c->d_leaves.alloc(c->leaves.size());
   c->d_leaves.upload(c->leaves.data(), c->leaves.size(), c->stream);
   c->d_leaf_shells.alloc(leaf_shells.size());
@@ -453,6 +580,16 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   c->d_pz.upload(pz.data(), pz.size(), c->stream);
   c->d_pw.alloc(pw.size());
   c->d_pw.upload(pw.data(), pw.size(), c->stream);
+  }
+  c->nleaves = (int)c->leaves.size();
+  c->ptot = pt_off;
+  c->mean_m = c->nleaves ? msum / c->nleaves : 0.0;
+  c->ao_elems = (size_t)ao_off;
+  c->ao_elem_size = opt->precision == 0 ? 8 : 4;
+
+  // device layout
+  c->d_mols.alloc(mols.size());
+  c->d_mols.upload(mols.data(), mols.size(), c->stream);
   const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
   // device memory the context will hold: AO values, the nbf^2 matrices
   // (D, D2, K, V and the FP32 copies), per-point data and the leaf tables
@@ -519,6 +656,11 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream);
   c->plan.two_pass = c->plan.two_pass && c->plan.use_mfma;
   if (c->plan.two_pass) {
+    if (ao_map.empty() && c->d_ao_map.size()) {
+      ao_map.resize(c->d_ao_map.size());
+      c->d_ao_map.download(ao_map.data(), ao_map.size(), c->stream);
+      OCCXC_HIP(hipStreamSynchronize(c->stream));
+    }
     // 32 x 32 AO tile pairs (TI <= TJ) per molecule and, for each, the leaves
     // whose AOs touch both tiles (in leaf order); a leaf's AOs in one tile are
     // a run of its (sorted) local AOs, recorded as a start and a bit mask

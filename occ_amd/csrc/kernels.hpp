@@ -80,6 +80,18 @@ struct MolDev {
   int pad_;
 };
 
+// Device set-up: for one active leaf, its leaf in the all-leaves sphere
+// arrays, its molecule's first shell and shell count, the offset of its
+// first sorted point in the permutation array and of its molecule's points.
+struct FillLeaf {
+  int src_leaf;
+  int sb;
+  int nsh;
+  int pad_;
+  long long src_pt;
+  long long mol_pt;
+};
+
 struct BuildArgs {
   int nmol;
   int maxnbf;     // largest nbf over the molecules
@@ -113,6 +125,22 @@ struct BuildArgs {
   int ntchunks;
   double *tpartial;     // [ntchunks][32 * 32] partial blocks
 };
+
+// device set-up of the screened layout (setup.hip)
+int hierarchy_leaf_count(long long n);
+size_t sort_temp_bytes(long long nmax);
+void launch_hierarchy(const double *xyz, long long n, int nleaves, double *part, double *box,
+                      unsigned long long *keys_in, unsigned long long *keys_out, int *idx_in,
+                      int *perm, void *temp, size_t temp_bytes, double *center, double *radius,
+                      hipStream_t s);
+void launch_screen_count(const double *center, const double *radius, int nleaves,
+                         const ShellDev *shells, const double *ext, int nsh, int *nsig, int *nao,
+                         hipStream_t s);
+void launch_fill_leaves(const LeafDev *leaves, const FillLeaf *fill, int nactive,
+                        const double *center, const double *radius, const ShellDev *shells,
+                        const double *ext, const double *xyz, const double *w, const int *perm,
+                        int *leaf_shells, int *ao_map, double *px, double *py, double *pz,
+                        double *pw, hipStream_t s);
 
 // host-callable launchers (kernels.hip)
 void upload_sph_table();

@@ -259,6 +259,55 @@ def test_density_variants_agree(monkeypatch):
         assert np.abs(out[0][0] - out[k][0]).max() <= 1e-12
 
 
+def test_device_setup_matches_host_setup(monkeypatch):
+    """The leaf layout built on the device (Morton sort, leaf spheres,
+    screening, leaf tables) is the host restatement's, bit for bit: same
+    leaves and work, identical builds.  Includes grids of <= 256 points (one
+    leaf) and a batch of molecules."""
+    from occ_amd.dft import XCBatch, XCContext, functional_ids
+
+    def both(make):
+        out = []
+        for host in (True, False):
+            if host:
+                monkeypatch.setenv("OCCXC_HOST_SETUP", "1")
+            else:
+                monkeypatch.delenv("OCCXC_HOST_SETUP", raising=False)
+            out.append(make())
+        return out
+
+    ids = functional_ids("pbe")
+    cases = [(molecules.water_cluster(4, seed=2), 31, 110), (molecules.water(), 3, 26),
+             (molecules.water(), 11, 26)]
+    for atoms, nrad, nang in cases:
+        basis = AOBasis.load(atoms, "def2-svp", pure=True)
+        pts, wts = _grid(atoms, nrad, nang)
+        D = random_density(basis.nbf, 5, 70)
+        print(f"set-up case: {len(atoms)} atoms, {len(wts)} points", flush=True)
+
+        def make():
+            ctx = XCContext(basis, pts, wts, ids)
+            V, e, n = ctx.build(D)
+            return ctx.info(), V, e, n
+        (i0, V0, e0, n0), (i1, V1, e1, n1) = both(make)
+        for k in ("nleaves", "mean_local_nbf", "contraction_flops", "ao_bytes", "ao_write_bytes"):
+            assert i0[k] == i1[k], k
+        assert e0 == e1 and n0 == n1 and np.abs(V0 - V1).max() <= 1e-14
+    mols = [molecules.water_cluster(k, seed=k) for k in (1, 2, 3)]
+    bases = [AOBasis.load(a, "6-31g", pure=False) for a in mols]
+    grids = [_grid(a, 21, 50) for a in mols]
+    Ds = [random_density(b.nbf, 5, 71 + k) for k, b in enumerate(bases)]
+
+    def make_batch():
+        bt = XCBatch(bases, grids, ids)
+        return bt.info(), bt.build(Ds)
+    (i0, r0), (i1, r1) = both(make_batch)
+    assert i0["nleaves"] == i1["nleaves"]
+    assert np.array_equal(r0[1], r1[1]) and np.array_equal(r0[2], r1[2])
+    for a, b in zip(r0[0], r1[0]):
+        assert np.abs(a - b).max() <= 1e-14
+
+
 def test_graph_replay_matches_direct_builds():
     """Builds replayed from the captured graph give the direct builds' results,
     on every replay, with eager work and synchronisation between replays, and
