@@ -590,6 +590,161 @@ int oracle_functional_eval(int id, long n, const double *rho,
   return 0;
 }
 
+/* ------------------------------------------------ polarised functionals -- */
+/* Spin-polarised energy densities e(rho_a, rho_b, s_aa, s_ab, s_bb), the
+ * inputs libxc's polarised functionals take (functional.h:399-501 with
+ * SpinorbitalKind::Unrestricted), again in complex arithmetic for
+ * complex-step derivatives.
+ *  - exchange (lda_x, gga_x_pbe, gga_x_b88): exact spin scaling,
+ *    E_x[ra, rb] = E_x[2 ra]/2 + E_x[2 rb]/2 (Oliver, Perdew, PRA 20, 397
+ *    (1979)), applied to the closed-shell forms above;
+ *  - VWN5: Vosko, Wilk, Nusair eq. 4.4 with the paramagnetic, ferromagnetic
+ *    and spin-stiffness fits and the interpolation
+ *    eps = eP + ac f(z)/f''(0) (1 - z^4) + (eF - eP) f(z) z^4;
+ *  - PW92 (and its "mod" constants): Perdew, Wang eqs. 8-10, Table I;
+ *  - PBE correlation: PRL 77, 3865 with phi(z) = ((1+z)^(2/3) + (1-z)^(2/3))/2;
+ *  - LYP: Miehlich et al. eq. 2 in its general (ra, rb, grad ra, grad rb)
+ *    form (e_lyp above is its closed-shell case).
+ * Powers of (1 +- z) below DBL_EPSILON are held at DBL_EPSILON^n (libxc's
+ * zeta threshold). */
+
+#define ORACLE_ZETA_THR 2.220446049250313e-16
+
+static cplx opz_pow(cplx x, double n) {
+  if (creal(x) <= ORACLE_ZETA_THR) return pow(ORACLE_ZETA_THR, n);
+  return cpow(x, n);
+}
+static cplx f_zeta(cplx z) {
+  return (opz_pow(1.0 + z, 4.0 / 3.0) + opz_pow(1.0 - z, 4.0 / 3.0) - 2.0) /
+         (pow(2.0, 4.0 / 3.0) - 2.0);
+}
+/* PW92 eq. 10: G(rs; A, a1, b1..b4), p = 1 */
+static cplx pw92_G(cplx rs, double A, double a1, double b1, double b2, double b3, double b4) {
+  cplx srs = csqrt(rs);
+  cplx den = 2.0 * A * (b1 * srs + b2 * rs + b3 * rs * srs + b4 * rs * rs);
+  return -2.0 * A * (1.0 + a1 * rs) * c_log1p(1.0 / den);
+}
+/* eps_c(rs, z) of PW92; mod = 1 uses libxc's "mod" constants */
+static cplx eps_pw92_pol(cplx rs, cplx z, int mod) {
+  const double A0 = mod ? 0.0310907 : 0.031091, A1 = mod ? 0.01554535 : 0.015545,
+               Aa = mod ? 0.0168869 : 0.016887;
+  const double fpp = mod ? 1.709920934161365617563962776245 : 1.709921;
+  cplx e0 = pw92_G(rs, A0, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  cplx e1 = pw92_G(rs, A1, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  cplx mac = pw92_G(rs, Aa, 0.11125, 10.357, 3.6231, 0.88026, 0.49671); /* -alpha_c */
+  cplx f = f_zeta(z), z4 = z * z * z * z;
+  return e0 - mac * f / fpp * (1.0 - z4) + (e1 - e0) * f * z4;
+}
+static cplx eps_vwn_pol(cplx rs, cplx z) {
+  const double fpp = 4.0 / (9.0 * (cbrt(2.0) - 1.0));
+  cplx eP = eps_vwn(rs, 0.0310907, 3.72744, 12.9352, -0.10498);
+  cplx eF = eps_vwn(rs, 0.01554535, 7.06042, 18.0578, -0.32500);
+  cplx ac = eps_vwn(rs, -1.0 / (6.0 * M_PI * M_PI), 1.13107, 13.0045, -0.0047584);
+  cplx f = f_zeta(z), z4 = z * z * z * z;
+  return eP + ac * f / fpp * (1.0 - z4) + (eF - eP) * f * z4;
+}
+static cplx e_pbe_c_pol(cplx ra, cplx rb, cplx sigma) {
+  double gamma = (1.0 - log(2.0)) / (M_PI * M_PI);
+  cplx rho = ra + rb, z = (ra - rb) / rho;
+  cplx rs = c_rs(rho);
+  cplx ec = eps_pw92_pol(rs, z, 1);
+  cplx phi = 0.5 * (opz_pow(1.0 + z, 2.0 / 3.0) + opz_pow(1.0 - z, 2.0 / 3.0));
+  cplx phi3 = phi * phi * phi;
+  cplx kf = c_cbrt(3.0 * M_PI * M_PI * rho);
+  cplx ks = csqrt(4.0 * kf / M_PI);
+  cplx t2 = sigma / (4.0 * phi * phi * ks * ks * rho * rho);
+  cplx A = (PBE_BETA / gamma) / (cexp(-ec / (gamma * phi3)) - 1.0);
+  cplx At2 = A * t2;
+  cplx H = gamma * phi3 *
+           c_log1p((PBE_BETA / gamma) * t2 * (1.0 + At2) / (1.0 + At2 + At2 * At2));
+  return rho * (ec + H);
+}
+static cplx e_lyp_pol(cplx ra, cplx rb, cplx saa, cplx sab, cplx sbb) {
+  const double a = 0.04918, b = 0.132, c = 0.2533, d = 0.349;
+  const double CF = 0.3 * pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  cplx rho = ra + rb, gg = saa + 2.0 * sab + sbb;
+  cplx rm13 = 1.0 / c_cbrt(rho);
+  cplx den = 1.0 + d * rm13;
+  cplx omega = cexp(-c * rm13) / den * cpow(rho, -11.0 / 3.0);
+  cplx delta = c * rm13 + d * rm13 / den;
+  cplx t1 = -a * 4.0 / den * ra * rb / rho;
+  cplx br = ra * rb * (pow(2.0, 11.0 / 3.0) * CF * (cpow(ra, 8.0 / 3.0) + cpow(rb, 8.0 / 3.0)) +
+                       (47.0 / 18.0 - 7.0 / 18.0 * delta) * gg -
+                       (2.5 - delta / 18.0) * (saa + sbb) -
+                       (delta - 11.0) / 9.0 * (ra / rho * saa + rb / rho * sbb));
+  br += -2.0 / 3.0 * rho * rho * gg + (2.0 / 3.0 * rho * rho - ra * ra) * sbb +
+        (2.0 / 3.0 * rho * rho - rb * rb) * saa;
+  return t1 - a * b * omega * br;
+}
+/* exchange by spin scaling; a channel below the density threshold adds 0 */
+static cplx e_x_spin(int id, cplx rs_, cplx ss) {
+  if (creal(rs_) < ORACLE_DENS_THR) return 0.0;
+  switch (id) {
+  case ORACLE_LDA_X:
+    return 0.5 * e_lda_x(2.0 * rs_);
+  case ORACLE_GGA_X_PBE:
+    return 0.5 * e_pbe_x(2.0 * rs_, 4.0 * ss);
+  case ORACLE_GGA_X_B88:
+    return 0.5 * e_b88(2.0 * rs_, 4.0 * ss);
+  }
+  return 0.0;
+}
+static cplx edens_pol(int id, cplx ra, cplx rb, cplx saa, cplx sab, cplx sbb) {
+  cplx rho = ra + rb;
+  switch (id) {
+  case ORACLE_LDA_X:
+  case ORACLE_GGA_X_PBE:
+  case ORACLE_GGA_X_B88:
+    return e_x_spin(id, ra, saa) + e_x_spin(id, rb, sbb);
+  case ORACLE_LDA_C_VWN:
+    return rho * eps_vwn_pol(c_rs(rho), (ra - rb) / rho);
+  case ORACLE_LDA_C_PW:
+    return rho * eps_pw92_pol(c_rs(rho), (ra - rb) / rho, 0);
+  case ORACLE_LDA_C_PW_MOD:
+    return rho * eps_pw92_pol(c_rs(rho), (ra - rb) / rho, 1);
+  case ORACLE_GGA_C_PBE:
+    return e_pbe_c_pol(ra, rb, saa + 2.0 * sab + sbb);
+  case ORACLE_GGA_C_LYP:
+    return e_lyp_pol(ra, rb, saa, sab, sbb);
+  }
+  return 0.0;
+}
+
+double oracle_functional_energy_density_pol(int id, double ra, double rb, double saa,
+                                            double sab, double sbb) {
+  if (ra + rb < ORACLE_DENS_THR) return 0.0;
+  if (saa < ORACLE_SIGMA_MIN) saa = ORACLE_SIGMA_MIN;
+  if (sbb < ORACLE_SIGMA_MIN) sbb = ORACLE_SIGMA_MIN;
+  return creal(edens_pol(id, ra, rb, saa, sab, sbb));
+}
+
+int oracle_functional_eval_pol(int id, long n, const double *rho, const double *sigma,
+                               double *exc, double *vrho, double *vsigma) {
+  int fam = oracle_functional_family(id);
+  if (fam < 0) return -1;
+  for (long i = 0; i < n; i++) {
+    double x[5] = {rho[2 * i], rho[2 * i + 1], 0.0, 0.0, 0.0};
+    double r = x[0] + x[1];
+    if (!(r >= ORACLE_DENS_THR)) continue;
+    if (fam == 1) {
+      x[2] = sigma[3 * i] < ORACLE_SIGMA_MIN ? ORACLE_SIGMA_MIN : sigma[3 * i];
+      x[3] = sigma[3 * i + 1];
+      x[4] = sigma[3 * i + 2] < ORACLE_SIGMA_MIN ? ORACLE_SIGMA_MIN : sigma[3 * i + 2];
+    }
+    exc[i] += creal(edens_pol(id, x[0], x[1], x[2], x[3], x[4])) / r;
+    const int nv = fam == 1 ? 5 : 2;
+    for (int k = 0; k < nv; k++) {
+      double h = 1e-20 * (fabs(x[k]) > 1e-300 ? fabs(x[k]) : 1.0);
+      cplx y[5] = {x[0], x[1], x[2], x[3], x[4]};
+      y[k] += I * h;
+      double dv = cimag(edens_pol(id, y[0], y[1], y[2], y[3], y[4])) / h;
+      if (k < 2) vrho[2 * i + k] += dv;
+      else vsigma[3 * i + k - 2] += dv;
+    }
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------- XC build -- */
 
 typedef struct {
@@ -1137,6 +1292,236 @@ int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
   }
   free(first_bf);
   free(D2);
+  free(sxyz);
+  free(sw);
+  free(perm);
+  free(loff);
+  free(lcnt);
+  free(lcen);
+  free(lrad);
+  free(ext);
+  return 0;
+}
+
+/* kernels::process_grid_block<d, Unrestricted> (dft_kernels.cpp:8-70) with
+ * evaluate_density<d, U> (density.cpp:18-30,45-70: T = phi D_s with the full
+ * spin block, no symmetrisation), set_params<U> (dft_kernels.h:16-46) and
+ * xc_potential_matrix<Unrestricted, d> (xc_potential_matrix.cpp:61-109).
+ * Dloc is [2m][m] column-major (alpha rows, then beta rows), Kblk likewise. */
+static void process_block_u(const block_cfg *cfg, int m, long n, const double *phi,
+                            const double *Dloc, const double *wts, double *Kblk,
+                            double *energy, double *nel_a, double *nel_b, double *work) {
+  const long mn = (long)m * n;
+  const double *p0 = phi, *px = phi + mn, *py = phi + 2 * mn, *pz = phi + 3 * mn;
+  double *T = work;                   /* [m][n] */
+  double *rho = T + mn;               /* [2][4][n] */
+  double *exc = rho + 8 * n;          /* [n] */
+  double *vr = exc + n;               /* [2n] interleaved a,b */
+  double *vs = vr + 2 * n;            /* [3n] interleaved aa,ab,bb */
+  double *rr = vs + 3 * n;            /* [2n] */
+  double *ss = rr + 2 * n;            /* [3n] */
+  double *G = ss + 3 * n;             /* [m][n] */
+  const int ld = 2 * m;
+  memset(rho, 0, sizeof(double) * 8 * n);
+  for (int sp = 0; sp < 2; sp++) {
+    double *r = rho + 4L * n * sp;
+    memset(T, 0, sizeof(double) * mn);
+    for (int nu = 0; nu < m; nu++) {
+      double *t = T + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double dm = Dloc[sp * m + mu + (long)nu * ld];
+        const double *f = p0 + (long)mu * n;
+        for (long p = 0; p < n; p++) t[p] += dm * f[p];
+      }
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *t = T + (long)nu * n, *f = p0 + (long)nu * n;
+      for (long p = 0; p < n; p++) r[p] += f[p] * t[p];
+      if (cfg->deriv) {
+        const double *fx = px + (long)nu * n, *fy = py + (long)nu * n, *fz = pz + (long)nu * n;
+        for (long p = 0; p < n; p++) {
+          r[n + p] += 2.0 * fx[p] * t[p];
+          r[2 * n + p] += 2.0 * fy[p] * t[p];
+          r[3 * n + p] += 2.0 * fz[p] * t[p];
+        }
+      }
+    }
+  }
+  const double *ra = rho, *rb = rho + 4 * n;
+  double na = 0.0, nb = 0.0, rmax = -1e300;
+  for (long p = 0; p < n; p++) {
+    na += ra[p] * wts[p];
+    nb += rb[p] * wts[p];
+    if (ra[p] > rmax) rmax = ra[p];
+    if (rb[p] > rmax) rmax = rb[p];
+  }
+  *nel_a += na;
+  *nel_b += nb;
+  if (rmax < cfg->dthr) return;
+  for (long p = 0; p < n; p++) {
+    exc[p] = 0.0;
+    vr[2 * p] = vr[2 * p + 1] = 0.0;
+    vs[3 * p] = vs[3 * p + 1] = vs[3 * p + 2] = 0.0;
+    rr[2 * p] = ra[p];
+    rr[2 * p + 1] = rb[p];
+    ss[3 * p] = ra[n + p] * ra[n + p] + ra[2 * n + p] * ra[2 * n + p] + ra[3 * n + p] * ra[3 * n + p];
+    ss[3 * p + 1] = ra[n + p] * rb[n + p] + ra[2 * n + p] * rb[2 * n + p] + ra[3 * n + p] * rb[3 * n + p];
+    ss[3 * p + 2] = rb[n + p] * rb[n + p] + rb[2 * n + p] * rb[2 * n + p] + rb[3 * n + p] * rb[3 * n + p];
+  }
+  for (int f = 0; f < cfg->nfunc; f++) oracle_functional_eval_pol(cfg->ids[f], n, rr, ss, exc, vr, vs);
+  double e = 0.0;
+  for (long p = 0; p < n; p++) { /* Result::weight_by, then exc . (rho_a + rho_b) */
+    exc[p] *= wts[p];
+    for (int k = 0; k < 2; k++) vr[2 * p + k] *= wts[p];
+    for (int k = 0; k < 3; k++) vs[3 * p + k] *= wts[p];
+    e += exc[p] * ra[p] + exc[p] * rb[p];
+  }
+  *energy += e;
+  for (int sp = 0; sp < 2; sp++) {
+    const double *r_s = sp ? rb : ra, *r_o = sp ? ra : rb;
+    /* LDA part: V_s = phi^T (vrho_s phi) */
+    for (int nu = 0; nu < m; nu++) {
+      const double *fn = p0 + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double *fm = p0 + (long)mu * n;
+        double s = 0.0;
+        for (long p = 0; p < n; p++) s += fm[p] * vr[2 * p + sp] * fn[p];
+        Kblk[sp * m + mu + (long)nu * ld] += s;
+      }
+    }
+    if (!cfg->deriv) continue;
+    /* g_s = 2 vsigma_ss grad rho_s + vsigma_ab grad rho_other; gamma = grad phi . g_s;
+     * V_s += phi^T gamma + (phi^T gamma)^T */
+    for (int nu = 0; nu < m; nu++) {
+      const long o = (long)nu * n;
+      for (long p = 0; p < n; p++) {
+        const double vss = vs[3 * p + (sp ? 2 : 0)], vab = vs[3 * p + 1];
+        const double gx = 2.0 * vss * r_s[n + p] + vab * r_o[n + p];
+        const double gy = 2.0 * vss * r_s[2 * n + p] + vab * r_o[2 * n + p];
+        const double gz = 2.0 * vss * r_s[3 * n + p] + vab * r_o[3 * n + p];
+        G[o + p] = px[o + p] * gx + py[o + p] * gy + pz[o + p] * gz;
+      }
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *gn = G + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double *fm = p0 + (long)mu * n;
+        double s = 0.0;
+        for (long p = 0; p < n; p++) s += fm[p] * gn[p];
+        Kblk[sp * m + mu + (long)nu * ld] += s;
+        Kblk[sp * m + nu + (long)mu * ld] += s;
+      }
+    }
+  }
+}
+
+int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double *xyz,
+                                 const double *w, int nfunc, const int *func_ids,
+                                 const double *D, double density_threshold,
+                                 double screening_threshold, int screened, int nthreads,
+                                 double *K, double *exc_out, double *nelec_out) {
+  if (!b || npts < 0 || nfunc <= 0) return -1;
+  int deriv = 0;
+  for (int f = 0; f < nfunc; f++) {
+    int fam = oracle_functional_family(func_ids[f]);
+    if (fam < 0) return -1;
+    if (fam > deriv) deriv = fam;
+  }
+  const int nbf = oracle_nbf(b), nsh = b->nshell;
+  const long ldD = 2L * nbf;
+  int *first_bf = malloc(sizeof(int) * (nsh + 1));
+  first_bf[0] = 0;
+  for (int s = 0; s < nsh; s++) first_bf[s + 1] = first_bf[s] + oracle_shell_size(b->l[s], b->pure[s]);
+  memset(K, 0, sizeof(double) * ldD * nbf);
+  block_cfg cfg = {deriv, nfunc, func_ids, density_threshold};
+  const double *pxyz = xyz, *pw = w;
+  double *sxyz = 0, *sw = 0, *lcen = 0, *lrad = 0, *ext = 0;
+  long *perm = 0, *loff = 0, *lcnt = 0;
+  long nblk;
+  if (screened) {
+    sxyz = malloc(sizeof(double) * 3 * (npts + 1));
+    sw = malloc(sizeof(double) * (npts + 1));
+    perm = malloc(sizeof(long) * (npts + 1));
+    long cap = npts / 32 + 2;
+    loff = malloc(sizeof(long) * cap);
+    lcnt = malloc(sizeof(long) * cap);
+    lcen = malloc(sizeof(double) * 3 * cap);
+    lrad = malloc(sizeof(double) * cap);
+    nblk = oracle_build_hierarchy(npts, xyz, w, sxyz, sw, perm, loff, lcnt, lcen, lrad);
+    pxyz = sxyz;
+    pw = sw;
+    ext = malloc(sizeof(double) * nsh);
+    oracle_decay_cutoff(b, screening_threshold, ext);
+  } else {
+    nblk = (npts + 127) / 128;
+    loff = malloc(sizeof(long) * (nblk + 1));
+    lcnt = malloc(sizeof(long) * (nblk + 1));
+    for (long i = 0; i < nblk; i++) {
+      loff[i] = 128 * i;
+      lcnt[i] = (npts - 128 * i < 128) ? npts - 128 * i : 128;
+    }
+  }
+  double energy = 0.0, na = 0.0, nb = 0.0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel reduction(+ : energy, na, nb)
+  {
+    double *Kloc = calloc((size_t)ldD * nbf, sizeof(double));
+    int *sig = malloc(sizeof(int) * (nsh + 1));
+    int *gidx = malloc(sizeof(int) * (nbf + 1));
+    long maxn = 256;
+    double *phi = malloc(sizeof(double) * 4 * (size_t)nbf * maxn);
+    double *Dloc = malloc(sizeof(double) * 2 * (size_t)nbf * nbf);
+    double *Kblk = malloc(sizeof(double) * 2 * (size_t)nbf * nbf);
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 20 * maxn));
+#pragma omp for schedule(dynamic, 4)
+    for (long li = 0; li < nblk; li++) {
+      long n = lcnt[li];
+      if (n == 0) continue;
+      int nsig = 0;
+      if (screened) {
+        for (int s = 0; s < nsh; s++) {
+          const double *o = b->origin + 3 * s;
+          double dx = o[0] - lcen[3 * li], dy = o[1] - lcen[3 * li + 1], dz = o[2] - lcen[3 * li + 2];
+          if (sqrt(dx * dx + dy * dy + dz * dz) - ext[s] < lrad[li]) sig[nsig++] = s;
+        }
+        if (nsig == 0) continue;
+      } else {
+        for (int s = 0; s < nsh; s++) sig[nsig++] = s;
+      }
+      int m = 0;
+      for (int k = 0; k < nsig; k++)
+        for (int q = first_bf[sig[k]]; q < first_bf[sig[k] + 1]; q++) gidx[m++] = q;
+      /* dft.h:340-348: Dloc per spin block from D2 = 2 D */
+      for (int sp = 0; sp < 2; sp++)
+        for (int j = 0; j < m; j++)
+          for (int i = 0; i < m; i++)
+            Dloc[sp * m + i + (long)j * 2 * m] = 2.0 * D[sp * nbf + gidx[i] + (long)gidx[j] * ldD];
+      eval_shells(b, sig, nsig, n, pxyz + 3 * loff[li], deriv, phi, m);
+      memset(Kblk, 0, sizeof(double) * 2 * m * m);
+      process_block_u(&cfg, m, n, phi, Dloc, pw + loff[li], Kblk, &energy, &na, &nb, work);
+      for (int sp = 0; sp < 2; sp++)
+        for (int j = 0; j < m; j++)
+          for (int i = 0; i < m; i++)
+            Kloc[sp * nbf + gidx[i] + (long)gidx[j] * ldD] += Kblk[sp * m + i + (long)j * 2 * m];
+    }
+#pragma omp critical
+    {
+      for (long i = 0; i < ldD * nbf; i++) K[i] += Kloc[i];
+    }
+    free(Kloc);
+    free(sig);
+    free(gidx);
+    free(phi);
+    free(Dloc);
+    free(Kblk);
+    free(work);
+  }
+  *exc_out = energy;
+  nelec_out[0] = na;
+  nelec_out[1] = nb;
+  free(first_bf);
   free(sxyz);
   free(sw);
   free(perm);

@@ -94,6 +94,15 @@ int oracle_functional_eval(int id, long n, const double *rho,
 /* Energy density e = rho*eps at one point, exposed for tests. */
 double oracle_functional_energy_density(int id, double rho, double sigma);
 
+/* The same functionals for a spin-polarised density, libxc's layout:
+ * rho[2n] = (rho_a, rho_b) per point, sigma[3n] = (s_aa, s_ab, s_bb);
+ * ACCUMULATES exc[n], vrho[2n], vsigma[3n].  sigma/vsigma may be NULL for
+ * LDA.  Returns 0 or -1 for an unknown id. */
+int oracle_functional_eval_pol(int id, long n, const double *rho, const double *sigma,
+                               double *exc, double *vrho, double *vsigma);
+double oracle_functional_energy_density_pol(int id, double ra, double rb, double saa,
+                                            double sab, double sbb);
+
 /* DFT::compute_K_dft<d, Restricted> (include/occ/dft/dft.h:190-388) and
  * kernels::process_grid_block (src/dft/dft_kernels.cpp:8-70).
  *   D: nbf x nbf column-major restricted density, no factor 2 (orb.h:64-66);
@@ -120,6 +129,19 @@ int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
                        const int *func_ids, const double *D,
                        double density_threshold, double screening_threshold,
                        int screened, int nthreads, double *grad);
+
+/* DFT::compute_K_dft<d, Unrestricted> (include/occ/dft/dft.h:190-388) with
+ * process_grid_block<d, Unrestricted> (src/dft/dft_kernels.cpp:8-70):
+ *   D: 2nbf x nbf column-major, alpha block in rows 0..nbf-1 and beta block
+ *      in rows nbf..2nbf-1 (qm::block::a / b), no factor 2; each block is
+ *      read in full (density.cpp:18-30,45-70 use no selfadjointView).
+ *   K: 2nbf x nbf column-major output; nelec[2] = {N_alpha, N_beta}.
+ * Returns 0, or -1 on invalid input. */
+int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double *xyz,
+                                 const double *w, int nfunc, const int *func_ids,
+                                 const double *D, double density_threshold,
+                                 double screening_threshold, int screened, int nthreads,
+                                 double *K, double *exc, double *nelec);
 
 /* Algorithmic work of the screened build: per-leaf (n_b, m_b) with the
  * reference screening rule.  Writes up to maxleaves entries; returns the

@@ -59,6 +59,14 @@ def lib():
                                          ctypes.c_long, _dp, _dp, ctypes.c_int, _ip, _dp,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_int, _dp]
+        L.oracle_functional_eval_pol.argtypes = [ctypes.c_int, ctypes.c_long, _dp, _dp, _dp, _dp,
+                                                 _dp]
+        L.oracle_functional_energy_density_pol.restype = ctypes.c_double
+        L.oracle_functional_energy_density_pol.argtypes = [ctypes.c_int] + [ctypes.c_double] * 5
+        L.oracle_xc_build_unrestricted.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp,
+                                                   ctypes.c_int, _ip, _dp, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                   _dp, _dp, _dp]
         L.oracle_screened_work.restype = ctypes.c_long
         L.oracle_screened_work.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp,
                                            ctypes.c_double, ctypes.c_long, _lp, _lp]
@@ -214,6 +222,47 @@ def xc_build(basis, xyz, w, func_ids, D, density_threshold=1e-10,
     if rc != 0:
         raise ValueError("oracle_xc_build rejected its input")
     return K, exc.value, nel.value
+
+
+def functional_eval_pol(fid, rho, sigma=None):
+    """Polarised evaluation: rho (n, 2), sigma (n, 3) -> exc (n), vrho (n, 2),
+    vsigma (n, 3)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+    n = rho.shape[0]
+    if lib().oracle_functional_family(fid) < 0:
+        raise ValueError(f"unknown functional id {fid}")
+    sig = np.ascontiguousarray(sigma if sigma is not None else np.zeros((n, 3)),
+                               dtype=np.float64).reshape(-1, 3)
+    exc, vrho, vsig = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3))
+    lib().oracle_functional_eval_pol(fid, n, _d(rho), _d(sig), _d(exc), _d(vrho), _d(vsig))
+    return exc, vrho, vsig
+
+
+def energy_density_pol(fid, ra, rb, saa=0.0, sab=0.0, sbb=0.0):
+    return lib().oracle_functional_energy_density_pol(fid, float(ra), float(rb), float(saa),
+                                                      float(sab), float(sbb))
+
+
+def xc_build_unrestricted(basis, xyz, w, func_ids, D, density_threshold=1e-10,
+                          screening_threshold=1e-10, screened=True, nthreads=0):
+    """compute_K_dft<d, Unrestricted> on the CPU: D is (2 nbf, nbf) (alpha
+    block over beta block); returns (K (2 nbf, nbf), Exc, (N_alpha, N_beta))."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    ids = np.ascontiguousarray(func_ids, dtype=np.int32)
+    D = np.asfortranarray(D, dtype=np.float64)
+    nbf = basis.nbf
+    assert D.shape == (2 * nbf, nbf)
+    K = np.zeros((2 * nbf, nbf), order="F")
+    exc = ctypes.c_double()
+    nel = np.zeros(2)
+    rc = lib().oracle_xc_build_unrestricted(basis.ptr, len(w), _d(xyz), _d(w), len(ids), _i(ids),
+                                            D.ctypes.data_as(_dp), density_threshold,
+                                            screening_threshold, int(screened), int(nthreads),
+                                            K.ctypes.data_as(_dp), ctypes.byref(exc), _d(nel))
+    if rc != 0:
+        raise ValueError("oracle_xc_build_unrestricted rejected its input")
+    return K, exc.value, (float(nel[0]), float(nel[1]))
 
 
 def xc_gradient(basis, shell_atom, natoms, xyz, w, func_ids, D, density_threshold=1e-10,
