@@ -129,6 +129,20 @@ occxc_status occxc_context_get_info(const occxc_context *ctx,
 occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc,
                          double *exc, double *nelec);
 
+/* compute_vxc for an unrestricted density: compute_K_dft<d, Unrestricted>
+ * (include/occ/dft/dft.h:190-388, dispatched at dft.h:398-410) with
+ * process_grid_block<d, Unrestricted> (src/dft/dft_kernels.cpp:8-70),
+ * evaluate_density<d, U> (src/gto/density.cpp:18-30,45-70) and
+ * xc_potential_matrix<Unrestricted, d> (src/dft/xc_potential_matrix.cpp:
+ * 61-109), with the polarised functionals.  D is 2nbf x nbf column-major:
+ * the alpha block in rows 0..nbf-1, the beta block in rows nbf..2nbf-1
+ * (qm::block::a / b; OCC's density_matrix_unrestricted, orb.h:68-82, holds
+ * C_s C_s^T / 2 per spin), each block read in full; Vxc is returned in the
+ * same 2nbf x nbf layout; nelec[2] = {N_alpha, N_beta}, which OCC logs.
+ * FP64 and FP32-cache contexts of one molecule. */
+occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, double *Vxc,
+                                      double *exc, double *nelec);
+
 /* Same build on device-resident buffers, enqueued on `stream` (a hipStream_t,
  * NULL for the context's own stream) without host synchronisation.
  * d_result receives {Exc, N_elec} (per molecule for a batch context, whose
@@ -293,6 +307,14 @@ occxc_status occxc_functional_evaluate(int id, long long n, const double *rho,
                                        const double *sigma, double *exc,
                                        double *vrho, double *vsigma,
                                        int device);
+
+/* The same for a spin-polarised density (DensityFunctional with
+ * polarized = true, functional.cpp:18-65), libxc's layout: rho[2n] =
+ * (rho_a, rho_b) per point, sigma[3n] = (s_aa, s_ab, s_bb); ACCUMULATES
+ * exc[n], vrho[2n], vsigma[3n]. */
+occxc_status occxc_functional_evaluate_polarized(int id, long long n, const double *rho,
+                                                 const double *sigma, double *exc,
+                                                 double *vrho, double *vsigma, int device);
 
 /* SpatialGridHierarchy (src/numint/spatial_grid_hierarchy.cpp:28-147):
  * Morton sort + 128-point leaves.  Returns the number of leaves in *nleaves;

@@ -851,6 +851,64 @@ void run_gradient(occxc_context *c, const double *d_D, double *d_grad, hipStream
   });
 }
 
+// Unrestricted build on stream s from the raw density in d_Du:
+// compute_K_dft<d, Unrestricted> (dft.h:190-388).  Each spin density is
+// contracted by the density kernel in store mode, the polarised functional
+// stage forms both spins' coefficients, and the Vxc kernels run once per spin.
+void run_build_u(occxc_context *c, hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const int prec = c->opt.precision;
+  const bool two_pass = c->plan.two_pass;
+  StageTimer t{c, s, {}};
+  t.run(OCCXC_STAGE_PREPARE, [&] {
+    launch_prepare_density_u(c->d_Du.get(), c->d_D2a.get(), c->d_D2b.get(), c->nbf, s);
+    if (!two_pass) {
+      launch_zero(c->d_K.get(), c->mtot, s);
+      launch_zero(c->d_K_b.get(), c->mtot, s);
+    }
+  });
+  t.run(OCCXC_STAGE_DENSITY, [&] {
+    BuildArgs x = a;
+    x.rho_out = c->d_rho_a.get();
+    run_density_store(x, c->d_ao.get(), prec, c->d_D2a.get(), s);
+    x.rho_out = c->d_rho_b.get();
+    run_density_store(x, c->d_ao.get(), prec, c->d_D2b.get(), s);
+    launch_functional_pol_leaves(a, c->d_rho_a.get(), c->d_rho_b.get(), c->d_coef_b.get(),
+                                 c->d_part_u.get(), s);
+  });
+  t.run(OCCXC_STAGE_VXC, [&] {
+    run_vxc(c->plan, a, c->d_ao.get(), prec, s);
+    if (two_pass) run_vxc_reduce(a, c->d_V.get(), s);
+    BuildArgs b = a;
+    b.coef = c->d_coef_b.get();
+    b.K = c->d_K_b.get();
+    run_vxc(c->plan, b, c->d_ao.get(), prec, s);
+    if (two_pass) run_vxc_reduce(b, c->d_V_b.get(), s);
+  });
+  t.run(OCCXC_STAGE_FINALIZE, [&] {
+    launch_finalize_u(two_pass ? nullptr : c->d_K.get(), c->d_K_b.get(), c->d_V.get(),
+                      c->d_V_b.get(), c->d_mols.get(), c->maxnbf, c->d_part_u.get(),
+                      c->nleaves, c->d_result_u.get(), s);
+  });
+}
+
+void uks_setup(occxc_context *c) {
+  if (c->uks_ready) return;
+  const size_t nn = (size_t)c->nbf * c->nbf;
+  c->d_Du.alloc(2 * nn);
+  c->d_D2a.alloc(nn);
+  c->d_D2b.alloc(nn);
+  c->d_rho_a.alloc(4 * (size_t)c->ptot);
+  c->d_rho_b.alloc(4 * (size_t)c->ptot);
+  c->d_coef_b.alloc(4 * (size_t)c->ptot);
+  c->d_K_b.alloc(nn);
+  c->d_V_b.alloc(nn);
+  c->d_part_u.alloc(3 * (size_t)std::max(c->nleaves, 1));
+  c->d_result_u.alloc(3);
+  c->device_bytes += 8.0 * (7 * nn + 12 * (size_t)c->ptot + 3 * (size_t)c->nleaves + 3);
+  c->uks_ready = true;
+}
+
 void drop_graph(occxc_context *c) {
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   c->graph_exec = nullptr;
@@ -1091,6 +1149,40 @@ occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int
   });
 }
 
+occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, double *Vxc,
+                                      double *exc, double *nelec) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && D && Vxc && exc && nelec, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "occxc_build_unrestricted needs a single-molecule context");
+    OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
+                      ctx->opt.precision == OCCXC_PREC_FP32_CACHE,
+                  OCCXC_ERR_UNSUPPORTED,
+                  "unrestricted builds need an FP64 or FP32-cache context");
+    OCCXC_REQUIRE(ctx->plan.use_mfma && ctx->plan.max_mp <= kStoreMaxMp, OCCXC_ERR_UNSUPPORTED,
+                  "unrestricted builds need the matrix-core kernels and leaves of at most " +
+                      std::to_string(kStoreMaxMp) + " AOs");
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    for (size_t i = 0; i < 2 * nn; ++i)
+      OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
+    DeviceGuard g(ctx->device);
+    uks_setup(ctx);
+    ctx->d_Du.upload(D, 2 * nn, ctx->stream);
+    run_build_u(ctx, ctx->stream);
+    const size_t row = (size_t)ctx->nbf * sizeof(double);
+    OCCXC_HIP(hipMemcpy2DAsync(Vxc, 2 * row, ctx->d_V.get(), row, row, ctx->nbf,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    OCCXC_HIP(hipMemcpy2DAsync(Vxc + ctx->nbf, 2 * row, ctx->d_V_b.get(), row, row, ctx->nbf,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    double r[3];
+    ctx->d_result_u.download(r, 3, ctx->stream);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+    *exc = r[0];
+    nelec[0] = r[1];
+    nelec[1] = r[2];
+  });
+}
+
 occxc_status occxc_batch_layout(const occxc_context *ctx, int *nmol, int *nbf,
                                 long long *offsets) {
   return guarded([&] {
@@ -1224,6 +1316,41 @@ occxc_status occxc_functional_evaluate(int id, long long n, const double *rho,
       exc[i] += e[i];
       vrho[i] += v[i];
       if (fam) vsigma[i] += w[i];
+    }
+  });
+}
+
+occxc_status occxc_functional_evaluate_polarized(int id, long long n, const double *rho,
+                                                 const double *sigma, double *exc, double *vrho,
+                                                 double *vsigma, int device) {
+  return guarded([&] {
+    const int fam = xc_family(id);
+    OCCXC_REQUIRE(fam >= 0, OCCXC_ERR_UNSUPPORTED, "functional id " + std::to_string(id) + " not found");
+    OCCXC_REQUIRE(n >= 0 && (n == 0 || (rho && exc && vrho)), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_polarized: null pointer");
+    OCCXC_REQUIRE(fam == 0 || n == 0 || (sigma && vsigma), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_polarized: GGA needs sigma and vsigma");
+    if (n == 0) return;
+    DeviceGuard g(device);
+    DevBuf<double> dr(2 * n), ds(fam ? 3 * n : 1), de(n), dv(2 * n), dw(fam ? 3 * n : 1);
+    dr.upload(rho, 2 * n);
+    if (fam) ds.upload(sigma, 3 * n);
+    launch_zero(de.get(), n, nullptr);
+    launch_zero(dv.get(), 2 * n, nullptr);
+    if (fam) launch_zero(dw.get(), 3 * n, nullptr);
+    launch_functional_pol(id, n, dr.get(), fam ? ds.get() : nullptr, de.get(), dv.get(),
+                          fam ? dw.get() : nullptr, nullptr);
+    std::vector<double> e(n), v(2 * n), w(fam ? 3 * n : 0);
+    de.download(e.data(), n);
+    dv.download(v.data(), 2 * n);
+    if (fam) dw.download(w.data(), 3 * n);
+    OCCXC_HIP(hipDeviceSynchronize());
+    for (long long i = 0; i < n; ++i) {
+      exc[i] += e[i];
+      vrho[2 * i] += v[2 * i];
+      vrho[2 * i + 1] += v[2 * i + 1];
+      if (fam)
+        for (int k = 0; k < 3; ++k) vsigma[3 * i + k] += w[3 * i + k];
     }
   });
 }

@@ -68,6 +68,13 @@ struct occxc_context {
   bool ao2_ready = false;
   std::vector<int> grad_shell_atom;
   int grad_natoms = 0;
+  // unrestricted builds (allocated on the first one): the raw 2 nbf x nbf
+  // density, 2 D_s^T per spin, rho and grad rho per spin, the beta-spin
+  // coefficients and K / Vxc (alpha uses the restricted buffers), per-leaf
+  // {Exc, N_alpha, N_beta} and the result {Exc, N_alpha, N_beta}
+  occxc::DevBuf<double> d_Du, d_D2a, d_D2b, d_rho_a, d_rho_b, d_coef_b, d_K_b, d_V_b, d_part_u,
+      d_result_u;
+  bool uks_ready = false;
 };
 
 namespace occxc {

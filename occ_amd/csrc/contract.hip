@@ -289,7 +289,10 @@ __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__re
 constexpr int kMapCap = 2048;
 constexpr int kDefaultDensityVariant = 2;
 
-template <typename S, typename T>
+// STORE: write rho and grad rho of every (padded) point to a.rho_out
+// instead of running the functional epilogue (the unrestricted build
+// evaluates each spin density this way).
+template <typename S, typename T, bool STORE = false>
 __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *__restrict__ ao,
                                                       const T *__restrict__ D2t, int min_mp) {
   using M = Mfma<T>;
@@ -400,7 +403,13 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
         }
   }
   __syncthreads();
-  density_epilogue(a, L, leaf, s_q, s_red);
+  if constexpr (STORE) {
+    for (int p = threadIdx.x; p < L.np; p += blockDim.x)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a.rho_out[q * a.ptot + L.pt_off + p] = s_q[q][p];
+  } else {
+    density_epilogue(a, L, leaf, s_q, s_red);
+  }
 }
 
 // Register-lean per-wavefront variant: as k_density_wave, but each 32-AO
@@ -1397,6 +1406,18 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_bf16),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)vxc_bf16_smem()));
   return p;
+}
+
+void run_density_store(const BuildArgs &a, const void *ao, int precision, const double *D2t,
+                       hipStream_t s) {
+  if (a.nleaves == 0) return;
+  if (precision == 0)
+    k_density_wave<double, double, true>
+        <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), D2t, 0);
+  else
+    k_density_wave<float, double, true>
+        <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), D2t, 0);
+  check_launch("k_density_wave (store)");
 }
 
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,

@@ -34,6 +34,11 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
                                 int precision, hipStream_t s);
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s);
+// rho and grad rho of every leaf point for the density D2t into a.rho_out
+// (FP64 or FP32-cache contexts, leaves of at most 2048 AOs)
+void run_density_store(const BuildArgs &a, const void *ao, int precision, const double *D2t,
+                       hipStream_t s);
+constexpr int kStoreMaxMp = 2048;
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s);
 // two-pass Vxc: V = sum over leaves of the stored K_loc blocks (+ transposes),

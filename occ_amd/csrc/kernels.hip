@@ -26,6 +26,7 @@
 
 #include "kernels.hpp"
 #include "xc_functionals.hpp"
+#include "xc_functionals_pol.hpp"
 
 namespace occxc {
 
@@ -475,6 +476,120 @@ __global__ void k_prepare_density(const double *__restrict__ D, double *__restri
 
 // blockIdx.y = molecule; Vxc = K + K^T
 // V = K + K^T
+// Unrestricted D (2 nbf x nbf, alpha rows over beta rows, one molecule):
+// D2t_s = 2 D_s^T for each spin, the full block (evaluate_density<d, U> uses
+// no selfadjointView, density.cpp:18-30,45-70); the contraction kernels read
+// D2t(nu, mu) as D2(mu, nu).
+__global__ void k_prepare_density_u(const double *__restrict__ D, double *__restrict__ D2a,
+                                    double *__restrict__ D2b, int nbf) {
+  const long long n = (long long)nbf * nbf;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int nu = (int)(k % nbf), mu = (int)(k / nbf);
+    D2a[k] = 2.0 * D[mu + (long long)nu * 2 * nbf];
+    D2b[k] = 2.0 * D[nbf + mu + (long long)nu * 2 * nbf];
+  }
+}
+
+// Unrestricted per-leaf functional stage (one thread per point, blockDim =
+// 256): N_alpha, N_beta before the skip, the skip on the largest alpha or
+// beta density of the leaf (dft_kernels.cpp:27-40), set_params<U>
+// (dft_kernels.h:16-46), the polarised functionals, weight_by, the energy
+// sum w exc (rho_a + rho_b), and per spin the Vxc coefficients of
+// xc_potential_matrix<Unrestricted, d> (xc_potential_matrix.cpp:61-109):
+//   c0 = w vrho_s / 2,  c_k = w (2 vsigma_ss d_k rho_s + vsigma_ab d_k rho_s')
+// so that V_s = K_s + K_s^T with K_s = phi^T (c0 phi + c . grad phi).
+__global__ __launch_bounds__(256) void k_functional_pol(BuildArgs a,
+                                                        const double *__restrict__ rho_a,
+                                                        const double *__restrict__ rho_b,
+                                                        double *__restrict__ coef_b,
+                                                        double *__restrict__ part) {
+  __shared__ double s_red[4];
+  const int leaf = blockIdx.x;
+  const LeafDev L = a.leaves[leaf];
+  const int p = threadIdx.x;
+  const bool live = p < L.np, real = p < L.n;
+  const long long q = L.pt_off + p, P = a.ptot;
+  double ra = 0, rb = 0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0}, w = 0;
+  if (live) {
+    ra = rho_a[q];
+    rb = rho_b[q];
+    for (int k = 0; k < 3; ++k) {
+      ga[k] = rho_a[(k + 1) * P + q];
+      gb[k] = rho_b[(k + 1) * P + q];
+    }
+    w = a.pw[q];
+  }
+  const double na = block_sum_256(real ? ra * w : 0.0, s_red);
+  const double nb = block_sum_256(real ? rb * w : 0.0, s_red);
+  const double rmax = block_max_256(real ? fmax(ra, rb) : -1e300, s_red);
+  const bool active = !(rmax < a.dthr);
+  double e = 0.0, vr[2] = {0, 0}, vs[3] = {0, 0, 0};
+  if (active && live && ra + rb >= kDensThreshold) {
+    const double saa = fmax(ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2], kSigmaMin);
+    const double sab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    const double sbb = fmax(gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2], kSigmaMin);
+    for (int f = 0; f < a.nfunc; ++f)
+      xc_point_pol(a.func_ids[f], ra, rb, a.deriv ? saa : 0.0, a.deriv ? sab : 0.0,
+                   a.deriv ? sbb : 0.0, e, vr, vs);
+  }
+  const double exc = block_sum_256(active && real ? w * e : 0.0, s_red);
+  if (live) {
+    double *ca = a.coef;
+    ca[q] = active ? 0.5 * w * vr[0] : 0.0;
+    coef_b[q] = active ? 0.5 * w * vr[1] : 0.0;
+    for (int k = 0; k < 3; ++k) {
+      ca[(k + 1) * P + q] = active ? w * (2.0 * vs[0] * ga[k] + vs[1] * gb[k]) : 0.0;
+      coef_b[(k + 1) * P + q] = active ? w * (2.0 * vs[2] * gb[k] + vs[1] * ga[k]) : 0.0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    part[3 * leaf] = active ? exc : 0.0;
+    part[3 * leaf + 1] = na;
+    part[3 * leaf + 2] = nb;
+    a.leaf_active[leaf] = active ? 1 : 0;
+  }
+}
+
+// {Exc, N_alpha, N_beta}: fixed-order sums of the per-leaf partials
+__global__ __launch_bounds__(256) void k_finalize_scalars_u(const double *__restrict__ part,
+                                                            int nleaves,
+                                                            double *__restrict__ result) {
+  __shared__ double s_red[4];
+  double v[3] = {0, 0, 0};
+  for (int i = threadIdx.x; i < nleaves; i += 256)
+    for (int k = 0; k < 3; ++k) v[k] += part[3 * i + k];
+  for (int k = 0; k < 3; ++k) {
+    const double t = block_sum_256(v[k], s_red);
+    if (threadIdx.x == 0) result[k] = t;
+  }
+}
+
+// Polarised functionals over points (libxc layout), ADDING to the outputs.
+__global__ void k_functional_pol_points(int id, long long n, const double *__restrict__ rho,
+                                        const double *__restrict__ sigma,
+                                        double *__restrict__ exc, double *__restrict__ vrho,
+                                        double *__restrict__ vsigma) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool gga = xc_family(id) == 1;
+  const double ra = rho[2 * i], rb = rho[2 * i + 1];
+  if (!(ra + rb >= kDensThreshold)) return;
+  double saa = 0, sab = 0, sbb = 0;
+  if (gga) {
+    saa = fmax(sigma[3 * i], kSigmaMin);
+    sab = sigma[3 * i + 1];
+    sbb = fmax(sigma[3 * i + 2], kSigmaMin);
+  }
+  double e = 0.0, vr[2] = {0, 0}, vs[3] = {0, 0, 0};
+  xc_point_pol(id, ra, rb, saa, sab, sbb, e, vr, vs);
+  exc[i] += e / (ra + rb);
+  vrho[2 * i] += vr[0];
+  vrho[2 * i + 1] += vr[1];
+  if (gga)
+    for (int k = 0; k < 3; ++k) vsigma[3 * i + k] += vs[k];
+}
+
 __global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V,
                                   const MolDev *__restrict__ mols) {
   const MolDev M = mols[blockIdx.y];
@@ -748,6 +863,41 @@ void launch_finalize_scalars(const MolDev *mols, int nmol, const double *leaf_pa
                              double *result, hipStream_t s) {
   if (nmol == 0) return;
   k_finalize_scalars<<<nmol, 256, 0, s>>>(leaf_part, mols, result);
+  check_launch();
+}
+
+void launch_prepare_density_u(const double *D, double *D2a, double *D2b, int nbf,
+                              hipStream_t s) {
+  const long long n = (long long)nbf * nbf;
+  if (n == 0) return;
+  k_prepare_density_u<<<grid_for(n, 256), 256, 0, s>>>(D, D2a, D2b, nbf);
+  check_launch();
+}
+
+void launch_functional_pol_leaves(const BuildArgs &a, const double *rho_a, const double *rho_b,
+                                  double *coef_b, double *part, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  k_functional_pol<<<a.nleaves, 256, 0, s>>>(a, rho_a, rho_b, coef_b, part);
+  check_launch();
+}
+
+void launch_finalize_u(const double *Ka, const double *Kb, double *Va, double *Vb,
+                       const MolDev *mols, int maxnbf, const double *part, int nleaves,
+                       double *result, hipStream_t s) {
+  if (Ka) {
+    k_finalize_matrix<<<mol_grid(maxnbf, 1), 256, 0, s>>>(Ka, Va, mols);
+    check_launch();
+    k_finalize_matrix<<<mol_grid(maxnbf, 1), 256, 0, s>>>(Kb, Vb, mols);
+    check_launch();
+  }
+  k_finalize_scalars_u<<<1, 256, 0, s>>>(part, nleaves, result);
+  check_launch();
+}
+
+void launch_functional_pol(int id, long long n, const double *rho, const double *sigma,
+                           double *exc, double *vrho, double *vsigma, hipStream_t s) {
+  if (n == 0) return;
+  k_functional_pol_points<<<grid_for(n, 256), 256, 0, s>>>(id, n, rho, sigma, exc, vrho, vsigma);
   check_launch();
 }
 

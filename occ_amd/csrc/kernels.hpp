@@ -124,6 +124,7 @@ struct BuildArgs {
   const TileChunkDev *tchunks;
   int ntchunks;
   double *tpartial;     // [ntchunks][32 * 32] partial blocks
+  double *rho_out;      // unrestricted: [4][ptot] rho, grad rho of one spin (density store)
 };
 
 // device set-up of the screened layout (setup.hip)
@@ -161,6 +162,19 @@ void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const 
 void launch_density_functional(const BuildArgs &a, const void *ao, bool fp32,
                                hipStream_t s);
 void launch_vxc(const BuildArgs &a, const void *ao, bool fp32, hipStream_t s);
+// unrestricted build (one molecule): D2t_s = 2 D_s^T; the polarised
+// functional stage per leaf (coef_a in a.coef, {Exc, N_a, N_b} per leaf in
+// part); V_s = K_s + K_s^T (skipped when Ka is null) and the fixed-order
+// scalar sums into result[3]
+void launch_prepare_density_u(const double *D, double *D2a, double *D2b, int nbf,
+                              hipStream_t s);
+void launch_functional_pol_leaves(const BuildArgs &a, const double *rho_a, const double *rho_b,
+                                  double *coef_b, double *part, hipStream_t s);
+void launch_finalize_u(const double *Ka, const double *Kb, double *Va, double *Vb,
+                       const MolDev *mols, int maxnbf, const double *part, int nleaves,
+                       double *result, hipStream_t s);
+void launch_functional_pol(int id, long long n, const double *rho, const double *sigma,
+                           double *exc, double *vrho, double *vsigma, hipStream_t s);
 void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol,
                      int maxnbf, const double *leaf_part, double *result, hipStream_t s);
 // {Exc, N_elec} per molecule only (the two-pass Vxc writes Vxc itself)

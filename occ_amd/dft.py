@@ -105,6 +105,21 @@ class XCContext:
                                                  ctypes.c_void_p(d_result),
                                                  ctypes.c_void_p(stream or 0)))
 
+    def build_unrestricted(self, D):
+        """Unrestricted build (occxc_build_unrestricted): D is (2 nbf, nbf),
+        alpha block over beta block; returns (Vxc (2 nbf, nbf), Exc,
+        (N_alpha, N_beta))."""
+        D = np.asfortranarray(D, dtype=np.float64)
+        if D.shape != (2 * self.nbf, self.nbf):
+            raise ValueError(f"D must be {2 * self.nbf}x{self.nbf}")
+        V = np.zeros((2 * self.nbf, self.nbf), order="F")
+        exc = ctypes.c_double()
+        nel = np.zeros(2)
+        _lib.check(_lib.lib().occxc_build_unrestricted(self._h, D.ctypes.data_as(_lib._dp),
+                                                       V.ctypes.data_as(_lib._dp),
+                                                       ctypes.byref(exc), _lib.dptr(nel)))
+        return V, exc.value, (float(nel[0]), float(nel[1]))
+
     def xc_gradient(self, D, shell_atom, natoms=None):
         """XC contribution to the nuclear gradient (occxc_xc_gradient):
         3 x natoms, column per atom."""
@@ -274,7 +289,14 @@ class DFT:
         self._nel = 0.0
 
     def compute_vxc(self, D):
-        """Returns the matrix OCC's compute_vxc returns (Fock adds it, dft.h:435)."""
+        """Returns the matrix OCC's compute_vxc returns (Fock adds it, dft.h:435).
+        A (2 nbf, nbf) D is unrestricted (alpha block over beta block, as
+        compute_vxc dispatches on mo.kind, dft.h:398-424); total_density() is
+        then (N_alpha, N_beta)."""
+        D = np.asarray(D)
+        if D.shape == (2 * self.basis.nbf, self.basis.nbf):
+            V, self._exc, self._nel = self.ctx.build_unrestricted(D)
+            return V
         V, self._exc, self._nel = self.ctx.build(D)
         return V
 
@@ -326,6 +348,20 @@ class DensityFunctional:
         sig = _lib.f64(sigma) if gga else None
         vsig = np.zeros(n) if gga else None
         _lib.check(_lib.lib().occxc_functional_evaluate(
+            self.id, n, _lib.dptr(rho), _lib.dptr(sig) if gga else None, _lib.dptr(exc),
+            _lib.dptr(vrho), _lib.dptr(vsig) if gga else None, device))
+        return exc, vrho, vsig
+
+    def evaluate_polarized(self, rho, sigma=None, device=0):
+        """Spin-polarised: rho (n, 2), sigma (n, 3) -> exc (n), vrho (n, 2),
+        vsigma (n, 3) (None for LDA)."""
+        rho = _lib.f64(rho).reshape(-1, 2)
+        n = rho.shape[0]
+        exc, vrho = np.zeros(n), np.zeros((n, 2))
+        gga = self.id > 100
+        sig = _lib.f64(sigma).reshape(-1, 3) if gga else None
+        vsig = np.zeros((n, 3)) if gga else None
+        _lib.check(_lib.lib().occxc_functional_evaluate_polarized(
             self.id, n, _lib.dptr(rho), _lib.dptr(sig) if gga else None, _lib.dptr(exc),
             _lib.dptr(vrho), _lib.dptr(vsig) if gga else None, device))
         return exc, vrho, vsig

@@ -112,3 +112,13 @@ def test_xc_gradient_rejects_null_context_without_gpu():
     st = L.occxc_xc_gradient(None, None, 1, _lib.iptr(sa), G.ctypes.data_as(_lib._dp))
     assert st == 1
     assert b"null" in L.occxc_last_error_message()
+
+
+def test_unrestricted_and_polarised_reject_without_gpu():
+    L = _lib.lib()
+    st = L.occxc_build_unrestricted(None, None, None, None, None)
+    assert st == 1
+    x = np.zeros(6)
+    st = L.occxc_functional_evaluate_polarized(9999, 1, _lib.dptr(x), _lib.dptr(x), _lib.dptr(x),
+                                               _lib.dptr(x), _lib.dptr(x), 0)
+    assert st == 2

@@ -136,3 +136,52 @@ def test_unrestricted_potential_is_the_derivative(dimer, method):
               oracle.xc_build_unrestricted(ob, pts, w, ids, Dm)[1]) / (2 * eps)
         assert fd == pytest.approx(2 * np.sum(Ku[sp * n:(sp + 1) * n] * Delta), rel=1e-7,
                                    abs=1e-9)
+
+
+@pytest.fixture(scope="module")
+def pol_shim(tmp_path_factory):
+    """The polarised functional header the GPU kernels use, compiled for the
+    host."""
+    import ctypes
+    import pathlib
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    d = tmp_path_factory.mktemp("polshim")
+    src = d / "shim.cpp"
+    src.write_text('#include "xc_functionals_pol.hpp"\n'
+                   'extern "C" int shim(int id, const double *x, double *out) {\n'
+                   '  out[0] = 0; for (int k = 1; k < 6; ++k) out[k] = 0;\n'
+                   '  return occxc::xc_point_pol(id, x[0], x[1], x[2], x[3], x[4], out[0],\n'
+                   '                             out + 1, out + 3) ? 0 : -1; }\n')
+    so = d / "shim.so"
+    subprocess.run(["g++", "-O2", "-fPIC", "-shared", f"-I{root / 'occ_amd' / 'csrc'}",
+                    str(src), "-o", str(so)], check=True)
+    L = ctypes.CDLL(str(so))
+    L.shim.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_double)]
+    return L
+
+
+@pytest.mark.parametrize("fid", IDS)
+def test_gpu_polarised_header_matches_oracle(pol_shim, fid, rng):
+    import ctypes
+    out = (ctypes.c_double * 6)()
+    for _ in range(300):
+        # spin polarisation up to rho_a / rho_b = 1e4 either way; beyond that
+        # 1 - zeta is formed by cancellation and the two implementations
+        # round differently (both are exact forms of the same expression)
+        ra = 10 ** rng.uniform(-8, 2)
+        rb = ra * 10 ** rng.uniform(-4, 4)
+        ga, gb = rng.normal(size=3) * ra ** (4 / 3), rng.normal(size=3) * rb ** (4 / 3)
+        x = np.array([ra, rb, ga @ ga, ga @ gb, gb @ gb]) if fid > 100 else \
+            np.array([ra, rb, 0.0, 0.0, 0.0])
+        assert pol_shim.shim(fid, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out) == 0
+        oe, ov, os_ = oracle.functional_eval_pol(fid, x[None, :2], x[None, 2:])
+        scale = abs(oe[0] * (ra + rb))
+        assert out[0] == pytest.approx(oe[0] * (ra + rb), rel=1e-11)
+        for k in range(2):
+            assert out[1 + k] == pytest.approx(ov[0, k], rel=1e-9, abs=1e-14 * scale / (ra + rb))
+        if fid > 100:
+            for k in range(3):
+                assert out[3 + k] == pytest.approx(os_[0, k], rel=1e-9,
+                                                   abs=1e-12 * abs(os_[0]).max())
