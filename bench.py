@@ -115,6 +115,8 @@ def main():
         return run_batch(args)
     if args.workload == "grad32":
         return run_gradient(args)
+    if args.workload == "uks32":
+        return run_unrestricted(args)
     if args.workload == "serve":
         return run_serving(args)
     atoms, bname, pure, method, nocc, desc = workload(args.workload)
@@ -338,6 +340,100 @@ def run_gradient(args):
                    "natoms": nat, "leaves": R["info"]["nleaves"], "precision": head},
         "roofline": roof,
         "accuracy": {m: results[m].get("dgrad") for m in results},
+        "modes_ms": {m: results[m]["ms_per_step"] for m in results},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_unrestricted(args):
+    """Unrestricted XC build of the (H2O)32 / def2-SVP / PBE workload for a
+    doublet cation: alpha C (768 x 160) and beta C (768 x 159) seeded Gaussian,
+    orthonormalised, D_s = C_s C_s^T / 2 (OCC's density_matrix_unrestricted).
+    A step is one build from D (2 nbf x nbf) in HBM to both spins' Vxc in HBM,
+    Exc, N_alpha, N_beta."""
+    from occ_amd import AOBasis
+    from occ_amd.dft import XCContext, functional_ids
+    from occ_amd.device import DeviceArray, synchronize
+    from occ_amd.grid import GridSettings, MolecularGrid
+    atoms, bname, pure, method, nocc, desc = workload("h2o32")
+    basis = AOBasis.load(atoms, bname, pure=pure)
+    nbf = basis.nbf
+    pts, wts = MolecularGrid(atoms, GridSettings(radial_points=76, angular_points=302)).build(
+        device=0)
+    Du = np.vstack([seeded_density(nbf, nocc, 0) / 2, seeded_density(nbf, nocc - 1, 1) / 2])
+    ids = functional_ids(method)
+    ref, cpu = None, None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        ob = oracle.OracleBasis.from_flat(basis.flat())
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        times = []
+        for _ in range(max(1, min(args.cpu_repeats, 3))):
+            t0 = time.time()
+            ref = oracle.xc_build_unrestricted(ob, pts, wts, ids, Du, nthreads=threads)
+            times.append(time.time() - t0)
+        dt = float(np.median(times))
+        cpu = {"value": 1.0 / dt, "unit": "UKS XC builds/s", "cores": threads, "kind": "port",
+               "sample": f"full FP64 screened unrestricted builds of the same workload on the "
+                         f"CPU oracle (oracle/occxc_oracle.c, OpenMP over leaves); median of "
+                         f"{len(times)} on {threads} threads = {dt:.2f} s",
+               "cpu_model": cpu_model()}
+        log(f"cpu oracle unrestricted build: {dt:.2f}s on {threads} threads")
+    d_D, d_V, d_r = DeviceArray(2 * nbf * nbf), DeviceArray(2 * nbf * nbf), DeviceArray(3)
+    d_D.upload(np.asfortranarray(Du))
+    results = {}
+    for mode in [m for m in args.modes.split(",") if m in ("fp64", "fp32cache")]:
+        ctx = XCContext(basis, pts, wts, ids, precision=mode)
+        info = ctx.info()
+        for _ in range(max(args.warmup, 1)):
+            ctx.build_unrestricted_device(d_D.ptr, d_V.ptr, d_r.ptr)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.build_unrestricted_device(d_D.ptr, d_V.ptr, d_r.ptr)
+        ctx.synchronize()
+        synchronize()
+        wall = time.perf_counter() - t0
+        tim = ctx.timing()
+        V = d_V.download((2 * nbf, nbf))
+        exc, na, nb = d_r.download()
+        ms = wall / args.steps * 1e3
+        r = {"ms_per_step": ms, "per_s": 1e3 / ms, "info": info, "timing": tim, "exc": exc}
+        if ref is not None:
+            r["dexc"] = abs(exc - ref[1])
+            r["dvxc"] = float(np.abs(V - ref[0]).max())
+            r["dnelec"] = max(abs(na - ref[2][0]), abs(nb - ref[2][1]))
+        log(f"{mode}: {ms:.3f} ms/UKS build, stages {tim}, dExc={r.get('dexc')}, "
+            f"dV={r.get('dvxc')}, dN={r.get('dnelec')}")
+        results[mode] = r
+        ctx.close()
+    head = "fp64" if "fp64" in results else next(iter(results))
+    R = results[head]
+    tim = R["timing"]
+    dens_ms = tim["density"][0] / max(tim["density"][1], 1)
+    vxc_ms = tim["vxc"][0] / max(tim["vxc"][1], 1)
+    kname, kms = ("density", dens_ms) if dens_ms >= vxc_ms else ("vxc", vxc_ms)
+    flops = R["info"]["contraction_flops"]   # per stage: one 2 n m^2 GEMM per spin
+    achieved = flops / (kms * 1e-3) / 1e12
+    out = {
+        "metric": "unrestricted XC builds/s (FP64)",
+        "value": R["per_s"], "unit": "UKS XC builds/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[head], "data": "synthetic",
+        "config": {"workload": "unrestricted " + desc + ", doublet cation (160 alpha, 159 beta)",
+                   "npts": len(wts), "nbf": nbf, "leaves": R["info"]["nleaves"],
+                   "precision": head},
+        "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved,
+                     "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS[head],
+                     "traffic": measured_traffic("uks32", head, kname), "kernel_ms": kms,
+                     "work": "sum over leaves of 2 n_b m_b^2 per spin (two GEMMs per stage)"},
+        "accuracy": {m: {k: results[m].get(k) for k in ("dexc", "dvxc", "dnelec")}
+                     for m in results},
         "modes_ms": {m: results[m]["ms_per_step"] for m in results},
         "cpu_baseline": cpu,
     }

@@ -143,6 +143,12 @@ occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc,
 occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, double *Vxc,
                                       double *exc, double *nelec);
 
+/* Same on device-resident buffers (d_D and d_Vxc 2nbf x nbf as above,
+ * d_result = {Exc, N_alpha, N_beta}), enqueued on `stream` (NULL: the
+ * context's own) without host synchronisation. */
+occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d_D,
+                                             double *d_Vxc, double *d_result, void *stream);
+
 /* Same build on device-resident buffers, enqueued on `stream` (a hipStream_t,
  * NULL for the context's own stream) without host synchronisation.
  * d_result receives {Exc, N_elec} (per molecule for a batch context, whose

@@ -78,6 +78,9 @@ SIGNATURES = {
     "occxc_build": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "occxc_xc_gradient": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _ip, _dp]),
     "occxc_build_unrestricted": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "occxc_build_unrestricted_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_void_p]),
     "occxc_functional_evaluate_polarized": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _dp,
                                                            _dp, _dp, _dp, _dp, ctypes.c_int]),
     "occxc_xc_gradient_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

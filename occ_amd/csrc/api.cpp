@@ -1183,6 +1183,36 @@ occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, doubl
   });
 }
 
+occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d_D,
+                                             double *d_Vxc, double *d_result, void *stream) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "occxc_build_unrestricted needs a single-molecule context");
+    OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
+                      ctx->opt.precision == OCCXC_PREC_FP32_CACHE,
+                  OCCXC_ERR_UNSUPPORTED,
+                  "unrestricted builds need an FP64 or FP32-cache context");
+    OCCXC_REQUIRE(ctx->plan.use_mfma && ctx->plan.max_mp <= kStoreMaxMp, OCCXC_ERR_UNSUPPORTED,
+                  "unrestricted builds need the matrix-core kernels and leaves of at most " +
+                      std::to_string(kStoreMaxMp) + " AOs");
+    DeviceGuard g(ctx->device);
+    uks_setup(ctx);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    OCCXC_HIP(hipMemcpyAsync(ctx->d_Du.get(), d_D, 2 * nn * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    run_build_u(ctx, s);
+    const size_t row = (size_t)ctx->nbf * sizeof(double);
+    OCCXC_HIP(hipMemcpy2DAsync(d_Vxc, 2 * row, ctx->d_V.get(), row, row, ctx->nbf,
+                               hipMemcpyDeviceToDevice, s));
+    OCCXC_HIP(hipMemcpy2DAsync(d_Vxc + ctx->nbf, 2 * row, ctx->d_V_b.get(), row, row, ctx->nbf,
+                               hipMemcpyDeviceToDevice, s));
+    OCCXC_HIP(hipMemcpyAsync(d_result, ctx->d_result_u.get(), 3 * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+  });
+}
+
 occxc_status occxc_batch_layout(const occxc_context *ctx, int *nmol, int *nbf,
                                 long long *offsets) {
   return guarded([&] {

@@ -120,6 +120,13 @@ class XCContext:
                                                        ctypes.byref(exc), _lib.dptr(nel)))
         return V, exc.value, (float(nel[0]), float(nel[1]))
 
+    def build_unrestricted_device(self, d_D, d_V, d_result, stream=None):
+        """occxc_build_unrestricted_device: device pointers to D and Vxc
+        (2 nbf x nbf each) and {Exc, N_alpha, N_beta}."""
+        _lib.check(_lib.lib().occxc_build_unrestricted_device(
+            self._h, ctypes.c_void_p(d_D), ctypes.c_void_p(d_V), ctypes.c_void_p(d_result),
+            ctypes.c_void_p(stream or 0)))
+
     def xc_gradient(self, D, shell_atom, natoms=None):
         """XC contribution to the nuclear gradient (occxc_xc_gradient):
         3 x natoms, column per atom."""

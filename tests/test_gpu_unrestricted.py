@@ -103,6 +103,16 @@ def test_uks_closed_shell_equals_restricted_on_gpu():
     assert abs(eu - e) <= 1e-11
     assert np.abs(Vu[:n] - V).max() <= 1e-11 and np.abs(Vu[n:] - V).max() <= 1e-11
     assert abs(na - nel / 2) <= 1e-11 and abs(nb - nel / 2) <= 1e-11
+    # the device-resident entry point gives the host path's numbers
+    from occ_amd.device import DeviceArray
+    Du = _open_shell(n, 10, 9, 43)
+    Vh, eh, (nah, nbh) = ctx.build_unrestricted(Du)
+    dD, dV, dr = DeviceArray(2 * n * n), DeviceArray(2 * n * n), DeviceArray(3)
+    dD.upload(np.asfortranarray(Du))
+    ctx.build_unrestricted_device(dD.ptr, dV.ptr, dr.ptr)
+    ctx.synchronize()
+    assert np.array_equal(dr.download(), np.array([eh, nah, nbh]))
+    assert np.abs(dV.download((2 * n, n)) - Vh).max() <= 1e-14
     # the restricted build is unaffected by the unrestricted one in between
     V2, e2, nel2 = ctx.build(D)
     assert e2 == e and nel2 == nel and np.abs(V2 - V).max() <= 1e-14
