@@ -175,7 +175,7 @@ __device__ void density_epilogue(const BuildArgs &a, const LeafDev &L, int leaf,
 // comes from the resident AO block, D (B operand) from LDS.
 template <typename S, typename T>
 __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__restrict__ ao,
-                                                      const T *__restrict__ D2t) {
+                                                      const T *__restrict__ D2t, int min_mp) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   constexpr int NV = 64, KC = 32, LDD = NV + 16 / (int)sizeof(T);
@@ -184,6 +184,7 @@ __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__re
   __shared__ T s_D[KC * LDD];
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
+  if (L.mp < min_mp) return;  // the per-wavefront kernels handle it
   const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -302,7 +303,7 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
   __shared__ int s_col[kMapCap];
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
-  if (L.mp < min_mp) return;  // k_density_blk handles it
+  if (L.mp < min_mp || L.mp > kMapCap) return;  // k_density_blk / k_density_mfma
   const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(256, 3) void k_density_lean(BuildArgs a, const S *_
   __shared__ int s_col[kMapCap];
   const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
   const LeafDev L = a.leaves[leaf];
-  (void)min_mp;
+  if (L.mp < min_mp || L.mp > kMapCap) return;  // k_density_mfma handles it
   const MolDev MD = a.mols[L.mol];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, g = lane >> 4;
@@ -1369,7 +1370,6 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   // compensated-density kernel and 3 the plain per-wavefront one.
   const char *dv = std::getenv("OCCXC_DENSITY_VARIANT");
   p.density_variant = dv ? std::atoi(dv) : kDefaultDensityVariant;
-  if (p.max_mp > kMapCap) p.density_variant = 1;
   // Vxc kernel: symmetric tile pairs for small leaves (0), general only (1),
   // or (2) as 0 with the K_loc blocks stored and reduced in a second pass
   p.vxc_variant = vxc_variant_setting();
@@ -1429,20 +1429,31 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
   const int wave_min = blk ? kBlkMaxMp + 1 : 0;
   const bool wave_too = !blk || plan.max_mp > kBlkMaxMp;
   const size_t blk_smem = density_blk_smem(std::min(plan.max_mp, kBlkMaxMp));
+  // the per-wavefront kernels keep a leaf's AO map in LDS and skip leaves of
+  // more than kMapCap AOs; the LDS-shared kernel takes those
+  const bool big = plan.max_mp > kMapCap;
+  const int big_min = kMapCap + 1;
   if (plan.density_variant == 4 && (precision == 0 || precision == 3)) {
-    if (precision == 0)
+    if (precision == 0) {
       k_density_lean<double, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, 0);
-    else
+      if (big)
+        k_density_mfma<double, double>
+            <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, big_min);
+    } else {
       k_density_lean<float, double>
           <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, 0);
+      if (big)
+        k_density_mfma<float, double>
+            <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, big_min);
+    }
     check_launch("k_density_lean");
     return;
   }
   if (precision == 0) {
     if (plan.density_variant == 1) {
       k_density_mfma<double, double>
-          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2);
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, 0);
     } else {
       if (blk)
         k_density_blk<double, double>
@@ -1450,6 +1461,9 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
       if (wave_too)
         k_density_wave<double, double>
             <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, wave_min);
+      if (big)
+        k_density_mfma<double, double>
+            <<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), a.D2, big_min);
     }
   } else if (precision == 3) {
     if (plan.density_variant != 1) {
@@ -1459,17 +1473,23 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
       if (wave_too)
         k_density_wave<float, double>
             <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, wave_min);
+      if (big)
+        k_density_mfma<float, double>
+            <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, big_min);
     } else
       k_density_mfma<float, double>
-          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2);
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), a.D2, 0);
   } else if (precision == 2) {
     k_density_bf16<<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao));
   } else {
     const long long nn = a.mtot;
     const int nb = (int)std::max<long long>(1, std::min<long long>((nn + 255) / 256, 4096));
     // FP32 mode: the compensated-density kernel unless the plain per-wavefront
-    // (variant 3) or LDS-shared (variant 1) kernel is asked for
-    if (plan.density_variant == 0 || plan.density_variant == 2) {
+    // (variant 3) or LDS-shared (variant 1) kernel is asked for; both of the
+    // former keep a leaf's AO map in LDS, so a context with a leaf of more
+    // than kMapCap AOs uses the LDS-shared kernel throughout
+    const int v = big ? 1 : plan.density_variant;
+    if (v == 0 || v == 2) {
       k_split_density<<<nb, 256, 0, s>>>(a.D2, plan.d2f.get(),
                                          reinterpret_cast<__bf16 *>(plan.d2l.get()), nn);
       check_launch("k_split_density");
@@ -1480,12 +1500,12 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
     }
     k_to_float<<<nb, 256, 0, s>>>(a.D2, plan.d2f.get(), nn);
     check_launch("k_to_float");
-    if (plan.density_variant == 3)
+    if (v == 3)
       k_density_wave<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
                                                              plan.d2f.get(), 0);
     else
       k_density_mfma<float, float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao),
-                                                             plan.d2f.get());
+                                                             plan.d2f.get(), 0);
   }
   check_launch("k_density_mfma");
 }

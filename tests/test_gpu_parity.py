@@ -259,6 +259,26 @@ def test_density_variants_agree(monkeypatch):
         assert np.abs(out[0][0] - out[k][0]).max() <= 1e-12
 
 
+def test_density_kernels_split_by_leaf_width(monkeypatch):
+    """Leaves wider than the per-wavefront kernels' LDS AO map (2048 AOs) go
+    to the LDS-shared kernel while the rest stay on the per-wavefront one:
+    same results as the LDS-shared kernel throughout.  (H2O)90 / def2-SVP
+    (2160 AOs) on a coarse grid has leaves of every width."""
+    atoms = molecules.water_cluster(90, seed=3)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms, 11, 26)
+    D = random_density(basis.nbf, 40, 53)
+    out = []
+    for v in ("2", "1"):
+        monkeypatch.setenv("OCCXC_DENSITY_VARIANT", v)
+        dft = DFT("pbe", basis, pts, wts)
+        out.append((dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density()))
+        print(f"density variant {v}: nbf {basis.nbf}, {len(wts)} points, "
+              f"mean m_b {dft.ctx.info()['mean_local_nbf']:.0f}", flush=True)
+    assert abs(out[0][1] - out[1][1]) <= 1e-10 and abs(out[0][2] - out[1][2]) <= 1e-10
+    assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12
+
+
 def test_device_setup_matches_host_setup(monkeypatch):
     """The leaf layout built on the device (Morton sort, leaf spheres,
     screening, leaf tables) is the host restatement's, bit for bit: same
