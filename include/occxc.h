@@ -180,6 +180,15 @@ occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
 occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int natoms,
                                       const int *shell_atom, double *d_grad, void *stream);
 
+/* The XC gradient for an unrestricted density: compute_xc_gradient_impl<d,
+ * Unrestricted> (dft.h:598-749) with process_grid_block_gradient<d,
+ * Unrestricted> (src/dft/dft_gradient_kernels.cpp:473-583) and the polarised
+ * functionals.  D as occxc_build_unrestricted (2nbf x nbf, alpha block over
+ * beta block), the rest as occxc_xc_gradient.  FP64 and FP32-cache contexts
+ * of one molecule. */
+occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D, int natoms,
+                                            const int *shell_atom, double *grad);
+
 /* --- batches: many molecules in one context, one launch per stage ----- */
 
 /* A context over nmol molecules (each its own basis and grid; shared

@@ -78,6 +78,8 @@ SIGNATURES = {
     "occxc_build": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "occxc_xc_gradient": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _ip, _dp]),
     "occxc_build_unrestricted": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "occxc_xc_gradient_unrestricted": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int, _ip,
+                                                      _dp]),
     "occxc_build_unrestricted_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                        ctypes.c_void_p, ctypes.c_void_p,
                                                        ctypes.c_void_p]),

@@ -892,6 +892,43 @@ void run_build_u(occxc_context *c, hipStream_t s) {
   });
 }
 
+void uks_setup(occxc_context *c);
+
+// Unrestricted XC gradient on stream s from the raw density in d_Du: both
+// spin densities and the polarised functional stage as in run_build_u, then
+// the gradient partials once per spin (P_s = 2 D_s, that spin's
+// coefficients) and one per-atom reduction of both
+// (process_grid_block_gradient<d, Unrestricted>, dft_gradient_kernels.cpp:
+// 473-583).
+void run_gradient_u(occxc_context *c, double *d_grad, hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const int prec = c->opt.precision;
+  StageTimer t{c, s, {}};
+  t.run(OCCXC_STAGE_PREPARE, [&] {
+    launch_prepare_density_u(c->d_Du.get(), c->d_D2a.get(), c->d_D2b.get(), c->nbf, s);
+  });
+  t.run(OCCXC_STAGE_DENSITY, [&] {
+    BuildArgs x = a;
+    x.rho_out = c->d_rho_a.get();
+    run_density_store(x, c->d_ao.get(), prec, c->d_D2a.get(), s);
+    x.rho_out = c->d_rho_b.get();
+    run_density_store(x, c->d_ao.get(), prec, c->d_D2b.get(), s);
+    launch_functional_pol_leaves(a, c->d_rho_a.get(), c->d_rho_b.get(), c->d_coef_b.get(),
+                                 c->d_part_u.get(), s);
+  });
+  t.run(OCCXC_STAGE_GRADIENT, [&] {
+    const bool fp32 = prec != OCCXC_PREC_FP64;
+    BuildArgs x = a;
+    x.D2 = c->d_D2a.get();
+    launch_xc_gradient_partials(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart.get(), s);
+    x.D2 = c->d_D2b.get();
+    x.coef = c->d_coef_b.get();
+    launch_xc_gradient_partials(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart2.get(), s);
+    launch_grad_reduce(c->d_gpart.get(), c->d_gpart2.get(), c->grad_natoms,
+                       c->d_gatom_off.get(), c->d_gatom_ent.get(), d_grad, s);
+  });
+}
+
 void uks_setup(occxc_context *c) {
   if (c->uks_ready) return;
   const size_t nn = (size_t)c->nbf * c->nbf;
@@ -1210,6 +1247,40 @@ occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d
                                hipMemcpyDeviceToDevice, s));
     OCCXC_HIP(hipMemcpyAsync(d_result, ctx->d_result_u.get(), 3 * sizeof(double),
                              hipMemcpyDeviceToDevice, s));
+  });
+}
+
+occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D, int natoms,
+                                            const int *shell_atom, double *grad) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "occxc_xc_gradient needs a single-molecule context");
+    OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
+    for (int k = 0; k < ctx->nsh; ++k)
+      OCCXC_REQUIRE(shell_atom[k] >= 0 && shell_atom[k] < natoms, OCCXC_ERR_INVALID_ARGUMENT,
+                    "shell_atom: atom index out of range");
+    OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
+                      ctx->opt.precision == OCCXC_PREC_FP32_CACHE,
+                  OCCXC_ERR_UNSUPPORTED,
+                  "unrestricted gradients need an FP64 or FP32-cache context");
+    OCCXC_REQUIRE(ctx->plan.use_mfma && ctx->plan.max_mp <= kStoreMaxMp, OCCXC_ERR_UNSUPPORTED,
+                  "unrestricted gradients need the matrix-core kernels and leaves of at most " +
+                      std::to_string(kStoreMaxMp) + " AOs");
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    for (size_t i = 0; i < 2 * nn; ++i)
+      OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
+    DeviceGuard g(ctx->device);
+    gradient_setup(ctx, natoms, shell_atom);
+    uks_setup(ctx);
+    if (ctx->d_gpart2.size() < ctx->d_gpart.size()) {
+      ctx->d_gpart2.alloc(ctx->d_gpart.size());
+      ctx->device_bytes += ctx->d_gpart2.bytes();
+    }
+    ctx->d_Du.upload(D, 2 * nn, ctx->stream);
+    run_gradient_u(ctx, ctx->d_grad.get(), ctx->stream);
+    ctx->d_grad.download(grad, 3 * (size_t)natoms, ctx->stream);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 

@@ -63,7 +63,7 @@ struct occxc_context {
   // per-AO partial sums, and the per-atom entry lists built for the
   // shell-to-atom map of the last call
   std::vector<int> bf_shell;
-  occxc::DevBuf<double> d_ao2, d_gpart, d_grad;
+  occxc::DevBuf<double> d_ao2, d_gpart, d_gpart2, d_grad;
   occxc::DevBuf<int> d_gatom_off, d_gatom_ent;
   bool ao2_ready = false;
   std::vector<int> grad_shell_atom;

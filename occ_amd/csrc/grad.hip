@@ -186,9 +186,11 @@ __global__ __launch_bounds__(256, 2) void k_xc_grad(BuildArgs a, const S *__rest
   }
 }
 
-// grad[3 A + k] = -2 sum of gpart over the entries of atom A, in a fixed
+// grad[3 A + k] = -2 sum of gpart (plus gpart2, the beta spin of an
+// unrestricted gradient, when given) over the entries of atom A, in a fixed
 // order (per-thread strided sums, then a fixed tree).
 __global__ __launch_bounds__(256) void k_grad_reduce(const double *__restrict__ gpart,
+                                                     const double *__restrict__ gpart2,
                                                      const int *__restrict__ atom_off,
                                                      const int *__restrict__ atom_ent,
                                                      double *__restrict__ grad) {
@@ -201,6 +203,11 @@ __global__ __launch_bounds__(256) void k_grad_reduce(const double *__restrict__ 
     v[0] += gpart[e];
     v[1] += gpart[e + 1];
     v[2] += gpart[e + 2];
+    if (gpart2) {
+      v[0] += gpart2[e];
+      v[1] += gpart2[e + 1];
+      v[2] += gpart2[e + 2];
+    }
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) s_r[k][threadIdx.x] = v[k];
@@ -221,6 +228,28 @@ void check(const char *what) {
 
 } // namespace
 
+void launch_xc_gradient_partials(const BuildArgs &a, const void *ao, bool ao_fp32,
+                                 const double *ao2, double *gpart, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  const bool gga = a.deriv != 0;
+  if (ao_fp32) {
+    const float *p = static_cast<const float *>(ao);
+    if (gga) k_xc_grad<float, true><<<a.nleaves, 256, 0, s>>>(a, p, ao2, gpart);
+    else k_xc_grad<float, false><<<a.nleaves, 256, 0, s>>>(a, p, ao2, gpart);
+  } else {
+    const double *p = static_cast<const double *>(ao);
+    if (gga) k_xc_grad<double, true><<<a.nleaves, 256, 0, s>>>(a, p, ao2, gpart);
+    else k_xc_grad<double, false><<<a.nleaves, 256, 0, s>>>(a, p, ao2, gpart);
+  }
+  check("k_xc_grad");
+}
+
+void launch_grad_reduce(const double *gpart, const double *gpart2, int natoms,
+                        const int *atom_off, const int *atom_ent, double *grad, hipStream_t s) {
+  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, gpart2, atom_off, atom_ent, grad);
+  check("k_grad_reduce");
+}
+
 void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const double *ao2,
                         double *gpart, int natoms, const int *atom_off, const int *atom_ent,
                         double *grad, hipStream_t s) {
@@ -237,7 +266,7 @@ void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const 
     }
     check("k_xc_grad");
   }
-  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, atom_off, atom_ent, grad);
+  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, nullptr, atom_off, atom_ent, grad);
   check("k_grad_reduce");
 }
 

@@ -156,6 +156,12 @@ void launch_ao_extra(const BuildArgs &a, double *out, bool hessian, hipStream_t 
 // the leaf's points of the gradient integrand into gpart[3 (map_off + mu) + k],
 // then grad[3 A + k] = -2 sum over the entries of atom A (CSR atom_off /
 // atom_ent), in a fixed order.
+// the same in two steps: per-AO partials of one spin density (a.D2 and
+// a.coef of that spin), then the per-atom reduction of one or two partial sets
+void launch_xc_gradient_partials(const BuildArgs &a, const void *ao, bool ao_fp32,
+                                 const double *ao2, double *gpart, hipStream_t s);
+void launch_grad_reduce(const double *gpart, const double *gpart2, int natoms,
+                        const int *atom_off, const int *atom_ent, double *grad, hipStream_t s);
 void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const double *ao2,
                         double *gpart, int natoms, const int *atom_off, const int *atom_ent,
                         double *grad, hipStream_t s);

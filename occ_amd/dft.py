@@ -120,6 +120,20 @@ class XCContext:
                                                        ctypes.byref(exc), _lib.dptr(nel)))
         return V, exc.value, (float(nel[0]), float(nel[1]))
 
+    def xc_gradient_unrestricted(self, D, shell_atom, natoms=None):
+        """Unrestricted XC gradient (occxc_xc_gradient_unrestricted): D is
+        (2 nbf, nbf); returns 3 x natoms."""
+        D = np.asfortranarray(D, dtype=np.float64)
+        if D.shape != (2 * self.nbf, self.nbf):
+            raise ValueError(f"D must be {2 * self.nbf}x{self.nbf}")
+        sa = _lib.i32(shell_atom)
+        natoms = int(natoms if natoms is not None else (sa.max() + 1 if len(sa) else 0))
+        G = np.zeros((3, max(natoms, 1)), order="F")
+        _lib.check(_lib.lib().occxc_xc_gradient_unrestricted(
+            self._h, D.ctypes.data_as(_lib._dp), natoms, _lib.iptr(sa),
+            G.ctypes.data_as(_lib._dp)))
+        return G
+
     def build_unrestricted_device(self, d_D, d_V, d_result, stream=None):
         """occxc_build_unrestricted_device: device pointers to D and Vxc
         (2 nbf x nbf each) and {Exc, N_alpha, N_beta}."""
@@ -311,7 +325,12 @@ class DFT:
         return self._exc
 
     def compute_xc_gradient(self, D):
-        """The 3 x natoms matrix OCC's compute_xc_gradient returns (dft.h:531)."""
+        """The 3 x natoms matrix OCC's compute_xc_gradient returns (dft.h:531);
+        a (2 nbf, nbf) D is unrestricted."""
+        D = np.asarray(D)
+        if D.shape == (2 * self.basis.nbf, self.basis.nbf):
+            return self.ctx.xc_gradient_unrestricted(D, self.basis.shell_atom,
+                                                     len(self.basis.atoms))
         return self.ctx.xc_gradient(D, self.basis.shell_atom, len(self.basis.atoms))
 
     def total_density(self):

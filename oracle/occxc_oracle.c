@@ -1533,6 +1533,277 @@ int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double 
   return 0;
 }
 
+/* kernels::process_grid_block_gradient<d, Unrestricted>
+ * (dft_gradient_kernels.cpp:473-583) with accumulate_gradient_contribution_U
+ * <0> and <1> (dft_gradient_kernels.cpp:176-290).  phi is [10][m][n]; Dloc
+ * is [2m][m] (alpha rows over beta rows) of 2 D; g is 3 x natoms. */
+static void process_block_gradient_u(const block_cfg *cfg, int m, long n, const double *phi,
+                                     const double *Dloc, const double *wts, const int *bf_atom,
+                                     double *g, double *work) {
+  const long mn = (long)m * n;
+  const double *p0 = phi, *pd[3] = {phi + mn, phi + 2 * mn, phi + 3 * mn};
+  const double *ph[3][3] = {{phi + 4 * mn, phi + 5 * mn, phi + 6 * mn},
+                            {phi + 5 * mn, phi + 7 * mn, phi + 8 * mn},
+                            {phi + 6 * mn, phi + 8 * mn, phi + 9 * mn}};
+  const int ld = 2 * m;
+  double *T = work;                 /* [m][n] */
+  double *rho = T + mn;             /* [2][4][n] */
+  double *exc = rho + 8 * n, *vr = exc + n, *vs = vr + 2 * n, *rr = vs + 3 * n, *ss = rr + 2 * n;
+  double *A = ss + 3 * n;           /* [m][n] */
+  double *V = A + mn;               /* [2][3][m x m] */
+  memset(rho, 0, sizeof(double) * 8 * n);
+  for (int sp = 0; sp < 2; sp++) { /* evaluate_density<d, U> */
+    double *r = rho + 4L * n * sp;
+    memset(T, 0, sizeof(double) * mn);
+    for (int nu = 0; nu < m; nu++) {
+      double *t = T + (long)nu * n;
+      for (int mu = 0; mu < m; mu++) {
+        const double dm = Dloc[sp * m + mu + (long)nu * ld];
+        const double *f = p0 + (long)mu * n;
+        for (long p = 0; p < n; p++) t[p] += dm * f[p];
+      }
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *t = T + (long)nu * n, *f = p0 + (long)nu * n;
+      for (long p = 0; p < n; p++) r[p] += f[p] * t[p];
+      if (cfg->deriv)
+        for (int k = 0; k < 3; k++) {
+          const double *fk = pd[k] + (long)nu * n;
+          for (long p = 0; p < n; p++) r[(k + 1) * n + p] += 2.0 * fk[p] * t[p];
+        }
+    }
+  }
+  const double *ra = rho, *rb = rho + 4 * n;
+  double rmax = -1e300;
+  for (long p = 0; p < n; p++) {
+    if (ra[p] > rmax) rmax = ra[p];
+    if (rb[p] > rmax) rmax = rb[p];
+  }
+  if (rmax < cfg->dthr) return; /* dft_gradient_kernels.cpp:488-491 */
+  for (long p = 0; p < n; p++) {
+    exc[p] = 0.0;
+    vr[2 * p] = vr[2 * p + 1] = 0.0;
+    vs[3 * p] = vs[3 * p + 1] = vs[3 * p + 2] = 0.0;
+    rr[2 * p] = ra[p];
+    rr[2 * p + 1] = rb[p];
+    ss[3 * p] = ra[n + p] * ra[n + p] + ra[2 * n + p] * ra[2 * n + p] + ra[3 * n + p] * ra[3 * n + p];
+    ss[3 * p + 1] = ra[n + p] * rb[n + p] + ra[2 * n + p] * rb[2 * n + p] + ra[3 * n + p] * rb[3 * n + p];
+    ss[3 * p + 2] = rb[n + p] * rb[n + p] + rb[2 * n + p] * rb[2 * n + p] + rb[3 * n + p] * rb[3 * n + p];
+  }
+  for (int f = 0; f < cfg->nfunc; f++) oracle_functional_eval_pol(cfg->ids[f], n, rr, ss, exc, vr, vs);
+  for (long p = 0; p < n; p++) {
+    for (int k = 0; k < 2; k++) vr[2 * p + k] *= wts[p];
+    for (int k = 0; k < 3; k++) vs[3 * p + k] *= wts[p];
+  }
+  memset(V, 0, sizeof(double) * 6 * m * m);
+  for (int sp = 0; sp < 2; sp++) {
+    const double *r_s = sp ? rb : ra, *r_o = sp ? ra : rb;
+    double *Vs[3] = {V + (long)(3 * sp) * m * m, V + (long)(3 * sp + 1) * m * m,
+                     V + (long)(3 * sp + 2) * m * m};
+    if (!cfg->deriv) {
+      /* <0>: V_k = d_k phi^T (phi vrho_s) */
+      for (int nu = 0; nu < m; nu++) {
+        const double *fn = p0 + (long)nu * n;
+        for (int mu = 0; mu < m; mu++)
+          for (int k = 0; k < 3; k++) {
+            const double *fk = pd[k] + (long)mu * n;
+            double acc = 0.0;
+            for (long p = 0; p < n; p++) acc += fk[p] * fn[p] * vr[2 * p + sp];
+            Vs[k][mu + (long)nu * m] += acc;
+          }
+      }
+      continue;
+    }
+    /* <1>: g_s = 2 vsigma_ss grad rho_s + vsigma_ab grad rho_other;
+     * aow = vrho_s/2 phi + g_s . grad phi; V_k += d_k phi^T aow;
+     * aow_k = vrho_s/2 d_k phi + g_s . grad d_k phi; V_k += (phi^T aow_k)^T */
+    for (int nu = 0; nu < m; nu++) {
+      const long o = (long)nu * n;
+      for (long p = 0; p < n; p++) {
+        double gsum = 0.5 * vr[2 * p + sp] * p0[o + p];
+        for (int j = 0; j < 3; j++)
+          gsum += (2.0 * vs[3 * p + (sp ? 2 : 0)] * r_s[(j + 1) * n + p] +
+                   vs[3 * p + 1] * r_o[(j + 1) * n + p]) * pd[j][o + p];
+        A[o + p] = gsum;
+      }
+    }
+    for (int nu = 0; nu < m; nu++) {
+      const double *an = A + (long)nu * n;
+      for (int mu = 0; mu < m; mu++)
+        for (int k = 0; k < 3; k++) {
+          const double *fk = pd[k] + (long)mu * n;
+          double acc = 0.0;
+          for (long p = 0; p < n; p++) acc += fk[p] * an[p];
+          Vs[k][mu + (long)nu * m] += acc;
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+      for (int mu = 0; mu < m; mu++) {
+        const long o = (long)mu * n;
+        for (long p = 0; p < n; p++) {
+          double gsum = 0.5 * vr[2 * p + sp] * pd[k][o + p];
+          for (int j = 0; j < 3; j++)
+            gsum += (2.0 * vs[3 * p + (sp ? 2 : 0)] * r_s[(j + 1) * n + p] +
+                     vs[3 * p + 1] * r_o[(j + 1) * n + p]) * ph[k][j][o + p];
+          A[o + p] = gsum;
+        }
+      }
+      for (int nu = 0; nu < m; nu++) {
+        const double *fn = p0 + (long)nu * n;
+        for (int mu = 0; mu < m; mu++) {
+          const double *am = A + (long)mu * n;
+          double acc = 0.0;
+          for (long p = 0; p < n; p++) acc += am[p] * fn[p];
+          Vs[k][mu + (long)nu * m] += acc;
+        }
+      }
+    }
+  }
+  /* dft_gradient_kernels.cpp:561-582: 2 (V_a D_a + V_b D_b), skipping
+   * |D_a + D_b| < 1e-12 */
+  for (int mu = 0; mu < m; mu++) {
+    const int at = bf_atom[mu];
+    for (int nu = 0; nu < m; nu++) {
+      const double da = Dloc[mu + (long)nu * ld], db = Dloc[m + mu + (long)nu * ld];
+      if (fabs(da + db) < 1e-12) continue;
+      for (int k = 0; k < 3; k++)
+        g[3 * at + k] -= 2.0 * (V[(long)k * m * m + mu + (long)nu * m] * da +
+                                V[(long)(3 + k) * m * m + mu + (long)nu * m] * db);
+    }
+  }
+}
+
+int oracle_xc_gradient_unrestricted(const oracle_basis *b, const int *shell_atom, int natoms,
+                                    long npts, const double *xyz, const double *w, int nfunc,
+                                    const int *func_ids, const double *D,
+                                    double density_threshold, double screening_threshold,
+                                    int screened, int nthreads, double *grad) {
+  if (!b || !shell_atom || natoms <= 0 || npts < 0 || nfunc <= 0) return -1;
+  int deriv = 0;
+  for (int f = 0; f < nfunc; f++) {
+    int fam = oracle_functional_family(func_ids[f]);
+    if (fam < 0) return -1;
+    if (fam > deriv) deriv = fam;
+  }
+  const int nsh = b->nshell;
+  for (int s = 0; s < nsh; s++)
+    if (shell_atom[s] < 0 || shell_atom[s] >= natoms) return -1;
+  const int nbf = oracle_nbf(b);
+  const long ldD = 2L * nbf;
+  int *first_bf = malloc(sizeof(int) * (nsh + 1));
+  first_bf[0] = 0;
+  for (int s = 0; s < nsh; s++) first_bf[s + 1] = first_bf[s] + oracle_shell_size(b->l[s], b->pure[s]);
+  memset(grad, 0, sizeof(double) * 3 * natoms);
+  block_cfg cfg = {deriv, nfunc, func_ids, density_threshold};
+  const double *pxyz = xyz, *pw = w;
+  double *sxyz = 0, *sw = 0, *lcen = 0, *lrad = 0, *ext = 0;
+  long *perm = 0, *loff = 0, *lcnt = 0;
+  long nblk;
+  if (screened) {
+    sxyz = malloc(sizeof(double) * 3 * (npts + 1));
+    sw = malloc(sizeof(double) * (npts + 1));
+    perm = malloc(sizeof(long) * (npts + 1));
+    long cap = npts / 32 + 2;
+    loff = malloc(sizeof(long) * cap);
+    lcnt = malloc(sizeof(long) * cap);
+    lcen = malloc(sizeof(double) * 3 * cap);
+    lrad = malloc(sizeof(double) * cap);
+    nblk = oracle_build_hierarchy(npts, xyz, w, sxyz, sw, perm, loff, lcnt, lcen, lrad);
+    pxyz = sxyz;
+    pw = sw;
+    ext = malloc(sizeof(double) * nsh);
+    oracle_decay_cutoff(b, screening_threshold, ext);
+  } else {
+    nblk = (npts + 127) / 128;
+    loff = malloc(sizeof(long) * (nblk + 1));
+    lcnt = malloc(sizeof(long) * (nblk + 1));
+    for (long i = 0; i < nblk; i++) {
+      loff[i] = 128 * i;
+      lcnt[i] = (npts - 128 * i < 128) ? npts - 128 * i : 128;
+    }
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    double *gloc = calloc((size_t)3 * natoms, sizeof(double));
+    int *sig = malloc(sizeof(int) * (nsh + 1));
+    int *gidx = malloc(sizeof(int) * (nbf + 1));
+    int *batom = malloc(sizeof(int) * (nbf + 1));
+    long maxn = 256;
+    double *phi = malloc(sizeof(double) * 10 * (size_t)nbf * maxn);
+    double *tmp = malloc(sizeof(double) * 10 * (ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2 * maxn);
+    double *Dloc = malloc(sizeof(double) * 2 * (size_t)nbf * nbf);
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 20 * maxn +
+                                            6 * (size_t)nbf * nbf));
+#pragma omp for schedule(dynamic, 4)
+    for (long li = 0; li < nblk; li++) {
+      long n = lcnt[li];
+      if (n == 0) continue;
+      int nsig = 0;
+      if (screened) {
+        for (int s = 0; s < nsh; s++) {
+          const double *o = b->origin + 3 * s;
+          double dx = o[0] - lcen[3 * li], dy = o[1] - lcen[3 * li + 1], dz = o[2] - lcen[3 * li + 2];
+          if (sqrt(dx * dx + dy * dy + dz * dz) - ext[s] < lrad[li]) sig[nsig++] = s;
+        }
+        if (nsig == 0) continue;
+      } else {
+        for (int s = 0; s < nsh; s++) sig[nsig++] = s;
+      }
+      int m = 0;
+      for (int k = 0; k < nsig; k++) {
+        int s = sig[k];
+        for (int q = first_bf[s]; q < first_bf[s + 1]; q++) {
+          batom[m] = shell_atom[s];
+          gidx[m++] = q;
+        }
+      }
+      /* dft.h:717-726: Dloc per spin block of D = 2 mo.D */
+      for (int sp = 0; sp < 2; sp++)
+        for (int j = 0; j < m; j++)
+          for (int i = 0; i < m; i++)
+            Dloc[sp * m + i + (long)j * 2 * m] = 2.0 * D[sp * nbf + gidx[i] + (long)gidx[j] * ldD];
+      const double *bx = pxyz + 3 * loff[li];
+      int off = 0;
+      for (int k = 0; k < nsig; k++) {
+        int s = sig[k], sz = oracle_shell_size(b->l[s], b->pure[s]);
+        oracle_eval_shell2(b->l[s], b->pure[s], b->nprim[s], b->exps + b->prim_offset[s],
+                           b->coefs + b->prim_offset[s], b->origin + 3 * s, n, bx, tmp, n);
+        for (int q = 0; q < 10; q++)
+          for (int o = 0; o < sz; o++)
+            memcpy(phi + ((long)q * m + off + o) * n, tmp + ((long)q * sz + o) * n,
+                   sizeof(double) * n);
+        off += sz;
+      }
+      process_block_gradient_u(&cfg, m, n, phi, Dloc, pw + loff[li], batom, gloc, work);
+    }
+#pragma omp critical
+    {
+      for (int i = 0; i < 3 * natoms; i++) grad[i] += gloc[i];
+    }
+    free(gloc);
+    free(sig);
+    free(gidx);
+    free(batom);
+    free(phi);
+    free(tmp);
+    free(Dloc);
+    free(work);
+  }
+  free(first_bf);
+  free(sxyz);
+  free(sw);
+  free(perm);
+  free(loff);
+  free(lcnt);
+  free(lcen);
+  free(lrad);
+  free(ext);
+  return 0;
+}
+
 long oracle_screened_work(const oracle_basis *b, long npts, const double *xyz,
                           const double *w, double screening_threshold,
                           long maxleaves, long *leaf_n, long *leaf_m) {

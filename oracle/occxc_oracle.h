@@ -143,6 +143,16 @@ int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double 
                                  double screening_threshold, int screened, int nthreads,
                                  double *K, double *exc, double *nelec);
 
+/* compute_xc_gradient_impl<d, Unrestricted> (dft.h:598-749) with
+ * process_grid_block_gradient<d, Unrestricted> (dft_gradient_kernels.cpp:
+ * 473-583): D as oracle_xc_build_unrestricted, the rest as
+ * oracle_xc_gradient. */
+int oracle_xc_gradient_unrestricted(const oracle_basis *b, const int *shell_atom, int natoms,
+                                    long npts, const double *xyz, const double *w, int nfunc,
+                                    const int *func_ids, const double *D,
+                                    double density_threshold, double screening_threshold,
+                                    int screened, int nthreads, double *grad);
+
 /* Algorithmic work of the screened build: per-leaf (n_b, m_b) with the
  * reference screening rule.  Writes up to maxleaves entries; returns the
  * number of leaves with >= 1 significant shell. */

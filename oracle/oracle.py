@@ -67,6 +67,7 @@ def lib():
                                                    ctypes.c_int, _ip, _dp, ctypes.c_double,
                                                    ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                    _dp, _dp, _dp]
+        L.oracle_xc_gradient_unrestricted.argtypes = L.oracle_xc_gradient.argtypes
         L.oracle_screened_work.restype = ctypes.c_long
         L.oracle_screened_work.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp,
                                            ctypes.c_double, ctypes.c_long, _lp, _lp]
@@ -283,6 +284,28 @@ def xc_gradient(basis, shell_atom, natoms, xyz, w, func_ids, D, density_threshol
                                   G.ctypes.data_as(_dp))
     if rc != 0:
         raise ValueError("oracle_xc_gradient rejected its input")
+    return G
+
+
+def xc_gradient_unrestricted(basis, shell_atom, natoms, xyz, w, func_ids, D,
+                             density_threshold=1e-10, screening_threshold=1e-10, screened=True,
+                             nthreads=0):
+    """compute_xc_gradient for an unrestricted D ((2 nbf, nbf), alpha block
+    over beta block): the 3 x natoms XC gradient."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    ids = np.ascontiguousarray(func_ids, dtype=np.int32)
+    sa = np.ascontiguousarray(shell_atom, dtype=np.int32)
+    D = np.asfortranarray(D, dtype=np.float64)
+    nbf = basis.nbf
+    assert D.shape == (2 * nbf, nbf) and len(sa) == len(basis.l)
+    G = np.zeros((3, natoms), order="F")
+    rc = lib().oracle_xc_gradient_unrestricted(basis.ptr, _i(sa), int(natoms), len(w), _d(xyz),
+                                               _d(w), len(ids), _i(ids), D.ctypes.data_as(_dp),
+                                               density_threshold, screening_threshold,
+                                               int(screened), int(nthreads), G.ctypes.data_as(_dp))
+    if rc != 0:
+        raise ValueError("oracle_xc_gradient_unrestricted rejected its input")
     return G
 
 

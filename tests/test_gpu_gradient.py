@@ -158,3 +158,30 @@ def test_gradient_argument_checks():
     st = L.occxc_xc_gradient(batch.handle, Df.ctypes.data_as(_lib._dp), 3, _lib.iptr(sa),
                              G.ctypes.data_as(_lib._dp))
     assert st == 1 and b"single-molecule" in L.occxc_last_error_message()
+
+
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+def test_unrestricted_gradient_matches_oracle(method):
+    """compute_xc_gradient_impl<d, Unrestricted> on the GPU against the oracle
+    (tolerance as above); its closed-shell limit equals the restricted GPU
+    gradient."""
+    for atoms, bname, pure in ((molecules.water(), "6-31g", False),
+                               (molecules.water_cluster(2, seed=1), "def2-svp", True)):
+        basis = AOBasis.load(atoms, bname, pure=pure)
+        pts, wts = _grid(atoms)
+        n = basis.nbf
+        ne = molecules.nuclear_charge(atoms)
+        Du = np.vstack([random_density(n, (ne + 1) // 2, 61) / 2,
+                        random_density(n, (ne - 1) // 2, 62) / 2])
+        dft = DFT(method, basis, pts, wts)
+        G = dft.compute_xc_gradient(Du)
+        ob = oracle.OracleBasis.from_flat(basis.flat())
+        G0 = oracle.xc_gradient_unrestricted(ob, basis.shell_atom, len(atoms), pts, wts,
+                                             dft.ids, Du)
+        err = np.abs(G - G0).max()
+        print(f"UKS gradient {bname} {method}: max|dG|={err:.2e}")
+        assert err <= TOL_G
+        D = random_density(n, ne // 2, 63)
+        Gr = dft.compute_xc_gradient(D)
+        Gc = dft.compute_xc_gradient(np.vstack([D / 2, D / 2]))
+        assert np.abs(Gc - Gr).max() <= 1e-11

@@ -76,3 +76,32 @@ def test_gradient_rejects_bad_atoms(water):
     bad[0] = 3
     with pytest.raises(ValueError):
         oracle.xc_gradient(ob, bad, 3, pts, w, [1, 7], D)
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+def test_unrestricted_gradient_is_the_derivative_of_exc(water, method):
+    """compute_xc_gradient_impl<d, Unrestricted>: the derivative of the
+    unrestricted Exc over shifted shell centres; its closed-shell limit
+    (D_a = D_b = D/2) is the restricted gradient."""
+    basis, pts, w = water
+    flat = basis.flat()
+    ob = oracle.OracleBasis.from_flat(flat)
+    n = basis.nbf
+    Du = np.vstack([random_density(n, 5, 4) / 2, random_density(n, 4, 5) / 2])
+    sa = basis.shell_atom
+    ids = METHODS[method]
+    G = oracle.xc_gradient_unrestricted(ob, sa, 3, pts, w, ids, Du, screened=False)
+    h = 1e-4
+    for A in range(3):
+        for k in range(3):
+            ep = oracle.xc_build_unrestricted(_shifted(flat, sa, A, k, h), pts, w, ids, Du,
+                                              screened=False)[1]
+            em = oracle.xc_build_unrestricted(_shifted(flat, sa, A, k, -h), pts, w, ids, Du,
+                                              screened=False)[1]
+            assert G[k, A] == pytest.approx((ep - em) / (2 * h), abs=1e-7)
+    Gs = oracle.xc_gradient_unrestricted(ob, sa, 3, pts, w, ids, Du, screened=True)
+    assert np.abs(Gs - G).max() < 1e-8
+    D = random_density(n, 5, 4)
+    Gr = oracle.xc_gradient(ob, sa, 3, pts, w, ids, D)
+    Gc = oracle.xc_gradient_unrestricted(ob, sa, 3, pts, w, ids, np.vstack([D / 2, D / 2]))
+    assert np.abs(Gc - Gr).max() < 1e-12
