@@ -104,7 +104,10 @@ OCCXC_HD inline D5 d5_cbrt(const D5 &x) {
   const double c = cbrt(x.v);
   return d5chain(x, c, c / (3.0 * x.v));
 }
+// x^p; at x = 0 (a spin channel with no density, e.g. the beta spin of a
+// one-electron system) value 0 and, for p > 1, derivative 0
 OCCXC_HD inline D5 d5_pow(const D5 &x, double p) {
+  if (x.v == 0.0 && p > 1.0) return d5c(0.0);
   const double y = pow(x.v, p);
   return d5chain(x, y, p * y / x.v);
 }

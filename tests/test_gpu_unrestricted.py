@@ -141,3 +141,26 @@ def test_uks_rejections():
     st = L.occxc_build_unrestricted(ctx.handle, bad.ctypes.data_as(_lib._dp),
                                     V.ctypes.data_as(_lib._dp), ctypes.byref(e), _lib.dptr(nel))
     assert st == 1
+
+
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+def test_uks_one_electron_beta_density_zero(method):
+    """A one-electron system (hydrogen atom, D_beta = 0): rho_beta vanishes
+    everywhere and zeta = 1; the build stays finite and matches the oracle."""
+    atoms = [(1, 0.0, 0.0, 0.0)]
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms, 41, 146)
+    n = basis.nbf
+    Du = np.vstack([random_density(n, 1, 81) / 2, np.zeros((n, n))])
+    dft = DFT(method, basis, pts, wts)
+    V = dft.compute_vxc(Du)
+    ob = oracle.OracleBasis.from_flat(basis.flat())
+    K, exc, (na, nb) = oracle.xc_build_unrestricted(ob, pts, wts, dft.ids, Du)
+    ga, gb = dft.total_density()
+    print(f"H atom UKS {method}: Exc={dft.exchange_correlation_energy():.10f} "
+          f"|dExc|={abs(dft.exchange_correlation_energy() - exc):.2e} "
+          f"max|dV|={np.abs(V - K).max():.2e} N=({ga:.6f}, {gb:.1e})")
+    assert np.all(np.isfinite(V))
+    assert abs(dft.exchange_correlation_energy() - exc) <= TOL_E
+    assert np.abs(V - K).max() <= TOL_V
+    assert abs(ga - na) <= TOL_N and gb == 0.0 and nb == 0.0

@@ -185,3 +185,20 @@ def test_gpu_polarised_header_matches_oracle(pol_shim, fid, rng):
             for k in range(3):
                 assert out[3 + k] == pytest.approx(os_[0, k], rel=1e-9,
                                                    abs=1e-12 * abs(os_[0]).max())
+
+
+@pytest.mark.parametrize("fid", IDS)
+def test_gpu_polarised_header_one_spin_empty(pol_shim, fid, rng):
+    """rho_b = 0 exactly (no beta electrons): finite, and equal to the oracle."""
+    import ctypes
+    out = (ctypes.c_double * 6)()
+    for _ in range(100):
+        ra = 10 ** rng.uniform(-6, 2)
+        ga = rng.normal(size=3) * ra ** (4 / 3)
+        x = np.array([ra, 0.0, ga @ ga if fid > 100 else 0.0, 0.0, 0.0])
+        assert pol_shim.shim(fid, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out) == 0
+        assert np.all(np.isfinite(np.array(out[:6])))
+        oe, ov, os_ = oracle.functional_eval_pol(fid, x[None, :2], x[None, 2:])
+        assert out[0] == pytest.approx(oe[0] * ra, rel=1e-11)
+        assert out[1] == pytest.approx(ov[0, 0], rel=1e-9)
+        assert out[2] == pytest.approx(ov[0, 1], rel=1e-8, abs=1e-12 * abs(ov[0, 0]))
