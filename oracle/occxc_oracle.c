@@ -21,6 +21,10 @@
  *    central differences.  libxc itself (6.2.2) is absent: VWN5, PW92, PBE-c,
  *    B88 and LYP energy expressions are pinned only by the papers' closed
  *    forms and limits ("parity unpinned" at the libxc boundary).
+ *  - meta-GGA (TPSS exchange and correlation): the reference's golden exc,
+ *    vrho, vsigma, vtau at four points (tests/dft_tests.cpp:138-190),
+ *    complex-step vs finite differences, uniform-gas limits
+ *    (tests/test_oracle_mgga.py).
  *  - XC build: screened == dense (tests/dft_tests.cpp:904-931 restated),
  *    N_elec = tr(2 D S) with analytic overlaps, brute-force tiny cases, and
  *    the directional-derivative identity dExc/dD = Vxc.
@@ -531,6 +535,9 @@ int oracle_functional_family(int id) {
   case ORACLE_GGA_C_PBE:
   case ORACLE_GGA_C_LYP:
     return 1;
+  case ORACLE_MGGA_X_TPSS:
+  case ORACLE_MGGA_C_TPSS:
+    return 2;
   default:
     return -1;
   }
@@ -568,7 +575,7 @@ int oracle_functional_eval(int id, long n, const double *rho,
                            const double *sigma, double *exc, double *vrho,
                            double *vsigma) {
   int fam = oracle_functional_family(id);
-  if (fam < 0) return -1;
+  if (fam < 0 || fam > 1) return -1;
   for (long i = 0; i < n; i++) {
     double r = rho[i];
     if (!(r >= ORACLE_DENS_THR)) continue;
@@ -721,7 +728,7 @@ double oracle_functional_energy_density_pol(int id, double ra, double rb, double
 int oracle_functional_eval_pol(int id, long n, const double *rho, const double *sigma,
                                double *exc, double *vrho, double *vsigma) {
   int fam = oracle_functional_family(id);
-  if (fam < 0) return -1;
+  if (fam < 0 || fam > 1) return -1;
   for (long i = 0; i < n; i++) {
     double x[5] = {rho[2 * i], rho[2 * i + 1], 0.0, 0.0, 0.0};
     double r = x[0] + x[1];
@@ -740,6 +747,102 @@ int oracle_functional_eval_pol(int id, long n, const double *rho, const double *
       double dv = cimag(edens_pol(id, y[0], y[1], y[2], y[3], y[4])) / h;
       if (k < 2) vrho[2 * i + k] += dv;
       else vsigma[3 * i + k - 2] += dv;
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------ meta-GGA functionals -- */
+/* TPSS: J. Tao, J.P. Perdew, V.N. Staroverov, G.E. Scuseria, PRL 91, 146401
+ * (2003).  Unpolarised energy densities e(rho, sigma, tau) with total
+ * quantities, tau = (1/2) sum_i |grad psi_i|^2 (density.cpp:98-100), in
+ * complex arithmetic for complex-step derivatives.  Inputs are limited as
+ * libxc limits meta-GGA inputs: tau >= 1e-20 and sigma <= 8 rho tau (the
+ * Fermi-hole curvature bound tau_W <= tau); derivatives are taken at the
+ * limited inputs. */
+
+#define ORACLE_TAU_MIN 1e-20
+
+/* exchange, eqs. 5-10: F_x(p, z) with p = s^2, z = tau_W / tau,
+ * alpha = (tau - tau_W) / tau_unif */
+static cplx e_tpss_x(cplx rho, cplx sigma, cplx tau) {
+  const double b = 0.40, c = 1.59096, e = 1.537, kappa = 0.804, mu = 0.21951;
+  const double mge = 10.0 / 81.0;
+  const double k2 = pow(3.0 * M_PI * M_PI, 2.0 / 3.0);
+  cplx p = sigma / (4.0 * k2 * cpow(rho, 8.0 / 3.0));
+  cplx z = sigma / (8.0 * rho * tau);
+  cplx alpha = (tau - sigma / (8.0 * rho)) / (0.3 * k2 * cpow(rho, 5.0 / 3.0));
+  cplx qb = 0.45 * (alpha - 1.0) / csqrt(1.0 + b * alpha * (alpha - 1.0)) + 2.0 * p / 3.0;
+  cplx z2 = z * z;
+  cplx num = (mge + c * z2 / ((1.0 + z2) * (1.0 + z2))) * p + 146.0 / 2025.0 * qb * qb -
+             73.0 / 405.0 * qb * csqrt(0.5 * (9.0 / 25.0 * z2 + p * p)) +
+             mge * mge / kappa * p * p + 2.0 * sqrt(e) * mge * 9.0 / 25.0 * z2 +
+             e * mu * p * p * p;
+  cplx den = (1.0 + sqrt(e) * p) * (1.0 + sqrt(e) * p);
+  cplx x = num / den;
+  cplx F = 1.0 + kappa - kappa / (1.0 + x / kappa);
+  return e_lda_x(rho) * F;
+}
+
+/* correlation, eqs. 11-14 at zeta = 0 (so xi = 0 and C(0, 0) = 0.53):
+ * eps_revPKZB = eps_PBE [1 + C z^2] - (1 + C) z^2 max(eps_PBE(n/2, 0), eps_PBE),
+ * eps = eps_revPKZB [1 + d eps_revPKZB z^3], z = min(tau_W / tau, 1) */
+static cplx e_tpss_c(cplx rho, cplx sigma, cplx tau) {
+  const double d = 2.8, C0 = 0.53;
+  cplx epbe = e_pbe_c_pol(0.5 * rho, 0.5 * rho, sigma) / rho;
+  cplx epol = e_pbe_c_pol(0.5 * rho, 0.0, 0.25 * sigma) / (0.5 * rho);
+  cplx et = creal(epol) > creal(epbe) ? epol : epbe;
+  cplx z = sigma / (8.0 * rho * tau);
+  if (creal(z) > 1.0) z = 1.0;
+  cplx z2 = z * z;
+  cplx f0 = (1.0 + C0 * z2) * epbe - (1.0 + C0) * z2 * et;
+  return rho * f0 * (1.0 + d * f0 * z2 * z);
+}
+
+static cplx edens_mgga(int id, cplx rho, cplx sigma, cplx tau) {
+  switch (id) {
+  case ORACLE_MGGA_X_TPSS:
+    return e_tpss_x(rho, sigma, tau);
+  case ORACLE_MGGA_C_TPSS:
+    return e_tpss_c(rho, sigma, tau);
+  }
+  return edens(id, rho, sigma);
+}
+
+/* input limits of the meta-GGA evaluation (see above) */
+static void mgga_inputs(double rho, double *sigma, double *tau) {
+  if (*sigma < ORACLE_SIGMA_MIN) *sigma = ORACLE_SIGMA_MIN;
+  if (*tau < ORACLE_TAU_MIN) *tau = ORACLE_TAU_MIN;
+  if (*sigma > 8.0 * rho * *tau) *sigma = 8.0 * rho * *tau;
+}
+
+double oracle_functional_energy_density_mgga(int id, double rho, double sigma, double tau) {
+  if (rho < ORACLE_DENS_THR) return 0.0;
+  mgga_inputs(rho, &sigma, &tau);
+  return creal(edens_mgga(id, rho, sigma, tau));
+}
+
+int oracle_functional_eval_mgga(int id, long n, const double *rho, const double *sigma,
+                                const double *tau, double *exc, double *vrho,
+                                double *vsigma, double *vtau) {
+  int fam = oracle_functional_family(id);
+  if (fam < 0) return -1;
+  for (long i = 0; i < n; i++) {
+    double r = rho[i];
+    if (!(r >= ORACLE_DENS_THR)) continue;
+    double s = fam >= 1 ? sigma[i] : 0.0, t = fam == 2 ? tau[i] : 1.0;
+    if (fam == 2) mgga_inputs(r, &s, &t);
+    else if (fam == 1 && s < ORACLE_SIGMA_MIN) s = ORACLE_SIGMA_MIN;
+    double hr = 1e-20 * r;
+    exc[i] += creal(edens_mgga(id, r, s, t)) / r;
+    vrho[i] += cimag(edens_mgga(id, r + I * hr, s, t)) / hr;
+    if (fam >= 1) {
+      double hs = 1e-20 * s;
+      vsigma[i] += cimag(edens_mgga(id, r, s + I * hs, t)) / hs;
+    }
+    if (fam == 2) {
+      double ht = 1e-20 * t;
+      vtau[i] += cimag(edens_mgga(id, r, s, t + I * ht)) / ht;
     }
   }
   return 0;
@@ -770,6 +873,8 @@ static void process_block(const block_cfg *cfg, int m, long n,
   double *rho = T + (long)m * n; /* [4][n] */
   double *exc = rho + 4 * n, *vrho = exc + n, *vsig = vrho + n, *sig = vsig + n;
   double *A = sig + n; /* [m][n] */
+  double *tau = A + (long)m * n, *vtau = tau + n; /* meta-GGA */
+  const int mgga = cfg->deriv == 2;
 
   /* T = phi * D  (Dphi in density.cpp:14) */
   memset(T, 0, sizeof(double) * m * n);
@@ -796,6 +901,27 @@ static void process_block(const block_cfg *cfg, int m, long n,
       }
     }
   }
+  /* density.cpp:94-101: tau = (1/2) sum_k rowsum((grad_k phi D2) o grad_k phi) */
+  if (mgga) {
+    memset(tau, 0, sizeof(double) * n);
+    for (int k = 1; k <= 3; k++) {
+      const double *pk = phi + (long)k * m * n;
+      memset(T, 0, sizeof(double) * m * n);
+      for (int nu = 0; nu < m; nu++) {
+        double *t = T + (long)nu * n;
+        for (int mu = 0; mu < m; mu++) {
+          double dm = Dloc[mu + (long)nu * m];
+          const double *f = pk + (long)mu * n;
+          for (long p = 0; p < n; p++) t[p] += dm * f[p];
+        }
+      }
+      for (int nu = 0; nu < m; nu++) {
+        const double *t = T + (long)nu * n, *f = pk + (long)nu * n;
+        for (long p = 0; p < n; p++) tau[p] += f[p] * t[p];
+      }
+    }
+    for (long p = 0; p < n; p++) tau[p] *= 0.5;
+  }
   /* dft_kernels.cpp:29: N_elec before the skip */
   double ne = 0.0, rmax = -1e300;
   for (long p = 0; p < n; p++) {
@@ -810,12 +936,19 @@ static void process_block(const block_cfg *cfg, int m, long n,
     sig[p] = rho[n + p] * rho[n + p] + rho[2 * n + p] * rho[2 * n + p] +
              rho[3 * n + p] * rho[3 * n + p];
   }
-  for (int f = 0; f < cfg->nfunc; f++)
-    oracle_functional_eval(cfg->ids[f], n, rho, sig, exc, vrho, vsig);
+  if (mgga) {
+    for (long p = 0; p < n; p++) vtau[p] = 0.0;
+    for (int f = 0; f < cfg->nfunc; f++)
+      oracle_functional_eval_mgga(cfg->ids[f], n, rho, sig, tau, exc, vrho, vsig, vtau);
+  } else {
+    for (int f = 0; f < cfg->nfunc; f++)
+      oracle_functional_eval(cfg->ids[f], n, rho, sig, exc, vrho, vsig);
+  }
   for (long p = 0; p < n; p++) {
     exc[p] *= wts[p];
     vrho[p] *= wts[p];
     vsig[p] *= wts[p];
+    if (mgga) vtau[p] *= wts[p];
   }
   double e = 0.0;
   for (long p = 0; p < n; p++) e += rho[p] * exc[p];
@@ -856,6 +989,22 @@ static void process_block(const block_cfg *cfg, int m, long n,
         for (long p = 0; p < n; p++) s += f[p] * a[p];
         Kblk[mu + (long)nu * m] += s;
         Kblk[nu + (long)mu * m] += s;
+      }
+    }
+    /* xc_potential_matrix<Restricted, 2> (xc_potential_matrix.cpp:44-58):
+     * K += sum_k grad_k phi^T (vtau / 2 o grad_k phi) */
+    if (mgga) {
+      for (int k = 1; k <= 3; k++) {
+        const double *pk = phi + (long)k * m * n;
+        for (int nu = 0; nu < m; nu++) {
+          const double *fn = pk + (long)nu * n;
+          for (int mu = 0; mu < m; mu++) {
+            const double *fm = pk + (long)mu * n;
+            double s = 0.0;
+            for (long p = 0; p < n; p++) s += fm[p] * (0.5 * vtau[p]) * fn[p];
+            Kblk[mu + (long)nu * m] += s;
+          }
+        }
       }
     }
   }
@@ -955,7 +1104,7 @@ int oracle_xc_build(const oracle_basis *b, long npts, const double *xyz,
     double *phi = malloc(sizeof(double) * 4 * (size_t)nbf * maxn);
     double *Dloc = malloc(sizeof(double) * (size_t)nbf * nbf);
     double *Kblk = malloc(sizeof(double) * (size_t)nbf * nbf);
-    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 9 * maxn));
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 12 * maxn));
 #pragma omp for schedule(dynamic, 4)
     for (long li = 0; li < nblk; li++) {
       long n = lcnt[li];
@@ -982,7 +1131,7 @@ int oracle_xc_build(const oracle_basis *b, long npts, const double *xyz,
       for (int j = 0; j < m; j++)
         for (int i = 0; i < m; i++)
           Dloc[i + (long)j * m] = D2[gidx[i] + (long)gidx[j] * nbf];
-      eval_shells(b, sig, nsig, n, pxyz + 3 * loff[li], deriv, phi, m);
+      eval_shells(b, sig, nsig, n, pxyz + 3 * loff[li], deriv ? 1 : 0, phi, m);
       memset(Kblk, 0, sizeof(double) * m * m);
       process_block(&cfg, m, n, phi, Dloc, pw + loff[li], Kblk, &energy, &nel, work);
       for (int j = 0; j < m; j++)
@@ -1173,7 +1322,7 @@ int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
   int deriv = 0;
   for (int f = 0; f < nfunc; f++) {
     int fam = oracle_functional_family(func_ids[f]);
-    if (fam < 0) return -1;
+    if (fam < 0 || fam > 1) return -1;
     if (fam > deriv) deriv = fam;
   }
   const int nsh = b->nshell;
@@ -1424,7 +1573,7 @@ int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double 
   int deriv = 0;
   for (int f = 0; f < nfunc; f++) {
     int fam = oracle_functional_family(func_ids[f]);
-    if (fam < 0) return -1;
+    if (fam < 0 || fam > 1) return -1;
     if (fam > deriv) deriv = fam;
   }
   const int nbf = oracle_nbf(b), nsh = b->nshell;
@@ -1682,7 +1831,7 @@ int oracle_xc_gradient_unrestricted(const oracle_basis *b, const int *shell_atom
   int deriv = 0;
   for (int f = 0; f < nfunc; f++) {
     int fam = oracle_functional_family(func_ids[f]);
-    if (fam < 0) return -1;
+    if (fam < 0 || fam > 1) return -1;
     if (fam > deriv) deriv = fam;
   }
   const int nsh = b->nshell;

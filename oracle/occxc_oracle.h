@@ -80,9 +80,11 @@ enum {
   ORACLE_GGA_X_PBE = 101,
   ORACLE_GGA_X_B88 = 106,
   ORACLE_GGA_C_PBE = 130,
-  ORACLE_GGA_C_LYP = 131
+  ORACLE_GGA_C_LYP = 131,
+  ORACLE_MGGA_X_TPSS = 202,
+  ORACLE_MGGA_C_TPSS = 231
 };
-/* 0 = LDA, 1 = GGA, -1 unknown id */
+/* 0 = LDA, 1 = GGA, 2 = meta-GGA, -1 unknown id */
 int oracle_functional_family(int id);
 /* DensityFunctional::evaluate (src/dft/functional.cpp:18-65) for an
  * unpolarised density: ACCUMULATES exc, vrho, vsigma (+=), like
@@ -93,6 +95,14 @@ int oracle_functional_eval(int id, long n, const double *rho,
                            double *vsigma);
 /* Energy density e = rho*eps at one point, exposed for tests. */
 double oracle_functional_energy_density(int id, double rho, double sigma);
+/* The same for meta-GGA densities (Family::MGGA, functional.cpp:49-57):
+ * tau[n] = (1/2) sum_i |grad psi_i|^2; ACCUMULATES exc, vrho, vsigma, vtau
+ * (vlaplacian is zero for every functional here).  LDA and GGA ids are
+ * accepted too (tau ignored). */
+int oracle_functional_eval_mgga(int id, long n, const double *rho, const double *sigma,
+                                const double *tau, double *exc, double *vrho,
+                                double *vsigma, double *vtau);
+double oracle_functional_energy_density_mgga(int id, double rho, double sigma, double tau);
 
 /* The same functionals for a spin-polarised density, libxc's layout:
  * rho[2n] = (rho_a, rho_b) per point, sigma[3n] = (s_aa, s_ab, s_bb);

@@ -68,6 +68,9 @@ def lib():
                                                    ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                    _dp, _dp, _dp]
         L.oracle_xc_gradient_unrestricted.argtypes = L.oracle_xc_gradient.argtypes
+        L.oracle_functional_eval_mgga.argtypes = [ctypes.c_int, ctypes.c_long] + [_dp] * 7
+        L.oracle_functional_energy_density_mgga.restype = ctypes.c_double
+        L.oracle_functional_energy_density_mgga.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3
         L.oracle_screened_work.restype = ctypes.c_long
         L.oracle_screened_work.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp,
                                            ctypes.c_double, ctypes.c_long, _lp, _lp]
@@ -202,6 +205,25 @@ def functional_eval(fid, rho, sigma=None):
 
 def energy_density(fid, rho, sigma=0.0):
     return lib().oracle_functional_energy_density(fid, float(rho), float(sigma))
+
+
+def functional_eval_mgga(fid, rho, sigma, tau):
+    """Meta-GGA evaluation (total rho, sigma, tau per point) -> exc, vrho,
+    vsigma, vtau."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    n = len(rho)
+    if lib().oracle_functional_family(fid) < 0:
+        raise ValueError(f"unknown functional id {fid}")
+    sig = np.ascontiguousarray(sigma, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    exc, vrho, vsig, vtau = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().oracle_functional_eval_mgga(fid, n, _d(rho), _d(sig), _d(tau), _d(exc), _d(vrho),
+                                      _d(vsig), _d(vtau))
+    return exc, vrho, vsig, vtau
+
+
+def energy_density_mgga(fid, rho, sigma, tau):
+    return lib().oracle_functional_energy_density_mgga(fid, float(rho), float(sigma), float(tau))
 
 
 def xc_build(basis, xyz, w, func_ids, D, density_threshold=1e-10,
