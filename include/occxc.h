@@ -58,7 +58,9 @@ enum {
   OCCXC_GGA_X_PBE = 101,
   OCCXC_GGA_X_B88 = 106,
   OCCXC_GGA_C_PBE = 130,
-  OCCXC_GGA_C_LYP = 131
+  OCCXC_GGA_C_LYP = 131,
+  OCCXC_MGGA_X_TPSS = 202,    /* meta-GGA: restricted builds only */
+  OCCXC_MGGA_C_TPSS = 231
 };
 
 typedef enum {
@@ -322,6 +324,16 @@ occxc_status occxc_functional_evaluate(int id, long long n, const double *rho,
                                        const double *sigma, double *exc,
                                        double *vrho, double *vsigma,
                                        int device);
+
+/* DensityFunctional::evaluate for Family::MGGA (src/dft/functional.cpp:
+ * 49-57), unpolarised: rho, sigma, tau = (1/2) sum_i |grad psi_i|^2 per point
+ * (density.cpp:94-101); ACCUMULATES exc, vrho, vsigma, vtau (vlaplacian is
+ * zero for every id here).  LDA and GGA ids are accepted (tau unused; sigma /
+ * vsigma may be NULL for LDA, tau / vtau for LDA and GGA). */
+occxc_status occxc_functional_evaluate_mgga(int id, long long n, const double *rho,
+                                            const double *sigma, const double *tau,
+                                            double *exc, double *vrho, double *vsigma,
+                                            double *vtau, int device);
 
 /* The same for a spin-polarised density (DensityFunctional with
  * polarized = true, functional.cpp:18-65), libxc's layout: rho[2n] =

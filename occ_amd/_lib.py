@@ -85,6 +85,8 @@ SIGNATURES = {
                                                        ctypes.c_void_p]),
     "occxc_functional_evaluate_polarized": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _dp,
                                                            _dp, _dp, _dp, _dp, ctypes.c_int]),
+    "occxc_functional_evaluate_mgga": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _dp, _dp,
+                                                      _dp, _dp, _dp, _dp, _dp, ctypes.c_int]),
     "occxc_xc_gradient_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                 _ip, ctypes.c_void_p, ctypes.c_void_p]),
     "occxc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CBasis),

@@ -28,7 +28,7 @@
 #include "context.hpp"
 #include "contract.hpp"
 #include "kernels.hpp"
-#include "xc_functionals.hpp"
+#include "xc_functionals_mgga.hpp"
 
 namespace occxc {
 
@@ -377,15 +377,21 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                 OCCXC_ERR_INVALID_ARGUMENT, "functional: need 1..8 functional ids");
   int deriv = 0;
   for (int f = 0; f < func->nfunc; ++f) {
-    const int fam = xc_family(func->ids[f]);
+    const int fam = xc_family_mgga(func->ids[f]);
     OCCXC_REQUIRE(fam >= 0, OCCXC_ERR_UNSUPPORTED,
                   "functional id " + std::to_string(func->ids[f]) + " not found");
     deriv = std::max(deriv, fam);
     c->funcs[f] = func->ids[f];
   }
   c->nfunc = func->nfunc;
-  c->deriv = deriv;
-  c->ncomp = deriv ? 4 : 1;
+  // a meta-GGA needs phi and grad phi in the AO cache, like a GGA (tau is
+  // formed from grad phi; no laplacian is formed)
+  c->mgga = deriv == 2;
+  c->deriv = std::min(deriv, 1);
+  c->ncomp = c->deriv ? 4 : 1;
+  OCCXC_REQUIRE(!c->mgga || opt->precision == OCCXC_PREC_FP64 ||
+                    opt->precision == OCCXC_PREC_FP32_CACHE,
+                OCCXC_ERR_UNSUPPORTED, "meta-GGA builds need an FP64 or FP32-cache context");
 
   // molecules: bases concatenated (shell and primitive offsets shifted)
   std::vector<HostBasis> hbs;
@@ -601,6 +607,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                            4.0 * (L.mp + L.nsig) + sizeof(LeafDev) +
                            (two_pass ? 8.0 * L.mp * L.mp : 0.0);
   const size_t planned = ao_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
+                         (c->mgga ? (size_t)c->ptot * 8 * 8 : 0) +
                          c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                          (two_pass ? (size_t)c->kloc_elems * 8 : 0);
   c->device_bytes = planned;
@@ -722,6 +729,29 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                     c->plan.bytes + c->d_kloc.bytes() + c->d_tpairs.bytes() +
                     c->d_tentries.bytes() + c->d_tchunks.bytes() + c->d_tpartial.bytes();
 
+  if (c->mgga) {
+    OCCXC_REQUIRE(c->plan.use_mfma && !c->plan.two_pass && c->plan.max_mp <= kStoreMaxMp,
+                  OCCXC_ERR_UNSUPPORTED,
+                  "meta-GGA builds need the matrix-core kernels, the one-pass Vxc and leaves of "
+                  "at most " + std::to_string(kStoreMaxMp) + " AOs");
+    const size_t P = (size_t)std::max<long long>(c->ptot, 1);
+    c->d_mrho.alloc(4 * P);
+    c->d_mtau.alloc(3 * P);
+    c->d_coef_tau.alloc(P);
+    // leaf tables whose AO block offsets point at the d/dx, d/dy, d/dz planes
+    std::vector<LeafDev> lk(3 * c->leaves.size());
+    for (int k = 0; k < 3; ++k)
+      for (size_t i = 0; i < c->leaves.size(); ++i) {
+        LeafDev L = c->leaves[i];
+        L.ao_off += (long long)(k + 1) * L.mp * L.np;
+        lk[(size_t)k * c->leaves.size() + i] = L;
+      }
+    c->d_leaves_k.alloc(std::max<size_t>(lk.size(), 1));
+    c->d_leaves_k.upload(lk.data(), lk.size(), c->stream);
+    c->device_bytes += c->d_mrho.bytes() + c->d_mtau.bytes() + c->d_coef_tau.bytes() +
+                       c->d_leaves_k.bytes();
+  }
+
   launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
   OCCXC_HIP(hipStreamSynchronize(c->stream));
   c->ao_ready = true;
@@ -764,8 +794,63 @@ void collect_timing(occxc_context *c) {
   c->pending.clear();
 }
 
+// Restricted meta-GGA build on stream s: compute_K_dft<2, Restricted>
+// (dft.h:190-388) with evaluate_density<2, R> (density.cpp:75-102) and
+// xc_potential_matrix<Restricted, 2> (xc_potential_matrix.cpp:44-58).  rho
+// and grad rho come from the density kernel in store mode; each direction's
+// rowsum((grad_k phi D2) o grad_k phi) from the same kernel run on that
+// derivative plane of the AO cache in place of phi (a.deriv = 0); then the
+// per-leaf meta-GGA functional stage.  The GGA part of Vxc is the restricted
+// Vxc kernel; the tau part is the same kernel run on each derivative plane in
+// its LDA form with the tau coefficients.  The laplacian (density.cpp:88-93)
+// is not formed: no functional here depends on it and the reference's Vxc
+// does not use vlaplacian.
+static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, double *d_result,
+                           hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const int prec = c->opt.precision;
+  StageTimer t{c, s, {}};
+  auto plane = [&](int k) {
+    BuildArgs y = a;
+    y.deriv = 0;
+    y.leaves = c->d_leaves_k.get() + (size_t)k * c->nleaves;
+    return y;
+  };
+  t.run(OCCXC_STAGE_PREPARE, [&] {
+    launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
+    launch_zero(c->d_K.get(), c->mtot, s);
+  });
+  t.run(OCCXC_STAGE_DENSITY, [&] {
+    BuildArgs x = a;
+    x.rho_out = c->d_mrho.get();
+    run_density_store(x, c->d_ao.get(), prec, c->d_D2.get(), s);
+    for (int k = 0; k < 3; ++k) {
+      BuildArgs y = plane(k);
+      y.rho_out = c->d_mtau.get() + (size_t)k * c->ptot;
+      run_density_store(y, c->d_ao.get(), prec, c->d_D2.get(), s);
+    }
+    launch_functional_mgga_leaves(a, c->d_mrho.get(), c->d_mtau.get(), c->d_coef_tau.get(), s);
+  });
+  t.run(OCCXC_STAGE_VXC, [&] {
+    run_vxc(c->plan, a, c->d_ao.get(), prec, s);
+    for (int k = 0; k < 3; ++k) {
+      BuildArgs y = plane(k);
+      y.coef = c->d_coef_tau.get();
+      run_vxc(c->plan, y, c->d_ao.get(), prec, s);
+    }
+  });
+  t.run(OCCXC_STAGE_FINALIZE, [&] {
+    launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
+                    d_result, s);
+  });
+}
+
 void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_result,
                hipStream_t s) {
+  if (c->mgga) {
+    run_build_mgga(c, d_D, d_V, d_result, s);
+    return;
+  }
   const BuildArgs &a = c->args;
   StageTimer t{c, s, {}};
   t.run(OCCXC_STAGE_PREPARE, [&] {
@@ -1149,6 +1234,8 @@ occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
                                const int *shell_atom, double *grad) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
+                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
@@ -1173,6 +1260,8 @@ occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int
   return guarded([&] {
     OCCXC_REQUIRE(ctx && d_D && shell_atom && d_grad, OCCXC_ERR_INVALID_ARGUMENT,
                   "null pointer");
+    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
+                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
@@ -1190,6 +1279,8 @@ occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, doubl
                                       double *exc, double *nelec) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && Vxc && exc && nelec, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
+                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_build_unrestricted needs a single-molecule context");
     OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
@@ -1224,6 +1315,8 @@ occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d
                                              double *d_Vxc, double *d_result, void *stream) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
+                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_build_unrestricted needs a single-molecule context");
     OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
@@ -1254,6 +1347,8 @@ occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D,
                                             const int *shell_atom, double *grad) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
+                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
@@ -1417,6 +1512,48 @@ occxc_status occxc_functional_evaluate(int id, long long n, const double *rho,
       exc[i] += e[i];
       vrho[i] += v[i];
       if (fam) vsigma[i] += w[i];
+    }
+  });
+}
+
+occxc_status occxc_functional_evaluate_mgga(int id, long long n, const double *rho,
+                                            const double *sigma, const double *tau,
+                                            double *exc, double *vrho, double *vsigma,
+                                            double *vtau, int device) {
+  return guarded([&] {
+    const int fam = xc_family_mgga(id);
+    OCCXC_REQUIRE(fam >= 0, OCCXC_ERR_UNSUPPORTED, "functional id " + std::to_string(id) + " not found");
+    OCCXC_REQUIRE(n >= 0 && (n == 0 || (rho && exc && vrho)), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_mgga: null pointer");
+    OCCXC_REQUIRE(fam == 0 || n == 0 || (sigma && vsigma), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_mgga: GGA and meta-GGA need sigma and vsigma");
+    OCCXC_REQUIRE(fam < 2 || n == 0 || (tau && vtau), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_mgga: meta-GGA needs tau and vtau");
+    if (n == 0) return;
+    DeviceGuard g(device);
+    const bool gga = fam >= 1, mg = fam == 2;
+    DevBuf<double> dr(n), ds(gga ? n : 1), dt(mg ? n : 1), de(n), dv(n), dw(gga ? n : 1),
+        dx(mg ? n : 1);
+    dr.upload(rho, n);
+    if (gga) ds.upload(sigma, n);
+    if (mg) dt.upload(tau, n);
+    launch_zero(de.get(), n, nullptr);
+    launch_zero(dv.get(), n, nullptr);
+    launch_zero(dw.get(), gga ? n : 1, nullptr);
+    launch_zero(dx.get(), mg ? n : 1, nullptr);
+    launch_functional_mgga(id, n, dr.get(), ds.get(), dt.get(), de.get(), dv.get(), dw.get(),
+                           dx.get(), nullptr);
+    std::vector<double> e(n), v(n), w(gga ? n : 0), x(mg ? n : 0);
+    de.download(e.data(), n);
+    dv.download(v.data(), n);
+    if (gga) dw.download(w.data(), n);
+    if (mg) dx.download(x.data(), n);
+    OCCXC_HIP(hipDeviceSynchronize());
+    for (long long i = 0; i < n; ++i) {
+      exc[i] += e[i];
+      vrho[i] += v[i];
+      if (gga) vsigma[i] += w[i];
+      if (mg) vtau[i] += x[i];
     }
   });
 }

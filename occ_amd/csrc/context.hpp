@@ -75,6 +75,13 @@ struct occxc_context {
   occxc::DevBuf<double> d_Du, d_D2a, d_D2b, d_rho_a, d_rho_b, d_coef_b, d_K_b, d_V_b, d_part_u,
       d_result_u;
   bool uks_ready = false;
+  // restricted meta-GGA (allocated at context creation): rho and grad rho
+  // ([4][ptot]), the per-direction tau passes ([3][ptot]), the tau
+  // coefficients, and three copies of the leaf table whose AO block offsets
+  // point at the d/dx, d/dy, d/dz planes of the AO cache
+  bool mgga = false;
+  occxc::DevBuf<double> d_mrho, d_mtau, d_coef_tau;
+  occxc::DevBuf<occxc::LeafDev> d_leaves_k;
 };
 
 namespace occxc {

@@ -292,7 +292,8 @@ constexpr int kDefaultDensityVariant = 2;
 
 // STORE: write rho and grad rho of every (padded) point to a.rho_out
 // instead of running the functional epilogue (the unrestricted build
-// evaluates each spin density this way).
+// evaluates each spin density this way; with a.deriv = 0 only rho is
+// written, one plane).
 template <typename S, typename T, bool STORE = false>
 __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *__restrict__ ao,
                                                       const T *__restrict__ D2t, int min_mp) {
@@ -405,9 +406,9 @@ __global__ __launch_bounds__(256, 2) void k_density_wave(BuildArgs a, const S *_
   }
   __syncthreads();
   if constexpr (STORE) {
+    const int nq = deriv ? 4 : 1;
     for (int p = threadIdx.x; p < L.np; p += blockDim.x)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) a.rho_out[q * a.ptot + L.pt_off + p] = s_q[q][p];
+      for (int q = 0; q < nq; ++q) a.rho_out[q * a.ptot + L.pt_off + p] = s_q[q][p];
   } else {
     density_epilogue(a, L, leaf, s_q, s_red);
   }
@@ -670,7 +671,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_mfma(BuildArgs a, const S *__res
   const int chw = L.np <= 128 ? CH : CH / 2;
   const int ld = L.np + 16 / (int)sizeof(T);
   for (int i = threadIdx.x; i < L.np; i += blockDim.x)
-    for (int k = 0; k < 4; ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+    for (int k = 0; k < (deriv ? 4 : 1); ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
   __syncthreads();
   const int ntile = L.mp >> 4;
   for (int c = 0; c < L.mp; c += chw) {
@@ -802,7 +803,7 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
   const long long cs = (long long)L.mp * L.np;
   const bool deriv = a.deriv != 0;
   for (int i = t; i < L.np; i += blockDim.x)
-    for (int k = 0; k < 4; ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+    for (int k = 0; k < (deriv ? 4 : 1); ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
   const int nt = L.mp >> 4;
   const int npair = nt * (nt + 1) / 2;
   acc_t acc[kSymPerWave];
@@ -1052,7 +1053,7 @@ __global__ __launch_bounds__(256) void k_vxc_bf16(BuildArgs a, const float *__re
   const int ld = L.np + PADP;
   const int plane = chw * ld;
   for (int i = threadIdx.x; i < L.np; i += blockDim.x)
-    for (int k = 0; k < 4; ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+    for (int k = 0; k < (deriv ? 4 : 1); ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
   __syncthreads();
   const int ntile = L.mp >> 4;
   for (int c = 0; c < L.mp; c += chw) {

@@ -27,6 +27,7 @@
 #include "kernels.hpp"
 #include "xc_functionals.hpp"
 #include "xc_functionals_pol.hpp"
+#include "xc_functionals_mgga.hpp"
 
 namespace occxc {
 
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(256) void k_vxc(BuildArgs a, const T *__restrict__ 
   const MolDev M = a.mols[L.mol];
   for (int i = threadIdx.x; i < L.m; i += blockDim.x) s_map[i] = a.ao_map[L.map_off + i];
   for (int i = threadIdx.x; i < L.np; i += blockDim.x)
-    for (int c = 0; c < 4; ++c) s_c[c][i] = a.coef[c * a.ptot + L.pt_off + i];
+    for (int c = 0; c < (a.deriv ? 4 : 1); ++c) s_c[c][i] = a.coef[c * a.ptot + L.pt_off + i];
   __syncthreads();
   const T *phi = ao + L.ao_off;
   const long long cstride = (long long)L.mp * L.np;
@@ -549,6 +550,84 @@ __global__ __launch_bounds__(256) void k_functional_pol(BuildArgs a,
     part[3 * leaf + 2] = nb;
     a.leaf_active[leaf] = active ? 1 : 0;
   }
+}
+
+// Restricted meta-GGA per-leaf functional stage (one thread per point,
+// blockDim = 256), process_grid_block<2, Restricted> (dft_kernels.cpp:8-70)
+// after the density store passes: rho and grad rho from rho4 ([4][ptot]),
+// tau = (1/2) sum_k tk[k] from the three per-direction passes (tk [3][ptot],
+// rowsum((grad_k phi D2) o grad_k phi), density.cpp:94-101); N_elec before
+// the skip, the max-rho skip, the functionals, weight_by, the energy sum, and
+// the Vxc coefficients of xc_potential_matrix<Restricted, 2>
+// (xc_potential_matrix.cpp:19-58): c0 = w vrho / 2, c_k = 2 w vsigma d_k rho
+// into a.coef and c_tau = w vtau / 4 into coef_tau, so that
+// K_tau = sum_k grad_k phi^T (c_tau grad_k phi) and V = K + K^T carry the
+// reference's (1/2) sum_k grad_k phi^T (w vtau o grad_k phi).
+__global__ __launch_bounds__(256) void k_functional_mgga(BuildArgs a,
+                                                         const double *__restrict__ rho4,
+                                                         const double *__restrict__ tk,
+                                                         double *__restrict__ coef_tau) {
+  __shared__ double s_red[4];
+  const int leaf = blockIdx.x;
+  const LeafDev L = a.leaves[leaf];
+  const int p = threadIdx.x;
+  const bool live = p < L.np, real = p < L.n;
+  const long long q = L.pt_off + p, P = a.ptot;
+  double rho = 0, g[3] = {0, 0, 0}, tau = 0, w = 0;
+  if (live) {
+    rho = rho4[q];
+    for (int k = 0; k < 3; ++k) g[k] = rho4[(k + 1) * P + q];
+    tau = 0.5 * (tk[q] + tk[P + q] + tk[2 * P + q]);
+    w = a.pw[q];
+  }
+  const double nel = block_sum_256(real ? rho * w : 0.0, s_red);
+  const double rmax = block_max_256(real ? rho : -1e300, s_red);
+  const bool active = !(rmax < a.dthr);
+  double e_sum = 0.0, vr_sum = 0.0, vs_sum = 0.0, vt_sum = 0.0;
+  if (active && live && rho >= kDensThreshold) {
+    const double sg = fmax(g[0] * g[0] + g[1] * g[1] + g[2] * g[2], kSigmaMin);
+    for (int f = 0; f < a.nfunc; ++f) {
+      double e, vr, vs, vt;
+      xc_point_mgga(a.func_ids[f], rho, sg, tau, e, vr, vs, vt);
+      e_sum += e;
+      vr_sum += vr;
+      vs_sum += vs;
+      vt_sum += vt;
+    }
+  }
+  const double exc = block_sum_256(active && real ? w * e_sum : 0.0, s_red);
+  if (live) {
+    a.coef[q] = active ? 0.5 * w * vr_sum : 0.0;
+    for (int k = 0; k < 3; ++k) a.coef[(k + 1) * P + q] = active ? 2.0 * w * vs_sum * g[k] : 0.0;
+    coef_tau[q] = active ? 0.25 * w * vt_sum : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    a.leaf_part[2 * leaf] = active ? exc : 0.0;
+    a.leaf_part[2 * leaf + 1] = nel;
+    a.leaf_active[leaf] = active ? 1 : 0;
+  }
+}
+
+// Meta-GGA (and LDA / GGA) functionals over points, ADDING to the outputs.
+__global__ void k_functional_mgga_points(int id, long long n, const double *__restrict__ rho,
+                                         const double *__restrict__ sigma,
+                                         const double *__restrict__ tau,
+                                         double *__restrict__ exc, double *__restrict__ vrho,
+                                         double *__restrict__ vsigma,
+                                         double *__restrict__ vtau) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int fam = xc_family_mgga(id);
+  const double r = rho[i];
+  if (!(r >= kDensThreshold)) return;
+  const double sg = fam >= 1 ? fmax(sigma[i], kSigmaMin) : 0.0;
+  const double t = fam == 2 ? tau[i] : 0.0;
+  double e, vr, vs, vt;
+  xc_point_mgga(id, r, sg, t, e, vr, vs, vt);
+  exc[i] += e / r;
+  vrho[i] += vr;
+  if (fam >= 1) vsigma[i] += vs;
+  if (fam == 2) vtau[i] += vt;
 }
 
 // {Exc, N_alpha, N_beta}: fixed-order sums of the per-leaf partials
@@ -878,6 +957,22 @@ void launch_functional_pol_leaves(const BuildArgs &a, const double *rho_a, const
                                   double *coef_b, double *part, hipStream_t s) {
   if (a.nleaves == 0) return;
   k_functional_pol<<<a.nleaves, 256, 0, s>>>(a, rho_a, rho_b, coef_b, part);
+  check_launch();
+}
+
+void launch_functional_mgga_leaves(const BuildArgs &a, const double *rho4, const double *tk,
+                                   double *coef_tau, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  k_functional_mgga<<<a.nleaves, 256, 0, s>>>(a, rho4, tk, coef_tau);
+  check_launch();
+}
+
+void launch_functional_mgga(int id, long long n, const double *rho, const double *sigma,
+                            const double *tau, double *exc, double *vrho, double *vsigma,
+                            double *vtau, hipStream_t s) {
+  if (n == 0) return;
+  k_functional_mgga_points<<<grid_for(n, 256), 256, 0, s>>>(id, n, rho, sigma, tau, exc, vrho,
+                                                            vsigma, vtau);
   check_launch();
 }
 

@@ -114,7 +114,7 @@ struct BuildArgs {
   const double *px, *py, *pz, *pw;
   const double *D2;     // doubled, symmetrised densities (concatenated nbf x nbf)
   double *K;            // accumulated phi^T A (concatenated nbf x nbf)
-  double *coef;         // [4][ptot] per-point Vxc coefficients
+  double *coef;         // per-point Vxc coefficients: [4][ptot] (deriv 1), [ptot] (deriv 0)
   double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
   int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
   double *kloc;         // two-pass Vxc: per-leaf K_loc blocks (null: atomics into K)
@@ -181,6 +181,14 @@ void launch_finalize_u(const double *Ka, const double *Kb, double *Va, double *V
                        double *result, hipStream_t s);
 void launch_functional_pol(int id, long long n, const double *rho, const double *sigma,
                            double *exc, double *vrho, double *vsigma, hipStream_t s);
+// restricted meta-GGA: the per-leaf functional stage from the stored rho /
+// grad rho (rho4, [4][ptot]) and per-direction tau passes (tk, [3][ptot]);
+// coefficients into a.coef and coef_tau, {Exc, N_elec} into a.leaf_part
+void launch_functional_mgga_leaves(const BuildArgs &a, const double *rho4, const double *tk,
+                                   double *coef_tau, hipStream_t s);
+void launch_functional_mgga(int id, long long n, const double *rho, const double *sigma,
+                            const double *tau, double *exc, double *vrho, double *vsigma,
+                            double *vtau, hipStream_t s);
 void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol,
                      int maxnbf, const double *leaf_part, double *result, hipStream_t s);
 // {Exc, N_elec} per molecule only (the two-pass Vxc writes Vxc itself)

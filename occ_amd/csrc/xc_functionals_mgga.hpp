@@ -1,0 +1,92 @@
+// xc_functionals_mgga.hpp -- TPSS meta-GGA exchange and correlation in FP64
+// for the restricted build, evaluated per grid point on the GPU.
+//
+// Replaces libxc 6.2.2's unpolarised xc_mgga_exc_vxc for mgga_x_tpss and
+// mgga_c_tpss as called from DensityFunctional::evaluate with Family::MGGA
+// (src/dft/functional.cpp:49-57): inputs (rho, sigma, tau) of the closed-shell
+// density, tau = (1/2) sum_i |grad psi_i|^2 (src/gto/density.cpp:94-101);
+// outputs e = rho*eps and vrho, vsigma, vtau (vlaplacian is zero for both
+// functionals and the reference's Vxc does not use it).  Each energy density
+// is written once over the forward-mode duals of xc_functionals_pol.hpp with
+// the partials with respect to (rho, sigma, tau) in slots 0..2; TPSS
+// correlation uses that header's spin-resolved PBE correlation.
+//
+// Formulas: J. Tao, J.P. Perdew, V.N. Staroverov, G.E. Scuseria, PRL 91,
+// 146401 (2003), eqs. 5-10 (exchange) and 11-14 at zeta = 0 (correlation:
+// xi = 0, C(0, 0) = 0.53, d = 2.8).  Inputs are limited as libxc limits
+// meta-GGA inputs (tau >= 1e-20, sigma <= 8 rho tau, the Fermi-hole
+// curvature bound tau_W <= tau), derivatives taken at the limited inputs;
+// the correlation's tau_W / tau is held at 1 above 1.
+#pragma once
+
+#include "xc_functionals_pol.hpp"
+
+namespace occxc {
+
+constexpr double kTauMin = 1e-20;
+
+enum : int { XC_MGGA_X_TPSS = 202, XC_MGGA_C_TPSS = 231 };
+
+// 0 LDA, 1 GGA, 2 meta-GGA, -1 unknown
+OCCXC_HD inline int xc_family_mgga(int id) {
+  if (id == XC_MGGA_X_TPSS || id == XC_MGGA_C_TPSS) return 2;
+  return xc_family(id);
+}
+
+// exchange: F_x(p, z) with p = s^2, z = tau_W / tau and
+// alpha = (tau - tau_W) / tau_unif
+OCCXC_HD inline D5 ex_tpss(const D5 &rho, const D5 &sigma, const D5 &tau) {
+  const double b = 0.40, c = 1.59096, e = 1.537, kappa = 0.804, mu = 0.21951;
+  const double mge = 10.0 / 81.0, se = sqrt(e);
+  const double k2 = pow(3.0 * kPi * kPi, 2.0 / 3.0);
+  const D5 p = sigma / (4.0 * k2 * d5_pow(rho, 8.0 / 3.0));
+  const D5 z = sigma / (8.0 * rho * tau);
+  const D5 alpha = (tau - sigma / (8.0 * rho)) / (0.3 * k2 * d5_pow(rho, 5.0 / 3.0));
+  const D5 qb = 0.45 * (alpha - 1.0) / d5_sqrt(1.0 + b * alpha * (alpha - 1.0)) + 2.0 * p / 3.0;
+  const D5 z2 = z * z;
+  const D5 opz2 = 1.0 + z2;
+  const D5 num = (mge + c * z2 / (opz2 * opz2)) * p + (146.0 / 2025.0) * qb * qb -
+                 (73.0 / 405.0) * qb * d5_sqrt(0.5 * ((9.0 / 25.0) * z2 + p * p)) +
+                 (mge * mge / kappa) * p * p + (2.0 * se * mge * 9.0 / 25.0) * z2 +
+                 (e * mu) * p * p * p;
+  const D5 den = (1.0 + se * p) * (1.0 + se * p);
+  const D5 x = num / den;
+  const D5 F = 1.0 + kappa - kappa / (1.0 + x / kappa);
+  return ex_dirac(rho) * F;
+}
+
+// correlation at zeta = 0: eps_revPKZB = eps_PBE (1 + C z^2)
+// - (1 + C) z^2 max(eps_PBE(n/2, 0), eps_PBE), eps = eps_revPKZB (1 + d eps_revPKZB z^3)
+OCCXC_HD inline D5 ec_tpss(const D5 &rho, const D5 &sigma, const D5 &tau) {
+  const double d = 2.8, C0 = 0.53;
+  const D5 half = 0.5 * rho;
+  const D5 epbe = ec_pbe_pol(half, half, sigma) / rho;
+  const D5 epol = ec_pbe_pol(half, d5c(0.0), 0.25 * sigma) / half;
+  const D5 et = epol.v > epbe.v ? epol : epbe;
+  D5 z = sigma / (8.0 * rho * tau);
+  if (z.v > 1.0) z = d5c(1.0);
+  const D5 z2 = z * z;
+  const D5 f0 = (1.0 + C0 * z2) * epbe - (1.0 + C0) * z2 * et;
+  return rho * f0 * (1.0 + d * f0 * z2 * z);
+}
+
+// One functional at one point of a meta-GGA build: e = rho*eps, vr, vs, vt
+// (SET, like xc_point).  LDA and GGA ids go to xc_point (tau unused, vt = 0).
+// Returns false for an unknown id.  The caller applies the density
+// threshold and sigma >= kSigmaMin.
+OCCXC_HD inline bool xc_point_mgga(int id, double rho, double sigma, double tau, double &e,
+                                   double &vr, double &vs, double &vt) {
+  vt = 0.0;
+  if (id != XC_MGGA_X_TPSS && id != XC_MGGA_C_TPSS) return xc_point(id, rho, sigma, e, vr, vs);
+  tau = fmax(tau, kTauMin);
+  sigma = fmin(sigma, 8.0 * rho * tau);
+  const D5 r = d5v(rho, 0), s = d5v(sigma, 1), t = d5v(tau, 2);
+  const D5 out = id == XC_MGGA_X_TPSS ? ex_tpss(r, s, t) : ec_tpss(r, s, t);
+  e = out.v;
+  vr = out.d[0];
+  vs = out.d[1];
+  vt = out.d[2];
+  return true;
+}
+
+} // namespace occxc

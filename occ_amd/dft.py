@@ -15,14 +15,16 @@ import numpy as np
 from . import _lib
 
 LIBXC_IDS = {"lda_x": 1, "lda_c_vwn": 7, "lda_c_pw": 12, "lda_c_pw_mod": 13,
-             "gga_x_pbe": 101, "gga_x_b88": 106, "gga_c_pbe": 130, "gga_c_lyp": 131}
+             "gga_x_pbe": 101, "gga_x_b88": 106, "gga_c_pbe": 130, "gga_c_lyp": 131,
+             "mgga_x_tpss": 202, "mgga_c_tpss": 231}
+MGGA_IDS = {202, 231}
 
 # method -> libxc components (entries of share/methods/dft_methods.json whose
 # components this build implements)
 METHODS = {"svwn5": ["lda_x", "lda_c_vwn"], "vwn5": ["lda_c_vwn"],
            "pbe": ["gga_x_pbe", "gga_c_pbe"], "pbepbe": ["gga_x_pbe", "gga_c_pbe"],
            "blyp": ["gga_x_b88", "gga_c_lyp"], "bpbe": ["gga_x_b88", "gga_c_pbe"],
-           "slater": ["lda_x"]}
+           "slater": ["lda_x"], "tpss": ["mgga_x_tpss", "mgga_c_tpss"]}
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32cache": 3}
 
@@ -377,6 +379,18 @@ class DensityFunctional:
             self.id, n, _lib.dptr(rho), _lib.dptr(sig) if gga else None, _lib.dptr(exc),
             _lib.dptr(vrho), _lib.dptr(vsig) if gga else None, device))
         return exc, vrho, vsig
+
+    def evaluate_mgga(self, rho, sigma, tau, device=0):
+        """Family::MGGA evaluation (functional.cpp:49-57): total rho, sigma and
+        tau = (1/2) sum_i |grad psi_i|^2 per point -> exc, vrho, vsigma, vtau."""
+        rho = _lib.f64(rho)
+        n = len(rho)
+        sig, t = _lib.f64(sigma), _lib.f64(tau)
+        exc, vrho, vsig, vtau = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        _lib.check(_lib.lib().occxc_functional_evaluate_mgga(
+            self.id, n, _lib.dptr(rho), _lib.dptr(sig), _lib.dptr(t), _lib.dptr(exc),
+            _lib.dptr(vrho), _lib.dptr(vsig), _lib.dptr(vtau), device))
+        return exc, vrho, vsig, vtau
 
     def evaluate_polarized(self, rho, sigma=None, device=0):
         """Spin-polarised: rho (n, 2), sigma (n, 3) -> exc (n), vrho (n, 2),

@@ -182,3 +182,42 @@ def test_gradient_and_unrestricted_reject_mgga(water_case):
     D = random_density(basis.nbf, 5, 3)
     with pytest.raises(ValueError):
         oracle.xc_build_unrestricted(ob, pts, w, TPSS, np.vstack([D, D]))
+
+
+@pytest.fixture(scope="module")
+def mgga_shim(tmp_path_factory):
+    """The GPU kernels' meta-GGA functional header compiled for the host."""
+    import ctypes
+    import pathlib
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    d = tmp_path_factory.mktemp("mgga_shim")
+    src = d / "shim.cpp"
+    src.write_text('#include "xc_functionals_mgga.hpp"\n'
+                   'extern "C" int shim(int id, double r, double s, double t, double *e, '
+                   'double *v, double *w, double *x) {\n'
+                   '  return occxc::xc_point_mgga(id, r, s, t, *e, *v, *w, *x) ? 0 : -1; }\n')
+    so = d / "shim.so"
+    subprocess.run(["g++", "-O2", "-fPIC", "-shared", f"-I{root / 'occ_amd' / 'csrc'}",
+                    str(src), "-o", str(so)], check=True)
+    L = ctypes.CDLL(str(so))
+    L.shim.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3 + \
+        [ctypes.POINTER(ctypes.c_double)] * 4
+    return L
+
+
+@pytest.mark.parametrize("fid", TPSS)
+def test_gpu_mgga_header_matches_oracle(mgga_shim, fid, rng):
+    import ctypes
+    out = [ctypes.c_double() for _ in range(4)]
+    rho, sigma, tau = random_mgga_points(rng, 200)
+    r0, s0, t0 = golden_inputs()
+    rho, sigma, tau = (np.concatenate([a, b]) for a, b in ((rho, r0), (sigma, s0), (tau, t0)))
+    e0, v0, w0, x0 = oracle.functional_eval_mgga(fid, rho, sigma, tau)
+    for i in range(len(rho)):
+        assert mgga_shim.shim(fid, rho[i], sigma[i], tau[i], *(ctypes.byref(q) for q in out)) == 0
+        e, v, w, x = (q.value for q in out)
+        assert e == pytest.approx(e0[i] * rho[i], rel=1e-12)
+        assert v == pytest.approx(v0[i], rel=1e-10, abs=1e-14)
+        assert w == pytest.approx(w0[i], rel=1e-10, abs=1e-14 * abs(w0[i]) + 1e-300)
+        assert x == pytest.approx(x0[i], rel=1e-10, abs=1e-14)
