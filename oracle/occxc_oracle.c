@@ -25,6 +25,10 @@
  *    vrho, vsigma, vtau at four points (tests/dft_tests.cpp:138-190),
  *    complex-step vs finite differences, uniform-gas limits
  *    (tests/test_oracle_mgga.py).
+ *  - VV10 (nonlocal_correlation.cpp): the reference's H2 known answer
+ *    (tests/dft_tests.cpp:608-660, to 1.5e-5 relative on a converged grid),
+ *    the symmetric double sum in numpy, and the forward + reverse point-order
+ *    potentials against the derivative of the energy (tests/test_oracle_vv10.py).
  *  - XC build: screened == dense (tests/dft_tests.cpp:904-931 restated),
  *    N_elec = tr(2 D S) with analytic overlaps, brute-force tiny cases, and
  *    the directional-derivative identity dExc/dD = Vxc.
@@ -1992,4 +1996,196 @@ long oracle_screened_work(const oracle_basis *b, long npts, const double *xyz,
   free(lrad);
   free(ext);
   return k;
+}
+
+/* -------------------------------------------------------------- VV10 NLC -- */
+/* vv10_kernel (nonlocal_correlation.cpp:37-103): per point i with
+ * rho_i >= 1e-8, F_i = -1.5 [rho_i w_i / (2 k_i^3) + sum_{j<i, rho_j >= 1e-8}
+ * 2 rho_j w_j / (g_i g_j (g_i + g_j))] with g = r^2 omega0 + kappa, and U_i,
+ * W_i the matching sums for the kappa and omega0 derivatives; every j < i
+ * pair counted twice, so that sum_i rho_i w_i F_i is the symmetric double sum.
+ * exc = (beta + F/2) w, vrho = (beta + F + 1.5 (U dkappa/drho + W
+ * domega0/drho)) w, vsigma = 1.5 W domega0/dsigma w.  Where the reference's
+ * domega0/dsigma is 0/0 (sigma = 0) it is taken as 0. */
+void oracle_vv10_kernel(long npts, const double *rho, const double *sigma, const double *xyz,
+                        const double *w, double vv10_b, double vv10_C, int nthreads,
+                        double *exc, double *vrho, double *vsigma) {
+  const double pi4_3 = 4.0 * M_PI / 3.0;
+  const double kappa_factor = vv10_b * 1.5 * M_PI * pow(9.0 * M_PI, -1.0 / 6.0);
+  const double beta = pow(3.0 / (vv10_b * vv10_b), 0.75) / 32.0;
+  double *rw = malloc(sizeof(double) * (npts + 1));
+  double *om = malloc(sizeof(double) * (npts + 1));
+  double *ka = malloc(sizeof(double) * (npts + 1));
+  double *F = calloc(npts + 1, sizeof(double));
+  double *U = calloc(npts + 1, sizeof(double));
+  double *W = calloc(npts + 1, sizeof(double));
+  for (long i = 0; i < npts; i++) {
+    rw[i] = rho[i] * w[i];
+    double t = sigma[i] / (rho[i] * rho[i]);
+    t = vv10_C * t * t;
+    om[i] = sqrt(t + pi4_3 * rho[i]);
+    ka[i] = kappa_factor * pow(rho[i], 1.0 / 6.0);
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long i = 0; i < npts; i++) {
+    if (rho[i] < 1e-8) continue;
+    const double k = ka[i];
+    double f = rw[i] / (2 * k * k * k);
+    double u = f * (1.0 / k + 1.0 / (2 * k));
+    double ww = 0.0;
+    const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+    for (long j = 0; j < i; j++) {
+      if (rho[j] < 1e-8) continue;
+      const double dx = xi - xyz[3 * j], dy = yi - xyz[3 * j + 1], dz = zi - xyz[3 * j + 2];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double gi = r2 * om[i] + ka[i];
+      const double gj = r2 * om[j] + ka[j];
+      double t = 2 * rw[j] / (gi * gj * (gi + gj));
+      f += t;
+      t *= 1.0 / gi + 1.0 / (gi + gj);
+      u += t;
+      ww += t * r2;
+    }
+    F[i] = f * -1.5;
+    U[i] = u;
+    W[i] = ww;
+  }
+  for (long i = 0; i < npts; i++) {
+    double t = sigma[i] / (rho[i] * rho[i]);
+    t = vv10_C * t * t;
+    const double domega0_dr = (0.5 * pi4_3 * rho[i] - 2.0 * t) / om[i];
+    const double domega0_dg = sigma[i] > 0.0 ? t * rho[i] / (sigma[i] * om[i]) : 0.0;
+    const double dkappa_dr = ka[i] / 6.0;
+    exc[i] = (beta + 0.5 * F[i]) * w[i];
+    if (rho[i] < 1e-8) {
+      vrho[i] = beta * w[i];
+      vsigma[i] = 0.0;
+    } else {
+      vrho[i] = (beta + F[i] + 1.5 * (U[i] * dkappa_dr + W[i] * domega0_dr)) * w[i];
+      vsigma[i] = (1.5 * W[i] * domega0_dg) * w[i];
+    }
+  }
+  free(rw);
+  free(om);
+  free(ka);
+  free(F);
+  free(U);
+  free(W);
+}
+
+/* all shells at n points: phi [4][nbf][n] */
+static void eval_all_shells(const oracle_basis *b, long n, const double *xyz, double *phi,
+                            int nbf) {
+  int off = 0;
+  for (int s = 0; s < b->nshell; s++) {
+    int l = b->l[s], pure = b->pure[s];
+    oracle_eval_shell(l, pure, b->nprim[s], b->exps + b->prim_offset[s],
+                      b->coefs + b->prim_offset[s], b->origin + 3 * s, n, xyz, 1,
+                      phi + (long)off * n, phi + ((long)nbf + off) * n,
+                      phi + (2L * nbf + off) * n, phi + (3L * nbf + off) * n, n);
+    off += oracle_shell_size(l, pure);
+  }
+}
+
+int oracle_vv10(const oracle_basis *b, long npts, const double *xyz, const double *w,
+                const double *D, double vv10_b, double vv10_C, int nthreads,
+                double *energy, double *K) {
+  if (!b || npts < 0) return -1;
+  const int nbf = oracle_nbf(b);
+  const long nblk = (npts + 127) / 128;
+  double *D2 = malloc(sizeof(double) * nbf * nbf);
+  for (long i = 0; i < (long)nbf * nbf; i++) D2[i] = 2.0 * D[i];
+  double *g = malloc(sizeof(double) * 3 * (npts + 1));
+  double *rho = malloc(sizeof(double) * (npts + 1));
+  double *sig = malloc(sizeof(double) * (npts + 1));
+  /* rho, grad rho: evaluate_density<1, R> (density.cpp:33-43) */
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+  {
+    double *phi = malloc(sizeof(double) * 4 * (size_t)nbf * 128);
+    double *T = malloc(sizeof(double) * (size_t)nbf * 128);
+#pragma omp for schedule(dynamic, 4)
+    for (long bi = 0; bi < nblk; bi++) {
+      const long p0 = 128 * bi, n = npts - p0 < 128 ? npts - p0 : 128;
+      eval_all_shells(b, n, xyz + 3 * p0, phi, nbf);
+      memset(T, 0, sizeof(double) * nbf * n);
+      for (int nu = 0; nu < nbf; nu++)
+        for (int mu = 0; mu < nbf; mu++) {
+          const double dm = D2[mu + (long)nu * nbf];
+          for (long p = 0; p < n; p++) T[nu * n + p] += dm * phi[(long)mu * n + p];
+        }
+      for (long p = 0; p < n; p++) {
+        double r = 0, gx = 0, gy = 0, gz = 0;
+        for (int nu = 0; nu < nbf; nu++) {
+          const double t = T[nu * n + p];
+          r += phi[(long)nu * n + p] * t;
+          gx += 2.0 * phi[((long)nbf + nu) * n + p] * t;
+          gy += 2.0 * phi[(2L * nbf + nu) * n + p] * t;
+          gz += 2.0 * phi[(3L * nbf + nu) * n + p] * t;
+        }
+        rho[p0 + p] = r;
+        g[3 * (p0 + p)] = gx;
+        g[3 * (p0 + p) + 1] = gy;
+        g[3 * (p0 + p) + 2] = gz;
+        sig[p0 + p] = gx * gx + gy * gy + gz * gz;
+      }
+    }
+    free(phi);
+    free(T);
+  }
+  double *exc = malloc(sizeof(double) * (npts + 1));
+  double *vr = malloc(sizeof(double) * (npts + 1));
+  double *vs = malloc(sizeof(double) * (npts + 1));
+  oracle_vv10_kernel(npts, rho, sig, xyz, w, vv10_b, vv10_C, nthreads, exc, vr, vs);
+  double e = 0.0;
+  for (long i = 0; i < npts; i++) e += exc[i] * rho[i];
+  *energy = e;
+  /* Vxc: ktmp = phi^T (vrho/2 phi + 2 vsigma grad rho . grad phi), K = ktmp + ktmp^T
+   * (nonlocal_correlation.cpp:295-303) */
+  memset(K, 0, sizeof(double) * nbf * nbf);
+#pragma omp parallel
+  {
+    double *phi = malloc(sizeof(double) * 4 * (size_t)nbf * 128);
+    double *A = malloc(sizeof(double) * (size_t)nbf * 128);
+    double *Kl = calloc((size_t)nbf * nbf, sizeof(double));
+#pragma omp for schedule(dynamic, 4)
+    for (long bi = 0; bi < nblk; bi++) {
+      const long p0 = 128 * bi, n = npts - p0 < 128 ? npts - p0 : 128;
+      eval_all_shells(b, n, xyz + 3 * p0, phi, nbf);
+      for (int nu = 0; nu < nbf; nu++)
+        for (long p = 0; p < n; p++) {
+          const long q = p0 + p;
+          A[nu * n + p] = 0.5 * vr[q] * phi[(long)nu * n + p] +
+                          2.0 * vs[q] *
+                              (g[3 * q] * phi[((long)nbf + nu) * n + p] +
+                               g[3 * q + 1] * phi[(2L * nbf + nu) * n + p] +
+                               g[3 * q + 2] * phi[(3L * nbf + nu) * n + p]);
+        }
+      for (int nu = 0; nu < nbf; nu++)
+        for (int mu = 0; mu < nbf; mu++) {
+          double s = 0.0;
+          for (long p = 0; p < n; p++) s += phi[(long)mu * n + p] * A[nu * n + p];
+          Kl[mu + (long)nu * nbf] += s;
+          Kl[nu + (long)mu * nbf] += s;
+        }
+    }
+#pragma omp critical
+    for (long i = 0; i < (long)nbf * nbf; i++) K[i] += Kl[i];
+    free(phi);
+    free(A);
+    free(Kl);
+  }
+  free(D2);
+  free(g);
+  free(rho);
+  free(sig);
+  free(exc);
+  free(vr);
+  free(vs);
+  return 0;
 }

@@ -163,6 +163,22 @@ int oracle_xc_gradient_unrestricted(const oracle_basis *b, const int *shell_atom
                                     double density_threshold, double screening_threshold,
                                     int screened, int nthreads, double *grad);
 
+/* NonLocalCorrelationFunctional::vv10 (src/dft/nonlocal_correlation.cpp:
+ * 246-306) with vv10_kernel (nonlocal_correlation.cpp:37-103): the VV10
+ * energy and Vxc (nbf x nbf column-major) of the restricted density D (no
+ * factor 2) on the grid xyz/w taken in the given point order (the per-point
+ * sums run over the points before each one, as the reference's do).  AOs are
+ * evaluated at every point (no screening), as evaluate_basis does there.
+ * Returns 0, or -1 on invalid input. */
+int oracle_vv10(const oracle_basis *b, long npts, const double *xyz, const double *w,
+                const double *D, double vv10_b, double vv10_C, int nthreads,
+                double *energy, double *K);
+/* The per-point part alone (vv10_kernel) from rho and sigma = |grad rho|^2:
+ * exc[i] and vxc (vrho[i], vsigma[i]), all weighted. */
+void oracle_vv10_kernel(long npts, const double *rho, const double *sigma, const double *xyz,
+                        const double *w, double vv10_b, double vv10_C, int nthreads,
+                        double *exc, double *vrho, double *vsigma);
+
 /* Algorithmic work of the screened build: per-leaf (n_b, m_b) with the
  * reference screening rule.  Writes up to maxleaves entries; returns the
  * number of leaves with >= 1 significant shell. */

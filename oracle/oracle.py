@@ -69,6 +69,10 @@ def lib():
                                                    _dp, _dp, _dp]
         L.oracle_xc_gradient_unrestricted.argtypes = L.oracle_xc_gradient.argtypes
         L.oracle_functional_eval_mgga.argtypes = [ctypes.c_int, ctypes.c_long] + [_dp] * 7
+        L.oracle_vv10.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp, _dp,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.oracle_vv10_kernel.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.oracle_functional_energy_density_mgga.restype = ctypes.c_double
         L.oracle_functional_energy_density_mgga.argtypes = [ctypes.c_int] + [ctypes.c_double] * 3
         L.oracle_screened_work.restype = ctypes.c_long
@@ -402,3 +406,31 @@ def gau2grid_deriv2(l, pure, exps, coefs, center, xyz):
     gau2grid().gg_collocation_deriv2(l, n, _d(xyz), 3, len(exps), _d(coefs), _d(exps),
                                      _d(center), order, *[_d(out[q]) for q in range(10)])
     return out
+
+
+def vv10(basis, xyz, w, D, b=6.0, C=0.01, nthreads=0):
+    """NonLocalCorrelationFunctional::vv10 on the CPU: (energy, Vxc) of the
+    restricted density D on the grid in the given point order."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    D = np.asfortranarray(D, dtype=np.float64)
+    K = np.zeros((basis.nbf, basis.nbf), order="F")
+    e = ctypes.c_double()
+    rc = lib().oracle_vv10(basis.ptr, len(w), _d(xyz), _d(w), D.ctypes.data_as(_dp), b, C,
+                           int(nthreads), ctypes.byref(e), K.ctypes.data_as(_dp))
+    if rc != 0:
+        raise ValueError("oracle_vv10 rejected its input")
+    return e.value, K
+
+
+def vv10_kernel(rho, sigma, xyz, w, b=6.0, C=0.01, nthreads=0):
+    """vv10_kernel: weighted exc, vrho, vsigma per point."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    n = len(rho)
+    exc, vr, vs = np.zeros(n), np.zeros(n), np.zeros(n)
+    lib().oracle_vv10_kernel(n, _d(rho), _d(sigma), _d(xyz), _d(w), b, C, int(nthreads),
+                             _d(exc), _d(vr), _d(vs))
+    return exc, vr, vs
