@@ -62,6 +62,10 @@ def workload(name):
     if name == "h2o":
         return (molecules.water(), "6-31g", False, "svwn5", 5,
                 "SVWN5 XC build, H2O, 6-31G (13 AO), 75 radial x 302 Lebedev per atom")
+    if name == "tpss32":
+        return (molecules.water_cluster(32, seed=0), "def2-svp", True, "tpss", 160,
+                "TPSS meta-GGA XC build, (H2O)32 cluster seed 0, def2-SVP spherical (768 AO), "
+                "75 radial x 302 Lebedev per atom, Becke/TA partition")
     if name == "h2o128":
         return (molecules.water_cluster(128, seed=0), "def2-svp", True, "blyp", 640,
                 "B88+LYP XC build, (H2O)128 cluster seed 0, def2-SVP spherical (3072 AO), "
@@ -119,6 +123,13 @@ def main():
         return run_unrestricted(args)
     if args.workload == "serve":
         return run_serving(args)
+    if args.workload == "vv10_32":
+        return run_vv10(args)
+    # meta-GGA: FP64 and FP32-cache contexts only; each stage is four GEMMs
+    # (rho and the three tau directions; the GGA and three tau Vxc terms)
+    mgga = args.workload == "tpss32"
+    if mgga:
+        args.modes = ",".join(m for m in args.modes.split(",") if m in ("fp64", "fp32cache"))
     atoms, bname, pure, method, nocc, desc = workload(args.workload)
     basis = AOBasis.load(atoms, bname, pure=pure)
     nbf = basis.nbf
@@ -216,12 +227,15 @@ def main():
     dens_ms = tim["density"][0] / max(tim["density"][1], 1)
     vxc_ms = tim["vxc"][0] / max(tim["vxc"][1], 1)
     flops_half = R["info"]["contraction_flops"] / 2.0   # 2 n m^2 per GEMM
+    if mgga:
+        flops_half *= 4.0
     kname, kms = ("density", dens_ms) if dens_ms >= vxc_ms else ("vxc", vxc_ms)
     achieved = flops_half / (kms * 1e-3) / 1e12
     roof = {"bound": "mfma", "kernel": kname, "achieved": achieved,
             "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
             "traffic": measured_traffic(args.workload, head, kname), "kernel_ms": kms,
-            "work": "sum over leaves of 2 n_b m_b^2 (one GEMM of the build, screened m_b)"}
+            "work": "sum over leaves of 2 n_b m_b^2 (one GEMM of the build, screened m_b)"
+                    + (" x 4 (rho and three tau GEMMs per stage)" if mgga else "")}
     ao_gbs = R["info"]["ao_write_bytes"] / (R["ao_ms"] * 1e-3) / 1e9 if R["ao_ms"] else None
     out = {
         "metric": "XC builds/s + grid pts/s at |dExc|<=1e-6 Ha vs FP64; p50/p99 request latency",
@@ -435,6 +449,89 @@ def run_unrestricted(args):
         "accuracy": {m: {k: results[m].get(k) for k in ("dexc", "dvxc", "dnelec")}
                      for m in results},
         "modes_ms": {m: results[m]["ms_per_step"] for m in results},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_vv10(args):
+    """VV10 non-local correlation (NonLocalCorrelationFunctional::vv10) of the
+    (H2O)32 / def2-SVP system on an NLC grid of 50 radial x 110 Lebedev points
+    per atom (the reference's NLC grid has at most 110 angular points):
+    a step is one evaluation from D in HBM to E_nlc and Vxc in HBM (D2, rho and
+    grad rho of every point, the O(N^2) pair sums over earlier points, the
+    coefficients and the Vxc contraction)."""
+    from occ_amd import AOBasis, molecules
+    from occ_amd.dft import XCContext
+    from occ_amd.device import DeviceArray, synchronize
+    from occ_amd.grid import GridSettings, MolecularGrid
+    atoms = molecules.water_cluster(32, seed=0)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    nbf = basis.nbf
+    pts, wts = MolecularGrid(atoms, GridSettings(radial_points=50, angular_points=110)).build(
+        device=0)
+    npts = len(wts)
+    D = seeded_density(nbf, 160, 0)
+    pairs = npts * (npts - 1) / 2.0
+    flops_per_pair = 27.0  # algorithmic FP64 operations of one pair (one reciprocal counted once)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        # bounded sample: the oracle's pair sums over the first m points (the
+        # pair loop is O(N^2); the AO and Vxc parts are O(N) and not timed)
+        m = min(npts, 360000)
+        rng = np.random.default_rng(1)
+        rho = 10 ** rng.uniform(-6, 0, m)
+        sig = 10 ** rng.uniform(-2, 0, m) * rho ** (8 / 3)
+        t0 = time.time()
+        oracle.vv10_kernel(rho, sig, pts[:m], wts[:m], 6.0, 0.01, nthreads=threads)
+        dt = time.time() - t0
+        pps = m * (m - 1) / 2.0 / dt
+        cpu = {"value": pps / pairs, "unit": "VV10 evaluations/s", "cores": threads,
+               "kind": "port",
+               "sample": f"the oracle's VV10 pair sums (oracle/occxc_oracle.c, OpenMP over "
+                         f"points) over the first {m} of the {npts} points, {dt:.2f} s on "
+                         f"{threads} threads = {pps:.3e} pairs/s, scaled to the workload's "
+                         f"{pairs:.3e} pairs",
+               "cpu_model": cpu_model()}
+        log(f"cpu oracle VV10 sample: {m} points in {dt:.2f}s ({pps:.3e} pairs/s)")
+    ctx = XCContext(basis, pts, wts, [101])
+    d_D, d_V, d_r = DeviceArray(nbf * nbf), DeviceArray(nbf * nbf), DeviceArray(2)
+    d_D.upload(D)
+    for _ in range(max(args.warmup, 1)):
+        ctx.nlc_vv10_device(d_D.ptr, d_V.ptr, d_r.ptr)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.nlc_vv10_device(d_D.ptr, d_V.ptr, d_r.ptr)
+    ctx.synchronize()
+    synchronize()
+    wall = time.perf_counter() - t0
+    tim = ctx.timing()
+    e_nlc, nel = d_r.download()
+    ms = wall / args.steps * 1e3
+    nlc_ms = tim["nlc"][0] / max(tim["nlc"][1], 1)
+    achieved = pairs * flops_per_pair / (nlc_ms * 1e-3) / 1e12
+    log(f"VV10: {ms:.3f} ms/evaluation, stages {tim}, E_nlc={e_nlc:.10f}, N={nel:.6f}")
+    ctx.close()
+    out = {
+        "metric": "VV10 evaluations/s (FP64)",
+        "value": 1e3 / ms, "unit": "VV10 evaluations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "VV10 (b = 6.0, C = 0.01), (H2O)32 cluster seed 0, def2-SVP "
+                               "spherical (768 AO), 50 radial x 110 Lebedev per atom",
+                   "npts": npts, "nbf": nbf, "pairs": pairs},
+        "roofline": {"bound": "fp64-valu", "kernel": "k_vv10_pairs (nlc stage)",
+                     "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": achieved / 78.6, "traffic": None, "kernel_ms": nlc_ms,
+                     "work": f"N(N-1)/2 pairs x {flops_per_pair:.0f} FP64 operations"},
+        "pairs_per_s": pairs / (nlc_ms * 1e-3),
+        "e_nlc": e_nlc,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

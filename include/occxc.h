@@ -241,7 +241,8 @@ enum {
   OCCXC_STAGE_FINALIZE = 3,  /* K + K^T, Exc / N_elec reduction */
   OCCXC_STAGE_AO = 4,        /* AO values of every leaf */
   OCCXC_STAGE_GRADIENT = 5,  /* XC gradient contraction and per-atom reduction */
-  OCCXC_NSTAGES = 6
+  OCCXC_STAGE_NLC = 6,       /* VV10 pair sums and coefficients */
+  OCCXC_NSTAGES = 7
 };
 occxc_status occxc_context_set_timing(occxc_context *ctx, int enable);
 occxc_status occxc_context_get_timing(occxc_context *ctx, double *ms,
@@ -302,6 +303,26 @@ occxc_status occxc_server_poll(occxc_server *srv, long long ticket, int *done);
 /* The SCF's pending builds still run; then its context is freed. */
 occxc_status occxc_server_close(occxc_server *srv, long long scf);
 occxc_status occxc_server_get_stats(occxc_server *srv, occxc_server_stats *stats);
+
+/* --- VV10 non-local correlation ------------------------------------- */
+
+/* NonLocalCorrelationFunctional::vv10 (src/dft/nonlocal_correlation.cpp:
+ * 246-306) with vv10_kernel (nonlocal_correlation.cpp:37-103): the VV10
+ * energy and its Vxc (nbf x nbf) for a restricted density D (nbf x nbf, no
+ * factor 2), on the context's grid.  Build the context from the NLC grid
+ * (set_integration_grid, nonlocal_correlation.cpp:17-31) with any GGA or
+ * meta-GGA functional id, which only selects the cached AO derivatives.  As
+ * in the reference, each point's sums run over the points before it in the
+ * grid's input order, so Vxc depends on that order and the energy does not.
+ * b, C: Parameters::vv10_b, vv10_C (nonlocal_correlation.h:14-18; 6.0,
+ * 0.01).  Single-molecule FP64 or FP32-cache contexts. */
+occxc_status occxc_nlc_vv10(occxc_context *ctx, const double *D, double b, double C,
+                            double *energy, double *Vxc);
+/* The same with device buffers: d_D (nbf x nbf) in, d_Vxc (nbf x nbf) and
+ * d_result[2] = {E_nlc, N_elec} out, on the given stream (NULL: the
+ * context's). */
+occxc_status occxc_nlc_vv10_device(occxc_context *ctx, const double *d_D, double b,
+                                   double C, double *d_Vxc, double *d_result, void *stream);
 
 /* --- stage-level entry points (each mirrors one reference function) --- */
 

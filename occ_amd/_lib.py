@@ -87,6 +87,11 @@ SIGNATURES = {
                                                            _dp, _dp, _dp, _dp, ctypes.c_int]),
     "occxc_functional_evaluate_mgga": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _dp, _dp,
                                                       _dp, _dp, _dp, _dp, _dp, ctypes.c_int]),
+    "occxc_nlc_vv10": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double, _dp,
+                                      _dp]),
+    "occxc_nlc_vv10_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "occxc_xc_gradient_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                 _ip, ctypes.c_void_p, ctypes.c_void_p]),
     "occxc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CBasis),

@@ -15,7 +15,8 @@ BUILD = HERE / "_build"
 LIB = HERE / "liboccxc.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "contract.hip", "grad.hip", "setup.hip", "api.cpp", "server.cpp"]
+SOURCES = ["kernels.hip", "contract.hip", "grad.hip", "setup.hip", "nlc.hip", "api.cpp",
+           "server.cpp"]
 COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
           "-Wno-unused-function", "-Wno-unused-command-line-argument",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -353,10 +353,11 @@ static void setup_leaves_device(occxc_context *c, int nmol, const occxc_grid *gr
   c->d_py.alloc(pt_off);
   c->d_pz.alloc(pt_off);
   c->d_pw.alloc(pt_off);
+  c->d_porig.alloc(std::max<long long>(pt_off, 1));
   launch_fill_leaves(c->d_leaves.get(), ds.fill.get(), (int)fill.size(), ds.center.get(),
                      ds.radius.get(), c->d_shells.get(), d_ext, ds.xyz.get(), ds.w.get(),
                      ds.perm.get(), c->d_leaf_shells.get(), c->d_ao_map.get(), c->d_px.get(),
-                     c->d_py.get(), c->d_pz.get(), c->d_pw.get(), s);
+                     c->d_py.get(), c->d_pz.get(), c->d_pw.get(), c->d_porig.get(), s);
 }
 
 } // namespace occxc
@@ -468,7 +469,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     }
   }
 
-  std::vector<int> leaf_shells, ao_map;
+  std::vector<int> leaf_shells, ao_map, porig;
   std::vector<double> px, py, pz, pw;
   long long pt_off = 0, ao_off = 0;
   double msum = 0.0;
@@ -562,6 +563,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
         py.push_back(h.xyz[3 * src + 1]);
         pz.push_back(h.xyz[3 * src + 2]);
         pw.push_back(p < L.n ? h.w[src] : 0.0);
+        porig.push_back(p < L.n ? (int)(h.perm.empty() ? src : h.perm[src]) : -1);
       }
       pt_off += L.np;
       msum += m;
@@ -586,6 +588,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   c->d_pz.upload(pz.data(), pz.size(), c->stream);
   c->d_pw.alloc(pw.size());
   c->d_pw.upload(pw.data(), pw.size(), c->stream);
+  c->d_porig.alloc(std::max<size_t>(porig.size(), 1));
+  c->d_porig.upload(porig.data(), porig.size(), c->stream);
   }
   c->nleaves = (int)c->leaves.size();
   c->ptot = pt_off;
@@ -725,6 +729,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     a.tpartial = c->d_tpartial.get();
   }
   c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
+                    c->d_porig.bytes() +
                     c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                     c->plan.bytes + c->d_kloc.bytes() + c->d_tpairs.bytes() +
                     c->d_tentries.bytes() + c->d_tchunks.bytes() + c->d_tpartial.bytes();
@@ -874,6 +879,59 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
                       d_result, s);
     }
   });
+}
+
+// VV10 on stream s (NonLocalCorrelationFunctional::vv10,
+// nonlocal_correlation.cpp:246-306): D2, rho and grad rho of every leaf point
+// (density kernel in store mode), the VV10 pair sums and coefficients
+// (nlc.hip), then the restricted Vxc kernel and the finalisation of a build.
+static void run_nlc(occxc_context *c, const double *d_D, double b, double C, double *d_V,
+                    double *d_result, hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const int prec = c->opt.precision;
+  StageTimer t{c, s, {}};
+  t.run(OCCXC_STAGE_PREPARE, [&] {
+    launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
+    if (!c->plan.two_pass) launch_zero(c->d_K.get(), c->mtot, s);
+  });
+  t.run(OCCXC_STAGE_DENSITY, [&] {
+    BuildArgs x = a;
+    x.rho_out = c->d_mrho.get();
+    run_density_store(x, c->d_ao.get(), prec, c->d_D2.get(), s);
+  });
+  t.run(OCCXC_STAGE_NLC, [&] {
+    launch_vv10(a, c->d_mrho.get(), c->d_porig.get(), b, C, c->d_nlc.get(), c->npts, s);
+  });
+  t.run(OCCXC_STAGE_VXC, [&] { run_vxc(c->plan, a, c->d_ao.get(), prec, s); });
+  t.run(OCCXC_STAGE_FINALIZE, [&] {
+    if (c->plan.two_pass) {
+      run_vxc_reduce(a, d_V, s);
+      launch_finalize_scalars(c->d_mols.get(), c->nmol, c->d_part.get(), d_result, s);
+    } else {
+      launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
+                      d_result, s);
+    }
+  });
+}
+
+static void nlc_check_and_setup(occxc_context *c) {
+  OCCXC_REQUIRE(c->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                "occxc_nlc_vv10 needs a single-molecule context");
+  OCCXC_REQUIRE(c->deriv == 1, OCCXC_ERR_UNSUPPORTED,
+                "occxc_nlc_vv10 needs a context built with a GGA or meta-GGA functional");
+  OCCXC_REQUIRE(c->opt.precision == OCCXC_PREC_FP64 || c->opt.precision == OCCXC_PREC_FP32_CACHE,
+                OCCXC_ERR_UNSUPPORTED, "occxc_nlc_vv10 needs an FP64 or FP32-cache context");
+  OCCXC_REQUIRE(c->plan.use_mfma && c->plan.max_mp <= kStoreMaxMp, OCCXC_ERR_UNSUPPORTED,
+                "occxc_nlc_vv10 needs the matrix-core kernels and leaves of at most " +
+                    std::to_string(kStoreMaxMp) + " AOs");
+  if (c->d_mrho.size() == 0) {
+    c->d_mrho.alloc(4 * (size_t)std::max<long long>(c->ptot, 1));
+    c->device_bytes += c->d_mrho.bytes();
+  }
+  if (c->d_nlc.size() == 0) {
+    c->d_nlc.alloc(std::max<size_t>(nlc_buffer_elems(c->npts), 1));
+    c->device_bytes += c->d_nlc.bytes();
+  }
 }
 
 // XC gradient set-up: the AO derivative components the cache lacks (once
@@ -1228,6 +1286,42 @@ occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc, doubl
   const double *Ds[1] = {D};
   double *Vs[1] = {Vxc};
   return occxc_batch_build(ctx, Ds, Vs, exc, nelec);
+}
+
+occxc_status occxc_nlc_vv10(occxc_context *ctx, const double *D, double b, double C,
+                            double *energy, double *Vxc) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && D && energy && Vxc, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                  "occxc_nlc_vv10 needs a single-molecule context");
+    OCCXC_REQUIRE(std::isfinite(b) && b > 0.0 && std::isfinite(C) && C >= 0.0,
+                  OCCXC_ERR_INVALID_ARGUMENT, "VV10 parameters: need b > 0, C >= 0");
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    for (size_t i = 0; i < nn; ++i)
+      OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
+    DeviceGuard g(ctx->device);
+    nlc_check_and_setup(ctx);
+    ctx->d_D.upload(D, nn, ctx->stream);
+    run_nlc(ctx, ctx->d_D.get(), b, C, ctx->d_V.get(), ctx->d_result.get(), ctx->stream);
+    ctx->d_V.download(Vxc, nn, ctx->stream);
+    double r[2];
+    ctx->d_result.download(r, 2, ctx->stream);
+    OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+    *energy = r[0];
+  });
+}
+
+occxc_status occxc_nlc_vv10_device(occxc_context *ctx, const double *d_D, double b,
+                                   double C, double *d_Vxc, double *d_result, void *stream) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    OCCXC_REQUIRE(std::isfinite(b) && b > 0.0 && std::isfinite(C) && C >= 0.0,
+                  OCCXC_ERR_INVALID_ARGUMENT, "VV10 parameters: need b > 0, C >= 0");
+    DeviceGuard g(ctx->device);
+    nlc_check_and_setup(ctx);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    run_nlc(ctx, d_D, b, C, d_Vxc, d_result, s);
+  });
 }
 
 occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,

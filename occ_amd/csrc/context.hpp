@@ -35,6 +35,7 @@ struct occxc_context {
   occxc::DevBuf<occxc::MolDev> d_mols;
   occxc::DevBuf<int> d_leaf_shells, d_ao_map, d_active;
   occxc::DevBuf<double> d_px, d_py, d_pz, d_pw;
+  occxc::DevBuf<int> d_porig;  // input index (within its molecule) of each padded point, -1 for padding
   occxc::DevBuf<char> d_ao;
   occxc::DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
   occxc::ContractPlan plan;
@@ -82,6 +83,8 @@ struct occxc_context {
   bool mgga = false;
   occxc::DevBuf<double> d_mrho, d_mtau, d_coef_tau;
   occxc::DevBuf<occxc::LeafDev> d_leaves_k;
+  // VV10 (allocated on the first call): input-order point planes and F, U, W
+  occxc::DevBuf<double> d_nlc;
 };
 
 namespace occxc {

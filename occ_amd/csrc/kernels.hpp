@@ -141,7 +141,7 @@ void launch_fill_leaves(const LeafDev *leaves, const FillLeaf *fill, int nactive
                         const double *center, const double *radius, const ShellDev *shells,
                         const double *ext, const double *xyz, const double *w, const int *perm,
                         int *leaf_shells, int *ao_map, double *px, double *py, double *pz,
-                        double *pw, hipStream_t s);
+                        double *pw, int *porig, hipStream_t s);
 
 // host-callable launchers (kernels.hip)
 void upload_sph_table();
@@ -189,6 +189,13 @@ void launch_functional_mgga_leaves(const BuildArgs &a, const double *rho4, const
 void launch_functional_mgga(int id, long long n, const double *rho, const double *sigma,
                             const double *tau, double *exc, double *vrho, double *vsigma,
                             double *vtau, hipStream_t s);
+// VV10 (nlc.hip): from rho / grad rho of the leaf points (rho4, [4][ptot])
+// and each padded point's input index (porig), the weighted VV10 exc / vrho /
+// vsigma as GGA Vxc coefficients in a.coef and {E_nlc, N_elec} per leaf in
+// a.leaf_part (every leaf active); nlc holds nlc_buffer_elems(npts) doubles
+size_t nlc_buffer_elems(long long npts);
+void launch_vv10(const BuildArgs &a, const double *rho4, const int *porig, double b, double C,
+                 double *nlc, long long npts, hipStream_t s);
 void launch_finalize(const double *K, double *Vxc, const MolDev *mols, int nmol,
                      int maxnbf, const double *leaf_part, double *result, hipStream_t s);
 // {Exc, N_elec} per molecule only (the two-pass Vxc writes Vxc itself)

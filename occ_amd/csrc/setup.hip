@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64) void k_fill_leaf(const LeafDev *__restrict__ le
                                                   int *__restrict__ leaf_shells,
                                                   int *__restrict__ ao_map, double *__restrict__ px,
                                                   double *__restrict__ py, double *__restrict__ pz,
-                                                  double *__restrict__ pw) {
+                                                  double *__restrict__ pw, int *__restrict__ porig) {
   const LeafDev L = leaves[blockIdx.x];
   const FillLeaf F = fill[blockIdx.x];
   const int lane = threadIdx.x;
@@ -220,6 +220,7 @@ __global__ __launch_bounds__(64) void k_fill_leaf(const LeafDev *__restrict__ le
     py[q] = xyz[3 * src + 1];
     pz[q] = xyz[3 * src + 2];
     pw[q] = p < L.n ? w[src] : 0.0;
+    porig[q] = p < L.n ? perm[F.src_pt + p] : -1;
   }
 }
 
@@ -271,10 +272,10 @@ void launch_fill_leaves(const LeafDev *leaves, const FillLeaf *fill, int nactive
                         const double *center, const double *radius, const ShellDev *shells,
                         const double *ext, const double *xyz, const double *w, const int *perm,
                         int *leaf_shells, int *ao_map, double *px, double *py, double *pz,
-                        double *pw, hipStream_t s) {
+                        double *pw, int *porig, hipStream_t s) {
   if (nactive <= 0) return;
   k_fill_leaf<<<nactive, 64, 0, s>>>(leaves, fill, center, radius, shells, ext, xyz, w, perm,
-                                     leaf_shells, ao_map, px, py, pz, pw);
+                                     leaf_shells, ao_map, px, py, pz, pw, porig);
   check("screening: leaf tables");
 }
 

@@ -107,6 +107,25 @@ class XCContext:
                                                  ctypes.c_void_p(d_result),
                                                  ctypes.c_void_p(stream or 0)))
 
+    def nlc_vv10(self, D, b=6.0, C=0.01):
+        """VV10 energy and Vxc of the restricted density D on this context's
+        grid (NonLocalCorrelationFunctional::vv10)."""
+        D = np.asfortranarray(D, dtype=np.float64)
+        if D.shape != (self.nbf, self.nbf):
+            raise ValueError(f"D must be {self.nbf}x{self.nbf}")
+        V = np.zeros((self.nbf, self.nbf), order="F")
+        e = ctypes.c_double()
+        _lib.check(_lib.lib().occxc_nlc_vv10(self._h, D.ctypes.data_as(_lib._dp), float(b),
+                                             float(C), ctypes.byref(e),
+                                             V.ctypes.data_as(_lib._dp)))
+        return e.value, V
+
+    def nlc_vv10_device(self, d_D, d_V, d_result, b=6.0, C=0.01, stream=None):
+        _lib.check(_lib.lib().occxc_nlc_vv10_device(self._h, ctypes.c_void_p(d_D), float(b),
+                                                    float(C), ctypes.c_void_p(d_V),
+                                                    ctypes.c_void_p(d_result),
+                                                    ctypes.c_void_p(stream or 0)))
+
     def build_unrestricted(self, D):
         """Unrestricted build (occxc_build_unrestricted): D is (2 nbf, nbf),
         alpha block over beta block; returns (Vxc (2 nbf, nbf), Exc,
@@ -165,7 +184,7 @@ class XCContext:
                                                        ctypes.c_void_p(d_grad),
                                                        ctypes.c_void_p(stream or 0)))
 
-    STAGES = ("prepare", "density", "vxc", "finalize", "ao", "gradient")
+    STAGES = ("prepare", "density", "vxc", "finalize", "ao", "gradient", "nlc")
 
     def set_timing(self, enable=True):
         _lib.check(_lib.lib().occxc_context_set_timing(self._h, int(bool(enable))))
@@ -405,6 +424,42 @@ class DensityFunctional:
             self.id, n, _lib.dptr(rho), _lib.dptr(sig) if gga else None, _lib.dptr(exc),
             _lib.dptr(vrho), _lib.dptr(vsig) if gga else None, device))
         return exc, vrho, vsig
+
+
+class NonLocalCorrelationFunctional:
+    """occ::dft::NonLocalCorrelationFunctional (include/occ/dft/
+    nonlocal_correlation.h, src/dft/nonlocal_correlation.cpp): VV10 on the GPU.
+
+        nlc = NonLocalCorrelationFunctional()
+        nlc.set_parameters(b=6.0, C=0.01)
+        nlc.set_integration_grid(basis, points, weights)   # the NLC grid
+        energy, Vxc = nlc(D)
+    """
+
+    def __init__(self, b=6.0, C=0.01):
+        self.b, self.C = float(b), float(C)
+        self.ctx = None
+
+    def set_parameters(self, b=6.0, C=0.01):
+        self.b, self.C = float(b), float(C)
+
+    def set_integration_grid(self, basis, points, weights, precision="fp64", screen=True,
+                             device=0):
+        if self.ctx is not None:
+            self.ctx.close()
+        # the functional id only selects a GGA AO cache (phi and grad phi)
+        self.ctx = XCContext(basis, points, weights, [LIBXC_IDS["gga_x_pbe"]],
+                             precision=precision, screen=screen, device=device)
+
+    def __call__(self, D):
+        if self.ctx is None:
+            raise RuntimeError("set_integration_grid first")
+        return self.ctx.nlc_vv10(D, self.b, self.C)
+
+    def close(self):
+        if self.ctx is not None:
+            self.ctx.close()
+            self.ctx = None
 
 
 def grid_hierarchy(points, weights):

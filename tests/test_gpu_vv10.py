@@ -1,0 +1,124 @@
+"""GPU parity of VV10 non-local correlation (occxc_nlc_vv10;
+NonLocalCorrelationFunctional::vv10, src/dft/nonlocal_correlation.cpp:
+246-306, with vv10_kernel, :37-103) against the CPU oracle, through the C ABI.
+Tolerances: |dE| <= 1e-10 Ha and max|dVxc| <= 1e-9 against the oracle on the
+same grid in the same point order (the per-point sums run over the points
+before each one, so Vxc depends on that order)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_density
+from gaussian_integrals import overlap_matrix
+from occ_amd import AOBasis, molecules
+from occ_amd.dft import NonLocalCorrelationFunctional, XCBatch, XCContext
+from occ_amd.grid import GridSettings, MolecularGrid
+
+pytestmark = pytest.mark.gpu
+
+TOL_E, TOL_V = 1e-10, 1e-9
+H2 = [(1, 0.0, 0.0, 0.0), (1, 0.0, 0.0, 1.3983973391)]
+
+
+def _grid(atoms, nrad, nang):
+    g = MolecularGrid(atoms, GridSettings(radial_points=nrad, angular_points=nang))
+    return g.build(device=0)
+
+
+def _compare(basis, pts, w, D, screen=True, precision="fp64", b=6.0, C=0.01):
+    print(f"vv10 case nbf={basis.nbf} npts={len(w)} screen={screen} {precision}", flush=True)
+    nlc = NonLocalCorrelationFunctional(b, C)
+    nlc.set_integration_grid(basis, pts, w, precision=precision, screen=screen)
+    e, V = nlc(D)
+    nlc.close()
+    e0, K0 = oracle.vv10(oracle.OracleBasis.from_flat(basis.flat()), pts, w, D, b, C)
+    return e, V, e0, K0
+
+
+def test_h2_known_answer_and_oracle():
+    basis = AOBasis.load(H2, "sto-3g", pure=False)
+    c = 1.0 / np.sqrt(overlap_matrix(basis).sum())
+    D = np.full((2, 2), c * c)
+    pts, w = _grid(H2, 75, 302)
+    e, V, e0, K0 = _compare(basis, pts, w, D)
+    print(f"H2 VV10: E={e:.10f} oracle {e0:.10f} |dE|={abs(e - e0):.2e} "
+          f"max|dV|={np.abs(V - K0).max():.2e}")
+    assert abs(e - e0) <= TOL_E and np.abs(V - K0).max() <= TOL_V
+    assert e == pytest.approx(0.008940471, rel=2e-5)  # tests/dft_tests.cpp:608-660
+
+
+@pytest.mark.parametrize("screen", [True, False])
+def test_water_631g(screen):
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, w = _grid(atoms, 31, 110)
+    D = random_density(basis.nbf, 5, 21)
+    e, V, e0, K0 = _compare(basis, pts, w, D, screen=screen)
+    print(f"water VV10 screen={screen}: |dE|={abs(e - e0):.2e} max|dV|={np.abs(V - K0).max():.2e}")
+    assert abs(e - e0) <= TOL_E and np.abs(V - K0).max() <= TOL_V
+
+
+def test_point_order_follows_the_input():
+    """Reversing the grid's input order changes Vxc as the reference's sums
+    over earlier points do, and leaves the energy."""
+    atoms = molecules.water_cluster(2, seed=2)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, w = _grid(atoms, 25, 86)
+    D = random_density(basis.nbf, 10, 22)
+    e1, V1, e01, K01 = _compare(basis, pts, w, D)
+    e2, V2, e02, K02 = _compare(basis, pts[::-1].copy(), w[::-1].copy(), D)
+    assert abs(e1 - e01) <= TOL_E and np.abs(V1 - K01).max() <= TOL_V
+    assert abs(e2 - e02) <= TOL_E and np.abs(V2 - K02).max() <= TOL_V
+    assert abs(e1 - e2) <= 1e-12
+    assert np.abs(V1 - V2).max() > 1e-8
+
+
+def test_device_entry_fp32_cache_and_host_setup(monkeypatch):
+    from occ_amd.device import DeviceArray
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, w = _grid(atoms, 31, 110)
+    D = random_density(basis.nbf, 5, 23)
+    n = basis.nbf
+    ctx = XCContext(basis, pts, w, [101, 130])
+    e, V = ctx.nlc_vv10(D)
+    dD, dV, dr = DeviceArray(n * n), DeviceArray(n * n), DeviceArray(2)
+    dD.upload(D)
+    ctx.nlc_vv10_device(dD.ptr, dV.ptr, dr.ptr)
+    ctx.synchronize()
+    # the Vxc scatter uses FP64 atomics: equal to rounding, not bit for bit
+    assert np.abs(dV.download((n, n)) - V).max() <= 1e-14 and dr.download()[0] == e
+    ctx.close()
+    monkeypatch.setenv("OCCXC_HOST_SETUP", "1")
+    hctx = XCContext(basis, pts, w, [101, 130])
+    eh, Vh = hctx.nlc_vv10(D)
+    hctx.close()
+    monkeypatch.delenv("OCCXC_HOST_SETUP")
+    assert eh == e and np.abs(Vh - V).max() <= 1e-14
+    e3, V3, e0, K0 = _compare(basis, pts, w, D, precision="fp32cache")
+    assert abs(e3 - e0) <= 1e-7 and np.abs(V3 - K0).max() <= 1e-6
+
+
+def test_rejections():
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, w = _grid(atoms, 21, 50)
+    D = random_density(basis.nbf, 5, 3)
+    lda = XCContext(basis, pts, w, [1, 7])
+    with pytest.raises(Exception):
+        lda.nlc_vv10(D)
+    lda.close()
+    ctx = XCContext(basis, pts, w, [101])
+    with pytest.raises(Exception):
+        ctx.nlc_vv10(D, b=-1.0)
+    ctx.close()
+    import ctypes
+    from occ_amd import _lib
+    batch = XCBatch([basis, basis], [(pts, w), (pts, w)], [101])
+    D2 = np.asfortranarray(D)
+    V = np.zeros_like(D2)
+    e = ctypes.c_double()
+    rc = _lib.lib().occxc_nlc_vv10(batch.handle, D2.ctypes.data_as(_lib._dp), 6.0, 0.01,
+                                   ctypes.byref(e), V.ctypes.data_as(_lib._dp))
+    assert rc == 1  # OCCXC_ERR_INVALID_ARGUMENT
+    batch.close()
