@@ -780,7 +780,7 @@ __device__ __forceinline__ void sym_pair(int p, int nt, int &I, int &J) {
   J = i + rem;
 }
 
-template <typename S, typename T>
+template <typename S, typename T, bool GGA>
 __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__restrict__ ao) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -801,26 +801,50 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
   const int l16 = lane & 15, g = lane >> 4;
   const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
-  const bool deriv = a.deriv != 0;
   for (int i = t; i < L.np; i += blockDim.x)
-    for (int k = 0; k < (deriv ? 4 : 1); ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+    for (int k = 0; k < (GGA ? 4 : 1); ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
   const int nt = L.mp >> 4;
   const int npair = nt * (nt + 1) / 2;
   acc_t acc[kSymPerWave];
 #pragma unroll
   for (int k = 0; k < kSymPerWave; ++k) acc[k] = M::zero();
+  // staging of one point chunk: groups of 4 consecutive points of one AO row;
+  // each thread takes kSymGrp groups per step and issues every load of them
+  // (4 points x 4 planes each, 32-byte aligned) before using any, so a step
+  // costs one memory round trip (indices past the end are clamped for the
+  // loads and skipped for the stores)
+  constexpr int kSymGrp = 2;
+  const int ngrp = L.mp * (kSymPts / 4);
   for (int p0 = 0; p0 < L.np; p0 += kSymPts) {
     __syncthreads();
-    for (int idx = t; idx < L.mp * kSymPts; idx += blockDim.x) {
-      const int mu = idx / kSymPts, p = idx - mu * kSymPts;
-      const long long o = (long long)mu * L.np + p0 + p;
-      const double f0 = (double)phi[o];
-      double v = s_c[0][p0 + p] * f0;
-      if (deriv)
-        v += s_c[1][p0 + p] * (double)phi[cs + o] + s_c[2][p0 + p] * (double)phi[2 * cs + o] +
-             s_c[3][p0 + p] * (double)phi[3 * cs + o];
-      s_phi[mu * LD + p] = static_cast<T>(f0);
-      s_A[mu * LD + p] = static_cast<T>(v);
+    for (int g0 = t; g0 < ngrp; g0 += kSymGrp * blockDim.x) {
+      typename Mfma<double>::vec_t f[kSymGrp], fx[kSymGrp], fy[kSymGrp], fz[kSymGrp];
+#pragma unroll
+      for (int u = 0; u < kSymGrp; ++u) {
+        const int gi = min(g0 + u * (int)blockDim.x, ngrp - 1);
+        const long long o = (long long)(gi >> 2) * L.np + p0 + 4 * (gi & 3);
+        f[u] = load4_as<double>(phi + o);
+        if constexpr (GGA) {
+          fx[u] = load4_as<double>(phi + cs + o);
+          fy[u] = load4_as<double>(phi + 2 * cs + o);
+          fz[u] = load4_as<double>(phi + 3 * cs + o);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kSymGrp; ++u) {
+        const int gi = g0 + u * (int)blockDim.x;
+        if (gi >= ngrp) break;
+        const int mu = gi >> 2, p = 4 * (gi & 3);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          double v = s_c[0][p0 + p + e] * f[u][e];
+          if constexpr (GGA)
+            v += s_c[1][p0 + p + e] * fx[u][e] + s_c[2][p0 + p + e] * fy[u][e] +
+                 s_c[3][p0 + p + e] * fz[u][e];
+          s_phi[mu * LD + p + e] = static_cast<T>(f[u][e]);
+          s_A[mu * LD + p + e] = static_cast<T>(v);
+        }
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -1395,13 +1419,22 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_density_blk<float, double>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)density_blk_smem(kBlkMaxMp)));
-  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<double, double>),
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<double, double, true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
-  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, double>),
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<double, double, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
-  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, float>),
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, double, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, double, false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, float, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(float), kSymMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, float, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)vxc_sym_smem(sizeof(float), kSymMaxMp)));
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_bf16),
@@ -1536,22 +1569,22 @@ void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int p
   const int smp = std::min(plan.max_mp, kSymMaxMp);
   if (precision == 0) {
     if (sym)
-      k_vxc_sym<double, double><<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp), s>>>(
-          a, static_cast<const double *>(ao));
+      (a.deriv ? k_vxc_sym<double, double, true> : k_vxc_sym<double, double, false>)
+          <<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp), s>>>(a, static_cast<const double *>(ao));
     if (!sym || plan.max_mp > kSymMaxMp)
       k_vxc_mfma<double, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
           a, static_cast<const double *>(ao), min_mp);
   } else if (precision == 3) {
     if (sym)
-      k_vxc_sym<float, double><<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp), s>>>(
-          a, static_cast<const float *>(ao));
+      (a.deriv ? k_vxc_sym<float, double, true> : k_vxc_sym<float, double, false>)
+          <<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp), s>>>(a, static_cast<const float *>(ao));
     if (!sym || plan.max_mp > kSymMaxMp)
       k_vxc_mfma<float, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(
           a, static_cast<const float *>(ao), min_mp);
   } else {
     if (sym)
-      k_vxc_sym<float, float><<<a.nleaves, 256, vxc_sym_smem(sizeof(float), smp), s>>>(
-          a, static_cast<const float *>(ao));
+      (a.deriv ? k_vxc_sym<float, float, true> : k_vxc_sym<float, float, false>)
+          <<<a.nleaves, 256, vxc_sym_smem(sizeof(float), smp), s>>>(a, static_cast<const float *>(ao));
     if (!sym || plan.max_mp > kSymMaxMp)
       k_vxc_mfma<float, float, 64><<<a.nleaves, 256, vxc_smem(sizeof(float)), s>>>(
           a, static_cast<const float *>(ao), min_mp);
