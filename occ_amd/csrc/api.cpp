@@ -805,9 +805,10 @@ void collect_timing(occxc_context *c) {
 // and grad rho come from the density kernel in store mode; each direction's
 // rowsum((grad_k phi D2) o grad_k phi) from the same kernel run on that
 // derivative plane of the AO cache in place of phi (a.deriv = 0); then the
-// per-leaf meta-GGA functional stage.  The GGA part of Vxc is the restricted
-// Vxc kernel; the tau part is the same kernel run on each derivative plane in
-// its LDA form with the tau coefficients.  The laplacian (density.cpp:88-93)
+// per-leaf meta-GGA functional stage.  Vxc: the symmetric kernel takes the
+// GGA term and the three tau terms of a leaf in one launch (one scatter);
+// larger leaves run the general kernel on the GGA term and on each
+// derivative plane in its LDA form with the tau coefficients.  The laplacian (density.cpp:88-93)
 // is not formed: no functional here depends on it and the reference's Vxc
 // does not use vlaplacian.
 static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, double *d_result,
@@ -837,12 +838,9 @@ static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, dou
     launch_functional_mgga_leaves(a, c->d_mrho.get(), c->d_mtau.get(), c->d_coef_tau.get(), s);
   });
   t.run(OCCXC_STAGE_VXC, [&] {
-    run_vxc(c->plan, a, c->d_ao.get(), prec, s);
-    for (int k = 0; k < 3; ++k) {
-      BuildArgs y = plane(k);
-      y.coef = c->d_coef_tau.get();
-      run_vxc(c->plan, y, c->d_ao.get(), prec, s);
-    }
+    BuildArgs x = a;
+    x.coef_tau = c->d_coef_tau.get();
+    run_vxc_mgga(c->plan, x, c->d_leaves_k.get(), c->d_ao.get(), prec, s);
   });
   t.run(OCCXC_STAGE_FINALIZE, [&] {
     launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),

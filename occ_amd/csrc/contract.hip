@@ -21,6 +21,8 @@
 // contiguous (one vector load).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cstdlib>
 #include <stdexcept>
@@ -764,9 +766,9 @@ constexpr int kSymPerWave = (kSymMaxTiles * (kSymMaxTiles + 1) / 2 + 3) / 4;
 
 template <typename T> constexpr int sym_ld() { return kSymPts + 16 / (int)sizeof(T); }
 
-static size_t vxc_sym_smem(size_t elem, int max_mp) {
+static size_t vxc_sym_smem(size_t elem, int max_mp, int ncoef = 4) {
   const size_t ld = kSymPts + 16 / elem;
-  return sizeof(double) * 4 * kMaxLeafPts + 2 * elem * ld * (size_t)max_mp;
+  return sizeof(double) * ncoef * kMaxLeafPts + 2 * elem * ld * (size_t)max_mp;
 }
 
 // upper-triangle tile pair p (row-major over I <= J of an nt x nt grid)
@@ -780,8 +782,13 @@ __device__ __forceinline__ void sym_pair(int p, int nt, int &I, int &J) {
   J = i + rem;
 }
 
-template <typename S, typename T, bool GGA>
+// TAU (restricted meta-GGA, GGA = true): after the GGA term, each chunk is
+// staged three more times with d_k Phi in place of Phi and c_tau d_k Phi in
+// place of A (c_tau = a.coef_tau), accumulating V += 2 d_k Phi^T c_tau d_k Phi
+// into the same tile pairs, so the leaf is scattered once.
+template <typename S, typename T, bool GGA, bool TAU = false>
 __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__restrict__ ao) {
+  constexpr int NC = TAU ? 5 : 4;  // coefficient planes in LDS
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
   using vec_t = typename M::vec_t;
@@ -792,7 +799,7 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
   if (!a.leaf_active[leaf]) return;
   const LeafDev L = a.leaves[leaf];
   if (L.mp > kSymMaxMp) return;  // k_vxc_mfma handles it
-  T *s_phi = reinterpret_cast<T *>(smem + 4 * kMaxLeafPts);
+  T *s_phi = reinterpret_cast<T *>(smem + NC * kMaxLeafPts);
   T *s_A = s_phi + L.mp * LD;
   const MolDev MD = a.mols[L.mol];
   double *Km = a.K + MD.moff;
@@ -801,8 +808,10 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
   const int l16 = lane & 15, g = lane >> 4;
   const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
-  for (int i = t; i < L.np; i += blockDim.x)
+  for (int i = t; i < L.np; i += blockDim.x) {
     for (int k = 0; k < (GGA ? 4 : 1); ++k) s_c[k][i] = a.coef[k * a.ptot + L.pt_off + i];
+    if constexpr (TAU) s_c[4][i] = a.coef_tau[L.pt_off + i];
+  }
   const int nt = L.mp >> 4;
   const int npair = nt * (nt + 1) / 2;
   acc_t acc[kSymPerWave];
@@ -815,7 +824,11 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
   // loads and skipped for the stores)
   constexpr int kSymGrp = 2;
   const int ngrp = L.mp * (kSymPts / 4);
-  for (int p0 = 0; p0 < L.np; p0 += kSymPts) {
+  // one term of a chunk: stage (Phi, A) of term 0 or (d_k Phi, c_tau d_k Phi)
+  // of term k, then the tile-pair MFMAs
+  auto term_pass = [&](int p0, auto term_tag) {
+    constexpr int TERM = decltype(term_tag)::value;
+    const S *src = phi + (long long)TERM * cs;
     __syncthreads();
     for (int g0 = t; g0 < ngrp; g0 += kSymGrp * blockDim.x) {
       typename Mfma<double>::vec_t f[kSymGrp], fx[kSymGrp], fy[kSymGrp], fz[kSymGrp];
@@ -823,8 +836,8 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
       for (int u = 0; u < kSymGrp; ++u) {
         const int gi = min(g0 + u * (int)blockDim.x, ngrp - 1);
         const long long o = (long long)(gi >> 2) * L.np + p0 + 4 * (gi & 3);
-        f[u] = load4_as<double>(phi + o);
-        if constexpr (GGA) {
+        f[u] = load4_as<double>(src + o);
+        if constexpr (GGA && TERM == 0) {
           fx[u] = load4_as<double>(phi + cs + o);
           fy[u] = load4_as<double>(phi + 2 * cs + o);
           fz[u] = load4_as<double>(phi + 3 * cs + o);
@@ -837,8 +850,8 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
         const int mu = gi >> 2, p = 4 * (gi & 3);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          double v = s_c[0][p0 + p + e] * f[u][e];
-          if constexpr (GGA)
+          double v = s_c[TERM == 0 ? 0 : 4][p0 + p + e] * f[u][e];
+          if constexpr (GGA && TERM == 0)
             v += s_c[1][p0 + p + e] * fx[u][e] + s_c[2][p0 + p + e] * fy[u][e] +
                  s_c[3][p0 + p + e] * fz[u][e];
           s_phi[mu * LD + p + e] = static_cast<T>(f[u][e]);
@@ -865,6 +878,14 @@ __global__ __launch_bounds__(256, 1) void k_vxc_sym(BuildArgs a, const S *__rest
           for (int s4 = 0; s4 < 4; ++s4) acc[k] = M::mma(ai[s4], fj[s4], acc[k]);
         }
       }
+    }
+  };
+  for (int p0 = 0; p0 < L.np; p0 += kSymPts) {
+    term_pass(p0, std::integral_constant<int, 0>{});
+    if constexpr (TAU) {
+      term_pass(p0, std::integral_constant<int, 1>{});
+      term_pass(p0, std::integral_constant<int, 2>{});
+      term_pass(p0, std::integral_constant<int, 3>{});
     }
   }
   // acc[k][r] = tile (I, J) element (mu = 16 I + row(g, r), nu = 16 J + l16)
@@ -1431,6 +1452,12 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, double, false>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)vxc_sym_smem(sizeof(double), kSymMaxMp)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<double, double, true, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(double), kSymMaxMp, 5)));
+  OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, double, true, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vxc_sym_smem(sizeof(double), kSymMaxMp, 5)));
   OCCXC_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_vxc_sym<float, float, true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)vxc_sym_smem(sizeof(float), kSymMaxMp)));
@@ -1590,6 +1617,38 @@ void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int p
           a, static_cast<const float *>(ao), min_mp);
   }
   check_launch("k_vxc_mfma");
+}
+
+void run_vxc_mgga(const ContractPlan &plan, const BuildArgs &a, const LeafDev *leaves_k,
+                  const void *ao, int precision, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  // leaves of up to kSymMaxMp AOs: the GGA term and the three tau terms in
+  // one symmetric launch; the others: the general kernel on the GGA term and
+  // on each derivative plane (LDA form, tau coefficients)
+  const bool sym = plan.vxc_variant == 0;
+  const int min_mp = sym ? kSymMaxMp + 1 : 0;
+  const int smp = std::min(plan.max_mp, kSymMaxMp);
+  auto launch = [&](auto s_tag) {
+    using S = decltype(s_tag);
+    const S *x = static_cast<const S *>(ao);
+    if (sym)
+      k_vxc_sym<S, double, true, true>
+          <<<a.nleaves, 256, vxc_sym_smem(sizeof(double), smp, 5), s>>>(a, x);
+    if (!sym || plan.max_mp > kSymMaxMp) {
+      k_vxc_mfma<S, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(a, x, min_mp);
+      for (int k = 0; k < 3; ++k) {
+        BuildArgs y = a;
+        y.deriv = 0;
+        y.leaves = leaves_k + (size_t)k * a.nleaves;
+        y.coef = const_cast<double *>(a.coef_tau);
+        y.coef_tau = nullptr;
+        k_vxc_mfma<S, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(y, x, min_mp);
+      }
+    }
+  };
+  if (precision == 0) launch(double{});
+  else launch(float{});
+  check_launch("k_vxc_sym (meta-GGA)");
 }
 
 } // namespace occxc

@@ -41,6 +41,12 @@ void run_density_store(const BuildArgs &a, const void *ao, int precision, const 
 constexpr int kStoreMaxMp = 2048;
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s);
+// restricted meta-GGA Vxc (atomics): the GGA term with a.coef and the three
+// tau terms with a.coef_tau (K += d_k Phi^T (c_tau d_k Phi)); leaves_k holds
+// the leaf tables offset to the d/dx, d/dy, d/dz planes ([3][nleaves]);
+// FP64 or FP32-cache AO values
+void run_vxc_mgga(const ContractPlan &plan, const BuildArgs &a, const LeafDev *leaves_k,
+                  const void *ao, int precision, hipStream_t s);
 // two-pass Vxc: V = sum over leaves of the stored K_loc blocks (+ transposes),
 // one workgroup per 32 x 32 tile pair, fixed summation order
 void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s);

@@ -125,6 +125,7 @@ struct BuildArgs {
   int ntchunks;
   double *tpartial;     // [ntchunks][32 * 32] partial blocks
   double *rho_out;      // unrestricted: [4][ptot] rho, grad rho of one spin (density store)
+  const double *coef_tau;  // restricted meta-GGA: [ptot] tau coefficients (null otherwise)
 };
 
 // device set-up of the screened layout (setup.hip)
