@@ -390,6 +390,18 @@ typedef enum {
 occxc_status occxc_radial_grid(int method, int n, double param, double *r,
                                double *w, int *npoints_out);
 
+/* generate_lmg_radial_grid (src/numint/grid_utils.cpp:397-515), the
+ * default radial rule of MolecularGrid (include/occ/numint/molecular_grid.h:
+ * 128): alpha_max = the atom's largest exponent, alpha_min[0..l_max] its
+ * smallest exponent per angular momentum (DBL_MAX where it has no shell of
+ * that l, as MolecularGrid::initialize_from_basis leaves it,
+ * molecular_grid.cpp:207-228; l_max = the basis' largest l), atomic_radius the
+ * Bragg radius in Bohr.  Radial weights include r^2 but not 4 pi.  Writes
+ * *npoints_out; fills r, w only when capacity >= that (size query first). */
+occxc_status occxc_lmg_radial_grid(double precision, double alpha_max, int l_max,
+                                   const double *alpha_min, double atomic_radius,
+                                   int capacity, double *r, double *w, int *npoints_out);
+
 /* Becke partition with Treutler-Ahlrichs radii adjustment
  * (calculate_atomic_grid_weights, grid_utils.cpp:125-232) and the per-atom
  * normalisation of MolecularGrid::get_partitioned_atom_grid

@@ -17,6 +17,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1769,6 +1770,97 @@ occxc_status occxc_radial_grid(int method, int n, double param, double *r, doubl
     }
     default:
       throw Error(OCCXC_ERR_UNSUPPORTED, "unknown radial grid method");
+    }
+  });
+}
+
+// LMG radial rule (R. Lindh, P.-A. Malmqvist, L. Gagliardi, TCA 106, 178
+// (2001)) as generate_lmg_radial_grid builds it (grid_utils.cpp:397-515): the
+// inner radius from eq. 25 (m = 0, d = 1.9) at twice the largest exponent, the
+// outer radius per angular momentum from eq. 19 and the log-grid step from
+// eqs. 17-18, both found by the same bracketing walk (step x 0.1 on each
+// sign change, until two iterates agree to 1e-14), then
+// r_i = c (e^{(i+1) h} - 1), w_i = (r_i + c) h r_i^2, c = r_in / (e^h - 1).
+// The expressions keep the reference's operation order: the point count is
+// the integer part of log(1 + r_out / c) / h.
+namespace {
+double lmg_inner_radius(double eps, double alpha) {
+  const int m = 0;
+  double r = 1.9 - std::log(1.0 / eps);
+  r = r * 2.0 / (m + 3.0);
+  r = std::exp(r) / alpha;
+  return std::sqrt(r);
+}
+
+double lmg_outer_radius(double eps, double alpha, int l, double guess) {
+  const int m = 2 * l;
+  double r = guess, prev = 1.0e50, step = 0.5, f = 1.0e50;
+  double sign = f > eps ? 1.0 : -1.0;
+  while (std::fabs(prev - r) > 1e-14) {
+    const double c = std::tgamma((m + 3.0) / 2.0);
+    const double a = std::pow(alpha * r * r, (m + 1.0) / 2.0);
+    const double e = std::exp(-alpha * r * r);
+    f = c * a * e;
+    const double last = sign;
+    sign = f > eps ? 1.0 : -1.0;
+    if (r < 0.0) sign = 1.0;
+    if (sign != last) step *= 0.1;
+    prev = r;
+    r += sign * step;
+  }
+  return r;
+}
+
+double lmg_step(double eps, int l, double guess) {
+  const int m = 2 * l;
+  const double pi = 3.14159265358979323846;
+  double h = guess, prev = 1.0e50, step = 0.1 * guess, f = 1.0e50;
+  double sign = f > eps ? -1.0 : 1.0;
+  while (std::fabs(prev - h) > 1e-14) {
+    const double c0 = 4.0 * std::sqrt(2.0) * pi;
+    const double cm = std::tgamma(3.0 / 2.0) / std::tgamma((m + 3.0) / 2.0);
+    const double p0 = 1.0 / h;
+    const double e0 = std::exp(-pi * pi / (2.0 * h));
+    const double pm = std::pow(pi / h, m / 2.0);
+    const double rd0 = c0 * p0 * e0;
+    f = cm * pm * rd0;
+    const double last = sign;
+    sign = f > eps ? -1.0 : 1.0;
+    if (h < 0.0) sign = 1.0;
+    if (sign != last) step *= 0.1;
+    prev = h;
+    h += sign * step;
+  }
+  return h;
+}
+} // namespace
+
+occxc_status occxc_lmg_radial_grid(double precision, double alpha_max, int l_max,
+                                   const double *alpha_min, double atomic_radius, int capacity,
+                                   double *r, double *w, int *npoints_out) {
+  return guarded([&] {
+    OCCXC_REQUIRE(precision > 0.0 && precision < 1.0 && alpha_max > 0.0 && l_max >= 0 &&
+                      alpha_min && atomic_radius > 0.0 && capacity >= 0 && npoints_out,
+                  OCCXC_ERR_INVALID_ARGUMENT, "lmg_radial_grid: bad arguments");
+    const double r_in = lmg_inner_radius(precision, 2.0 * alpha_max);
+    double h = (double)std::numeric_limits<float>::max();
+    double r_out = 0.0;
+    for (int l = 0; l <= l_max; ++l) {
+      if (alpha_min[l] > 0.0) {
+        r_out = std::max(r_out, lmg_outer_radius(precision, alpha_min[l], l, 4.0 * atomic_radius));
+        h = std::min(h, lmg_step(precision, l, 0.1 * (r_out - r_in)));
+      }
+    }
+    OCCXC_REQUIRE(r_out > r_in && r_out > h && h > 0.0, OCCXC_ERR_INVALID_ARGUMENT,
+                  "lmg_radial_grid: no valid grid for these exponents");
+    const double c = r_in / (std::exp(h) - 1.0);
+    const int n = static_cast<int>(std::log(1.0 + (r_out / c)) / h);
+    *npoints_out = n;
+    if (capacity < n) return;  // size query
+    OCCXC_REQUIRE(r && w, OCCXC_ERR_INVALID_ARGUMENT, "lmg_radial_grid: null output");
+    for (int i = 0; i < n; ++i) {
+      r[i] = c * (std::exp((i + 1) * h) - 1.0);
+      w[i] = (r[i] + c) * h * r[i] * r[i];
     }
   });
 }

@@ -428,3 +428,20 @@ def test_batch_estimate_bytes_tracks_context_footprint():
     real = batch.info()["device_bytes"]
     batch.close()
     assert 0.5 * real <= est.sum() <= 1.5 * real
+
+
+def test_xc_on_the_occ_default_grid():
+    """A build on OCC's default molecular grid (LMG radial rule, NWChem
+    pruning, reduced H grid, Becke partition on the GPU) against the oracle."""
+    from occ_amd.grid import OccGridSettings
+    atoms = molecules.water_cluster(2, seed=1)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = MolecularGrid(atoms, OccGridSettings(), basis=basis).build(device=0)
+    D = random_density(basis.nbf, 10, 31)
+    print(f"occ default grid: {len(wts)} points", flush=True)
+    dft = DFT("pbe", basis, pts, wts)
+    V = dft.compute_vxc(D)
+    K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(basis.flat()), pts, wts,
+                                  dft.ids, D)
+    assert abs(dft.exchange_correlation_energy() - exc) <= TOL_E
+    assert np.abs(V - K).max() <= TOL_V and abs(dft.total_density() - nel) <= TOL_N

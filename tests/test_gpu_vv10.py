@@ -122,3 +122,18 @@ def test_rejections():
                                    ctypes.byref(e), V.ctypes.data_as(_lib._dp))
     assert rc == 1  # OCCXC_ERR_INVALID_ARGUMENT
     batch.close()
+
+
+def test_h2_known_answer_on_the_reference_nlc_grid():
+    """OCC's NLC grid (LMG at 1e-7, at most 110 angular points, NWChem
+    pruning, reduced H grid; Becke partition on the GPU): the reference's test
+    tolerance (Catch::Approx default, 100 float epsilons relative)."""
+    from occ_amd.grid import OccGridSettings
+    basis = AOBasis.load(H2, "sto-3g", pure=False)
+    c = 1.0 / np.sqrt(overlap_matrix(basis).sum())
+    D = np.full((2, 2), c * c)
+    pts, w = MolecularGrid(H2, OccGridSettings.nlc(), basis=basis).build(device=0)
+    e, V, e0, K0 = _compare(basis, pts, w, D)
+    print(f"H2 VV10 on the NLC grid: E={e:.10f} (ORCA 0.008940471) |dE oracle|={abs(e - e0):.2e}")
+    assert abs(e - e0) <= TOL_E and np.abs(V - K0).max() <= TOL_V
+    assert abs(e - 0.008940471) <= 100 * np.finfo(np.float32).eps * 0.008940471

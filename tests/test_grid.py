@@ -108,3 +108,73 @@ def test_hierarchy_invariants_and_identity(rng, n):
     np.testing.assert_allclose(a["center"], b["center"], rtol=1e-15, atol=1e-15)
     np.testing.assert_allclose(a["radius"], b["radius"], rtol=1e-15)
     np.testing.assert_array_equal(a["points"], pts[a["perm"]])
+
+
+# --- OCC's default atom grids: LMG radial rule, NWChem pruning, reduced H/He
+# grid (grid_utils.cpp:234-274, 397-515, 576-758).  What pins them: the LMG
+# rule integrates the Gaussian radial moments it is built for to its
+# precision; every angular count is a Lebedev level, the largest is the
+# requested (or, for H/He, the next lower) level; and the VV10 known answer
+# of tests/test_oracle_vv10.py is met on the NLC grid at the reference's own
+# tolerance.
+
+def _lmg_case(name, atoms, ia, prec):
+    from occ_amd import AOBasis
+    from occ_amd.grid import basis_exponents, lmg_radial_grid
+    b = AOBasis.load(atoms, name, pure=True)
+    amax, lmax, amin = basis_exponents(b, ia)
+    r, w = lmg_radial_grid(atoms[ia][0], prec, amax, lmax, amin)
+    return r, w, amax, amin
+
+
+@pytest.mark.parametrize("ia", [0, 1])
+def test_lmg_integrates_gaussian_moments(ia):
+    from math import gamma
+    atoms = molecules.water()
+    r, w, amax, amin = _lmg_case("def2-svp", atoms, ia, 1e-12)
+    assert np.all(np.diff(r) > 0) and np.all(w > 0)
+    for l, a_lo in enumerate(amin):
+        if a_lo > 1e300:
+            continue
+        for a in np.geomspace(a_lo, amax, 7):
+            exact = gamma(l + 1.5) / (2 * a ** (l + 1.5))
+            num = np.sum(w * r ** (2 * l) * np.exp(-a * r * r))
+            assert num == pytest.approx(exact, rel=1e-9)
+
+
+def test_lmg_precision_controls_size():
+    atoms = molecules.water()
+    r12, _, _, _ = _lmg_case("def2-svp", atoms, 0, 1e-12)
+    r7, _, _, _ = _lmg_case("def2-svp", atoms, 0, 1e-7)
+    assert len(r7) < len(r12)
+
+
+def test_nwchem_pruning_and_reduced_first_row():
+    from occ_amd import AOBasis
+    from occ_amd.grid import (LEBEDEV_LEVELS, OccGridSettings, basis_exponents,
+                              lmg_radial_grid, prune_nwchem, level_below)
+    atoms = molecules.water()
+    b = AOBasis.load(atoms, "def2-svp", pure=True)
+    for ia, top in ((0, 302), (1, level_below(302))):
+        z = atoms[ia][0]
+        r, _ = lmg_radial_grid(z, 1e-12, *basis_exponents(b, ia))
+        nang = prune_nwchem(z, top, r)
+        assert set(nang) <= set(LEBEDEV_LEVELS) and max(nang) == top
+        assert nang[0] == 50 and nang[-1] < top  # innermost and outermost shells pruned
+    assert level_below(302) == 266 and level_below(110) == 86
+    assert OccGridSettings.nlc().max_angular_points == 110
+
+
+def test_occ_grid_integrates_density_volume():
+    """The OCC default grid of water integrates a normalised Gaussian
+    density per atom to 1 (partitioned, numpy Becke)."""
+    from occ_amd import AOBasis
+    from occ_amd.grid import OccGridSettings
+    atoms = molecules.water()
+    b = AOBasis.load(atoms, "def2-svp", pure=True)
+    g = MolecularGrid(atoms, OccGridSettings(), basis=b)
+    pts, w = grid_ref.molecular_grid(g)
+    for (_, x, y, z) in atoms:
+        d2 = ((pts - np.array([x, y, z])) ** 2).sum(1)
+        val = np.sum(w * (2.0 / np.pi) ** 1.5 * np.exp(-2.0 * d2))
+        assert val == pytest.approx(1.0, abs=1e-6)  # pruned grid: quadrature error ~1e-7

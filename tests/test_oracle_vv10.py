@@ -87,3 +87,18 @@ def test_forward_and_reverse_potentials_average_to_the_derivative():
     fd = (ep - em) / (2 * eps)
     assert fd == pytest.approx(2 * np.sum(0.5 * (Kf + Kr) * Delta), rel=1e-6, abs=1e-12)
     np.testing.assert_allclose(Kf, Kf.T, atol=1e-15)
+
+
+def test_h2_known_answer_on_the_reference_nlc_grid():
+    """On OCC's NLC grid (set_integration_grid's settings: LMG at 1e-7, at most
+    110 angular points, NWChem pruning, reduced H grid) the energy meets the
+    reference's test at its own tolerance (Catch::Approx default: 100 float
+    epsilons relative)."""
+    from occ_amd.grid import OccGridSettings
+    basis = AOBasis.load(H2, "sto-3g", pure=False)
+    c = 1.0 / np.sqrt(overlap_matrix(basis).sum())
+    D = np.full((2, 2), c * c)
+    g = MolecularGrid(H2, OccGridSettings.nlc(), basis=basis)
+    pts, w = grid_ref.molecular_grid(g)
+    e, _ = oracle.vv10(oracle.OracleBasis.from_flat(basis.flat()), pts, w, D)
+    assert abs(e - 0.008940471) <= 100 * np.finfo(np.float32).eps * 0.008940471
