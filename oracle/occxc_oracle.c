@@ -1187,6 +1187,8 @@ static void process_block_gradient(const block_cfg *cfg, int m, long n,
   double *exc = rho + 4 * n, *vrho = exc + n, *vsig = vrho + n, *sig = vsig + n;
   double *A = sig + n;             /* [m][n] */
   double *V = A + mn;              /* 3 x [m x m], column-major */
+  double *tau = V + 3L * m * m, *vtau = tau + n; /* meta-GGA */
+  const int mgga = cfg->deriv == 2;
 
   /* evaluate_density<d,R>: T = phi * Dloc.selfadjointView<Upper>() */
   memset(T, 0, sizeof(double) * mn);
@@ -1212,6 +1214,28 @@ static void process_block_gradient(const block_cfg *cfg, int m, long n,
       }
     }
   }
+  /* meta-GGA: tau = (1/2) sum_j rowsum((d_j phi Dloc) o d_j phi) (density.cpp:94-101) */
+  if (mgga) {
+    memset(tau, 0, sizeof(double) * n);
+    const double *dj[3] = {px, py, pz};
+    for (int j = 0; j < 3; j++) {
+      memset(T, 0, sizeof(double) * mn);
+      for (int nu = 0; nu < m; nu++) {
+        double *t = T + (long)nu * n;
+        for (int mu = 0; mu < m; mu++) {
+          int r = mu < nu ? mu : nu, c = mu < nu ? nu : mu;
+          double dm = Dloc[r + (long)c * m];
+          const double *f = dj[j] + (long)mu * n;
+          for (long p = 0; p < n; p++) t[p] += dm * f[p];
+        }
+      }
+      for (int nu = 0; nu < m; nu++) {
+        const double *t = T + (long)nu * n, *f = dj[j] + (long)nu * n;
+        for (long p = 0; p < n; p++) tau[p] += f[p] * t[p];
+      }
+    }
+    for (long p = 0; p < n; p++) tau[p] *= 0.5;
+  }
   /* dft_gradient_kernels.cpp:488-491: skip when max rho < threshold */
   double rmax = -1e300;
   for (long p = 0; p < n; p++)
@@ -1222,11 +1246,18 @@ static void process_block_gradient(const block_cfg *cfg, int m, long n,
     sig[p] = rho[n + p] * rho[n + p] + rho[2 * n + p] * rho[2 * n + p] +
              rho[3 * n + p] * rho[3 * n + p];
   }
-  for (int f = 0; f < cfg->nfunc; f++)
-    oracle_functional_eval(cfg->ids[f], n, rho, sig, exc, vrho, vsig);
+  if (mgga) {
+    for (long p = 0; p < n; p++) vtau[p] = 0.0;
+    for (int f = 0; f < cfg->nfunc; f++)
+      oracle_functional_eval_mgga(cfg->ids[f], n, rho, sig, tau, exc, vrho, vsig, vtau);
+  } else {
+    for (int f = 0; f < cfg->nfunc; f++)
+      oracle_functional_eval(cfg->ids[f], n, rho, sig, exc, vrho, vsig);
+  }
   for (long p = 0; p < n; p++) { /* Result::weight_by */
     vrho[p] *= wts[p];
     vsig[p] *= wts[p];
+    if (mgga) vtau[p] *= wts[p];
   }
   double *Vx = V, *Vy = V + (long)m * m, *Vz = V + 2L * m * m;
   memset(V, 0, sizeof(double) * 3 * m * m);
@@ -1302,6 +1333,28 @@ static void process_block_gradient(const block_cfg *cfg, int m, long n,
         }
       }
     }
+    /* accumulate_gradient_contribution_R<2> (dft_gradient_kernels.cpp:
+     * 152-172): aow_j = (vtau / 2) d_j phi, V_k += d_kj phi^T aow_j */
+    if (mgga) {
+      for (int j = 0; j < 3; j++) {
+        for (int nu = 0; nu < m; nu++) {
+          const long o = (long)nu * n;
+          for (long p = 0; p < n; p++) A[o + p] = 0.5 * vtau[p] * dk[j][o + p];
+        }
+        for (int k = 0; k < 3; k++) {
+          const double *h = hk[k][j];
+          for (int nu = 0; nu < m; nu++) {
+            const double *a = A + (long)nu * n;
+            for (int mu = 0; mu < m; mu++) {
+              const double *hm = h + (long)mu * n;
+              double s = 0;
+              for (long p = 0; p < n; p++) s += hm[p] * a[p];
+              Vk[k][mu + (long)nu * m] += s;
+            }
+          }
+        }
+      }
+    }
   }
   /* dft_gradient_kernels.cpp:533-550: distribute with 2 D(mu, nu), skipping
    * |2 D| < 1e-12 */
@@ -1326,7 +1379,7 @@ int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
   int deriv = 0;
   for (int f = 0; f < nfunc; f++) {
     int fam = oracle_functional_family(func_ids[f]);
-    if (fam < 0 || fam > 1) return -1;
+    if (fam < 0) return -1;
     if (fam > deriv) deriv = fam;
   }
   const int nsh = b->nshell;
@@ -1386,7 +1439,7 @@ int oracle_xc_gradient(const oracle_basis *b, const int *shell_atom, int natoms,
     double *phi = malloc(sizeof(double) * 10 * (size_t)nbf * maxn);
     double *tmp = malloc(sizeof(double) * 10 * (ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2 * maxn);
     double *Dloc = malloc(sizeof(double) * (size_t)nbf * nbf);
-    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 8 * maxn +
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 10 * maxn +
                                             3 * (size_t)nbf * nbf));
 #pragma omp for schedule(dynamic, 4)
     for (long li = 0; li < nblk; li++) {
