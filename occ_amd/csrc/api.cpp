@@ -812,32 +812,35 @@ void collect_timing(occxc_context *c) {
 // derivative plane in its LDA form with the tau coefficients.  The laplacian (density.cpp:88-93)
 // is not formed: no functional here depends on it and the reference's Vxc
 // does not use vlaplacian.
+// The meta-GGA density stage: rho and grad rho, the three per-direction tau
+// passes, then the per-leaf functional stage (coefficients in a.coef and
+// c->d_coef_tau, {Exc, N_elec} per leaf, leaf skip flags).
+static void mgga_density(occxc_context *c, hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const int prec = c->opt.precision;
+  BuildArgs x = a;
+  x.rho_out = c->d_mrho.get();
+  run_density_store(x, c->d_ao.get(), prec, c->d_D2.get(), s);
+  for (int k = 0; k < 3; ++k) {
+    BuildArgs y = a;
+    y.deriv = 0;
+    y.leaves = c->d_leaves_k.get() + (size_t)k * c->nleaves;
+    y.rho_out = c->d_mtau.get() + (size_t)k * c->ptot;
+    run_density_store(y, c->d_ao.get(), prec, c->d_D2.get(), s);
+  }
+  launch_functional_mgga_leaves(a, c->d_mrho.get(), c->d_mtau.get(), c->d_coef_tau.get(), s);
+}
+
 static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, double *d_result,
                            hipStream_t s) {
   const BuildArgs &a = c->args;
   const int prec = c->opt.precision;
   StageTimer t{c, s, {}};
-  auto plane = [&](int k) {
-    BuildArgs y = a;
-    y.deriv = 0;
-    y.leaves = c->d_leaves_k.get() + (size_t)k * c->nleaves;
-    return y;
-  };
   t.run(OCCXC_STAGE_PREPARE, [&] {
     launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
     launch_zero(c->d_K.get(), c->mtot, s);
   });
-  t.run(OCCXC_STAGE_DENSITY, [&] {
-    BuildArgs x = a;
-    x.rho_out = c->d_mrho.get();
-    run_density_store(x, c->d_ao.get(), prec, c->d_D2.get(), s);
-    for (int k = 0; k < 3; ++k) {
-      BuildArgs y = plane(k);
-      y.rho_out = c->d_mtau.get() + (size_t)k * c->ptot;
-      run_density_store(y, c->d_ao.get(), prec, c->d_D2.get(), s);
-    }
-    launch_functional_mgga_leaves(a, c->d_mrho.get(), c->d_mtau.get(), c->d_coef_tau.get(), s);
-  });
+  t.run(OCCXC_STAGE_DENSITY, [&] { mgga_density(c, s); });
   t.run(OCCXC_STAGE_VXC, [&] {
     BuildArgs x = a;
     x.coef_tau = c->d_coef_tau.get();
@@ -942,6 +945,10 @@ void gradient_setup(occxc_context *c, int natoms, const int *shell_atom) {
     c->d_ao2.alloc(std::max<size_t>(n, 1));
     c->d_gpart.alloc(std::max<size_t>(3 * c->d_ao_map.size(), 1));
     c->device_bytes += c->d_ao2.bytes() + c->d_gpart.bytes();
+    if (c->mgga) {
+      c->d_gpart2.alloc(c->d_gpart.size());
+      c->device_bytes += c->d_gpart2.bytes();
+    }
     StageTimer t{c, c->stream, {}};
     t.run(OCCXC_STAGE_AO,
           [&] { launch_ao_extra(c->args, c->d_ao2.get(), c->deriv != 0, c->stream); });
@@ -980,6 +987,21 @@ void run_gradient(occxc_context *c, const double *d_D, double *d_grad, hipStream
   t.run(OCCXC_STAGE_PREPARE, [&] {
     launch_prepare_density(d_D, c->d_D2.get(), c->d_mols.get(), c->nmol, c->maxnbf, s);
   });
+  if (c->mgga) {
+    // process_grid_block_gradient<2, Restricted>: the meta-GGA density stage,
+    // the GGA terms and the tau terms as two partial sets, one reduction
+    const bool fp32 = c->opt.precision != OCCXC_PREC_FP64;
+    t.run(OCCXC_STAGE_DENSITY, [&] { mgga_density(c, s); });
+    t.run(OCCXC_STAGE_GRADIENT, [&] {
+      launch_xc_gradient_partials(a, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart.get(), s);
+      BuildArgs x = a;
+      x.coef_tau = c->d_coef_tau.get();
+      launch_xc_gradient_tau(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart2.get(), s);
+      launch_grad_reduce(c->d_gpart.get(), c->d_gpart2.get(), c->grad_natoms,
+                         c->d_gatom_off.get(), c->d_gatom_ent.get(), d_grad, s);
+    });
+    return;
+  }
   t.run(OCCXC_STAGE_DENSITY, [&] {
     if (c->plan.use_mfma)
       run_density(c->plan, a, c->d_ao.get(), c->opt.precision, s);
@@ -1327,8 +1349,6 @@ occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
                                const int *shell_atom, double *grad) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
-    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
@@ -1353,8 +1373,6 @@ occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int
   return guarded([&] {
     OCCXC_REQUIRE(ctx && d_D && shell_atom && d_grad, OCCXC_ERR_INVALID_ARGUMENT,
                   "null pointer");
-    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
@@ -1373,7 +1391,7 @@ occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, doubl
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && Vxc && exc && nelec, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
+                  "meta-GGA functionals: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_build_unrestricted needs a single-molecule context");
     OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
@@ -1409,7 +1427,7 @@ occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d
   return guarded([&] {
     OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
+                  "meta-GGA functionals: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_build_unrestricted needs a single-molecule context");
     OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
@@ -1441,7 +1459,7 @@ occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D,
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted builds only (no gradient, no unrestricted)");
+                  "meta-GGA functionals: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");

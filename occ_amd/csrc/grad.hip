@@ -186,6 +186,104 @@ __global__ __launch_bounds__(256, 2) void k_xc_grad(BuildArgs a, const S *__rest
   }
 }
 
+// Meta-GGA tau terms (accumulate_gradient_contribution_R<2>,
+// dft_gradient_kernels.cpp:152-172): V_k += sum_j d_kj phi^T ((vtau / 2) d_j phi)
+// contracted with P, i.e. per AO mu and direction k
+//   sum_p sum_j d_kj phi_mu (vtau / 2) T_j,mu,   T_j = (d_j Phi) P,
+// with vtau / 2 = 2 c_tau (c_tau = w vtau / 4, a.coef_tau), into gpart in the
+// normalisation of k_xc_grad's partials.  Same leaf / chunk / point-block
+// structure as k_xc_grad: the three T_j of a chunk and point block on the
+// FP64 matrix cores (A operand d_j phi from the AO cache, B operand P
+// gathered through the AO map), the Hessian from ao2 in the integrand.
+template <typename S>
+__global__ __launch_bounds__(256, 2) void k_xc_grad_tau(BuildArgs a, const S *__restrict__ ao,
+                                                       const double *__restrict__ ao2,
+                                                       double *__restrict__ gpart) {
+  __shared__ double s_t[kMaxLeafPts];
+  const int leaf = xcd_leaf(blockIdx.x, gridDim.x);
+  const LeafDev L = a.leaves[leaf];
+  double *gp = gpart + 3LL * L.map_off;
+  if (!a.leaf_active[leaf]) {
+    for (int i = threadIdx.x; i < 3 * L.mp; i += blockDim.x) gp[i] = 0.0;
+    return;
+  }
+  for (int i = threadIdx.x; i < L.np; i += blockDim.x)
+    s_t[i] = 2.0 * a.coef_tau[L.pt_off + i];
+  __syncthreads();
+  const MolDev MD = a.mols[L.mol];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int pa = 4 * (l16 & 3) + (l16 >> 2);
+  const long long cs = (long long)L.mp * L.np;
+  const S *phi = ao + L.ao_off;
+  const double *x2 = ao2 + L.ao_off / a.ncomp * 6;
+  const double *Dm = a.D2 + MD.moff;
+  const int *map = a.ao_map + L.map_off;
+  for (int c = 32 * wave; c < L.mp; c += 128) {
+    const int gn0 = map[c + l16], gn1 = map[c + 16 + l16];
+    double gs[2][3];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) gs[j][0] = gs[j][1] = gs[j][2] = 0.0;
+    for (int p0 = 0; p0 < L.np; p0 += 32) {
+      d4 acc[3][2][2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[d][i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int mc = 0; mc < L.mp; mc += 16) {
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+          const int mu = mc + 4 * g + s4;
+          const long long ro = (long long)mu * L.np + p0 + pa;
+          const long long col = (long long)map[mu] * MD.nbf;
+          const double d0 = Dm[col + gn0], d1 = Dm[col + gn1];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double f0 = (double)phi[(d + 1) * cs + ro];
+            const double f1 = (double)phi[(d + 1) * cs + ro + 16];
+            acc[d][0][0] = mma(f0, d0, acc[d][0][0]);
+            acc[d][0][1] = mma(f0, d1, acc[d][0][1]);
+            acc[d][1][0] = mma(f1, d0, acc[d][1][0]);
+            acc[d][1][1] = mma(f1, d1, acc[d][1][1]);
+          }
+        }
+      }
+#pragma unroll
+      for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+          const int pb = p0 + pt * 16 + 4 * g;
+          const long long off = (long long)(c + nt * 16 + l16) * L.np + pb;
+          const d4 hxx = load4d<double>(x2 + off), hxy = load4d<double>(x2 + cs + off),
+                   hxz = load4d<double>(x2 + 2 * cs + off),
+                   hyy = load4d<double>(x2 + 3 * cs + off),
+                   hyz = load4d<double>(x2 + 4 * cs + off),
+                   hzz = load4d<double>(x2 + 5 * cs + off);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double ct = s_t[pb + r];
+            const double tx = ct * acc[0][pt][nt][r], ty = ct * acc[1][pt][nt][r],
+                         tz = ct * acc[2][pt][nt][r];
+            gs[nt][0] += hxx[r] * tx + hxy[r] * ty + hxz[r] * tz;
+            gs[nt][1] += hxy[r] * tx + hyy[r] * ty + hyz[r] * tz;
+            gs[nt][2] += hxz[r] * tx + hyz[r] * ty + hzz[r] * tz;
+          }
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double v = gs[nt][k];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (g == 0) gp[3 * (c + nt * 16 + l16) + k] = v;
+      }
+  }
+}
+
 // grad[3 A + k] = -2 sum of gpart (plus gpart2, the beta spin of an
 // unrestricted gradient, when given) over the entries of atom A, in a fixed
 // order (per-thread strided sums, then a fixed tree).
@@ -242,6 +340,16 @@ void launch_xc_gradient_partials(const BuildArgs &a, const void *ao, bool ao_fp3
     else k_xc_grad<double, false><<<a.nleaves, 256, 0, s>>>(a, p, ao2, gpart);
   }
   check("k_xc_grad");
+}
+
+void launch_xc_gradient_tau(const BuildArgs &a, const void *ao, bool ao_fp32,
+                            const double *ao2, double *gpart, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  if (ao_fp32)
+    k_xc_grad_tau<float><<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao), ao2, gpart);
+  else
+    k_xc_grad_tau<double><<<a.nleaves, 256, 0, s>>>(a, static_cast<const double *>(ao), ao2, gpart);
+  check("k_xc_grad_tau");
 }
 
 void launch_grad_reduce(const double *gpart, const double *gpart2, int natoms,

@@ -161,6 +161,10 @@ void launch_ao_extra(const BuildArgs &a, double *out, bool hessian, hipStream_t 
 // a.coef of that spin), then the per-atom reduction of one or two partial sets
 void launch_xc_gradient_partials(const BuildArgs &a, const void *ao, bool ao_fp32,
                                  const double *ao2, double *gpart, hipStream_t s);
+// restricted meta-GGA: the tau terms' per-AO partials (a.coef_tau, Hessian in
+// ao2) into gpart, reduced together with k_xc_grad's by launch_grad_reduce
+void launch_xc_gradient_tau(const BuildArgs &a, const void *ao, bool ao_fp32,
+                            const double *ao2, double *gpart, hipStream_t s);
 void launch_grad_reduce(const double *gpart, const double *gpart2, int natoms,
                         const int *atom_off, const int *atom_ent, double *grad, hipStream_t s);
 void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const double *ao2,

@@ -185,3 +185,44 @@ def test_unrestricted_gradient_matches_oracle(method):
         Gr = dft.compute_xc_gradient(D)
         Gc = dft.compute_xc_gradient(np.vstack([D / 2, D / 2]))
         assert np.abs(Gc - Gr).max() <= 1e-11
+
+
+@pytest.mark.parametrize("case", ["water", "dimer", "fp32cache"])
+def test_tpss_gradient_matches_oracle(case):
+    """Meta-GGA (TPSS) restricted gradient: the GGA terms and the tau terms
+    of accumulate_gradient_contribution_R<2> (dft_gradient_kernels.cpp:95-172)
+    against the oracle (itself pinned by finite differences of Exc)."""
+    if case == "water":
+        G, G0, _, _ = _case(molecules.water(), "6-31g", "tpss", 41, pure=False)
+        tol = TOL_G
+    elif case == "dimer":
+        G, G0, _, _ = _case(molecules.water_cluster(2, seed=1), "def2-svp", "tpss", 42, pure=True)
+        tol = TOL_G
+    else:
+        G, G0, _, _ = _case(molecules.water_cluster(2, seed=1), "def2-svp", "tpss", 43, pure=True,
+                            precision="fp32cache")
+        tol = 1e-5
+    err = np.abs(G - G0).max()
+    print(f"TPSS gradient {case}: max|dG|={err:.2e} max|G|={np.abs(G0).max():.3f}")
+    assert err <= tol
+
+
+def test_tpss_gradient_device_entry_and_build_after():
+    """The device entry point gives the host one's gradient, and a build after
+    a gradient on the same context is unchanged."""
+    from occ_amd.device import DeviceArray
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 5, 44)
+    ctx = XCContext(basis, pts, wts, functional_ids("tpss"))
+    V0, e0, _ = ctx.build(D)
+    G = ctx.xc_gradient(D, basis.shell_atom, 3)
+    dD, dG = DeviceArray(basis.nbf ** 2), DeviceArray(9)
+    dD.upload(np.asfortranarray(D))
+    ctx.xc_gradient_device(dD.ptr, basis.shell_atom, 3, dG.ptr)
+    ctx.synchronize()
+    assert np.abs(dG.download((3, 3)) - G).max() <= 1e-14
+    V1, e1, _ = ctx.build(D)
+    assert abs(e1 - e0) <= 1e-12 and np.abs(V1 - V0).max() <= 1e-14
+    ctx.close()

@@ -147,6 +147,6 @@ def test_mgga_rejections():
     dft = DFT("tpss", basis, pts, wts)
     D = random_density(basis.nbf, 5, 3)
     with pytest.raises(Exception):
-        dft.compute_xc_gradient(D)
+        dft.compute_xc_gradient(np.vstack([D, D]) / 2)  # unrestricted
     with pytest.raises(Exception):
         dft.compute_vxc(np.vstack([D, D]) / 2)
