@@ -117,7 +117,9 @@ def main():
 
     if args.workload == "batch512":
         return run_batch(args)
-    if args.workload == "grad32":
+    if args.workload in ("grad32", "tpssgrad32"):
+        if args.workload == "tpssgrad32":
+            args.modes = ",".join(m for m in args.modes.split(",") if m in ("fp64", "fp32cache"))
         return run_gradient(args)
     if args.workload == "uks32":
         return run_unrestricted(args)
@@ -276,7 +278,8 @@ def run_gradient(args):
     from occ_amd.dft import XCContext, functional_ids
     from occ_amd.device import synchronize
     from occ_amd.grid import GridSettings, MolecularGrid
-    atoms, bname, pure, method, nocc, desc = workload("h2o32")
+    atoms, bname, pure, method, nocc, desc = workload(
+        "tpss32" if args.workload == "tpssgrad32" else "h2o32")
     basis = AOBasis.load(atoms, bname, pure=pure)
     nbf = basis.nbf
     grid = MolecularGrid(atoms, GridSettings(radial_points=76, angular_points=302))
@@ -340,11 +343,15 @@ def run_gradient(args):
     tim = R["timing"]
     gms = tim["gradient"][0] / max(tim["gradient"][1], 1)
     flops = R["info"]["contraction_flops"]   # T = Phi P and U = Aow P: 2 x 2 n m^2
+    mg = args.workload == "tpssgrad32"
+    if mg:
+        flops *= 2.5   # + T_x, T_y, T_z of the tau terms: 5 x 2 n m^2
     achieved = flops / (gms * 1e-3) / 1e12
     roof = {"bound": "mfma", "kernel": "gradient", "achieved": achieved,
             "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
-            "traffic": measured_traffic("grad32", head, "gradient"), "kernel_ms": gms,
-            "work": "sum over leaves of 4 n_b m_b^2 (T = Phi P and U = Aow P, screened m_b)"}
+            "traffic": measured_traffic(args.workload, head, "gradient"), "kernel_ms": gms,
+            "work": "sum over leaves of 4 n_b m_b^2 (T = Phi P and U = Aow P, screened m_b)"
+                    + (" + 6 n_b m_b^2 (the three tau GEMMs)" if mg else "")}
     out = {
         "metric": "XC nuclear gradients/s (FP64)",
         "value": R["per_s"], "unit": "XC gradients/s", "n_gpus": 1, "steps": args.steps,
