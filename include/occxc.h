@@ -169,8 +169,10 @@ occxc_status occxc_build_device(occxc_context *ctx, const double *d_D,
  * the 3 x natoms column-major Mat3N (x, y, z of atom 0, then atom 1, ...).
  * The gradient is that of Exc with the grid and weights held fixed, as in the
  * reference.  The first call evaluates and keeps the AO derivative components
- * the build does not need (the Hessian for GGA, the gradient for LDA).
- * Single-molecule contexts only; repeats bit for bit. */
+ * the build does not need (the Hessian for GGA and meta-GGA, the gradient for
+ * LDA).  Meta-GGA ids add the tau terms of accumulate_gradient_contribution_R
+ * <2> (dft_gradient_kernels.cpp:95-172).  Single-molecule contexts only;
+ * repeats bit for bit. */
 occxc_status occxc_xc_gradient(occxc_context *ctx, const double *D, int natoms,
                                const int *shell_atom, double *grad);
 
