@@ -852,6 +852,118 @@ int oracle_functional_eval_mgga(int id, long n, const double *rho, const double 
   return 0;
 }
 
+/* Spin-polarised TPSS (the same paper): exchange by exact spin scaling,
+ * E_x[na, nb] = E_x[2 na]/2 + E_x[2 nb]/2 with (2 n_s, 4 s_ss, 2 t_s); the
+ * correlation's eqs. 11-14 in full, with zeta = (na - nb)/n,
+ * xi = |grad zeta| / (2 (3 pi^2 n)^(1/3)), |grad zeta|^2 = 4 (nb^2 s_aa
+ * - 2 na nb s_ab + na^2 s_bb) / n^4, C(zeta, xi) = (0.53 + 0.87 zeta^2
+ * + 0.50 zeta^4 + 2.26 zeta^6) / (1 + xi^2 ((1+zeta)^(-4/3) + (1-zeta)^(-4/3))/2)^4,
+ * the spin channels' eps~_s = max(eps_PBE(n_s, 0), eps_PBE) weighted n_s / n,
+ * and z = tau_W / tau of the total density.  Inputs limited per spin
+ * (tau_s >= 1e-20, s_ss <= 8 n_s tau_s). */
+static cplx e_tpss_x_pol(cplx ra, cplx rb, cplx saa, cplx sbb, cplx ta, cplx tb) {
+  cplx e = 0.0;
+  if (creal(ra) >= ORACLE_DENS_THR) e += 0.5 * e_tpss_x(2.0 * ra, 4.0 * saa, 2.0 * ta);
+  if (creal(rb) >= ORACLE_DENS_THR) e += 0.5 * e_tpss_x(2.0 * rb, 4.0 * sbb, 2.0 * tb);
+  return e;
+}
+
+static cplx e_tpss_c_pol(cplx ra, cplx rb, cplx saa, cplx sab, cplx sbb, cplx ta, cplx tb) {
+  const double d = 2.8;
+  cplx n = ra + rb, zeta = (ra - rb) / n;
+  cplx sig = saa + 2.0 * sab + sbb;
+  cplx epbe = e_pbe_c_pol(ra, rb, sig) / n;
+  cplx par = 0.0;
+  if (creal(ra) >= ORACLE_DENS_THR) {
+    cplx ea = e_pbe_c_pol(ra, 0.0, saa) / ra;
+    par += ra / n * (creal(ea) > creal(epbe) ? ea : epbe);
+  }
+  if (creal(rb) >= ORACLE_DENS_THR) {
+    cplx eb = e_pbe_c_pol(rb, 0.0, sbb) / rb;
+    par += rb / n * (creal(eb) > creal(epbe) ? eb : epbe);
+  }
+  cplx z = sig / (8.0 * n * (ta + tb));
+  if (creal(z) > 1.0) z = 1.0;
+  cplx gz2 = 4.0 * (rb * rb * saa - 2.0 * ra * rb * sab + ra * ra * sbb) / (n * n * n * n);
+  cplx xi2 = gz2 / (4.0 * cpow(3.0 * M_PI * M_PI * n, 2.0 / 3.0));
+  cplx q = zeta * zeta;
+  cplx c0 = 0.53 + 0.87 * q + 0.50 * q * q + 2.26 * q * q * q;
+  cplx den = 1.0 + xi2 * (opz_pow(1.0 + zeta, -4.0 / 3.0) + opz_pow(1.0 - zeta, -4.0 / 3.0)) / 2.0;
+  cplx C = c0 / (den * den * den * den);
+  cplx z2 = z * z;
+  cplx f0 = (1.0 + C * z2) * epbe - (1.0 + C) * z2 * par;
+  return n * f0 * (1.0 + d * f0 * z2 * z);
+}
+
+/* x = (ra, rb, saa, sab, sbb, ta, tb) */
+static cplx edens_mgga_pol(int id, const cplx *x) {
+  switch (id) {
+  case ORACLE_MGGA_X_TPSS:
+    return e_tpss_x_pol(x[0], x[1], x[2], x[4], x[5], x[6]);
+  case ORACLE_MGGA_C_TPSS:
+    return e_tpss_c_pol(x[0], x[1], x[2], x[3], x[4], x[5], x[6]);
+  }
+  return edens_pol(id, x[0], x[1], x[2], x[3], x[4]);
+}
+
+static void mgga_inputs_pol(double *x) {
+  for (int s = 0; s < 2; s++) {
+    double *sg = x + 2 + 2 * s, *t = x + 5 + s;
+    if (*sg < ORACLE_SIGMA_MIN) *sg = ORACLE_SIGMA_MIN;
+    if (*t < ORACLE_TAU_MIN) *t = ORACLE_TAU_MIN;
+    if (*sg > 8.0 * x[s] * *t) *sg = 8.0 * x[s] * *t;
+  }
+}
+
+double oracle_functional_energy_density_mgga_pol(int id, const double *in) {
+  double x[7];
+  memcpy(x, in, sizeof(x));
+  if (x[0] + x[1] < ORACLE_DENS_THR) return 0.0;
+  mgga_inputs_pol(x);
+  cplx y[7];
+  for (int k = 0; k < 7; k++) y[k] = x[k];
+  return creal(edens_mgga_pol(id, y));
+}
+
+int oracle_functional_eval_mgga_pol(int id, long n, const double *rho, const double *sigma,
+                                    const double *tau, double *exc, double *vrho,
+                                    double *vsigma, double *vtau) {
+  int fam = oracle_functional_family(id);
+  if (fam < 0) return -1;
+  for (long i = 0; i < n; i++) {
+    double x[7] = {rho[2 * i], rho[2 * i + 1], 0.0, 0.0, 0.0, 1.0, 1.0};
+    double r = x[0] + x[1];
+    if (!(r >= ORACLE_DENS_THR)) continue;
+    if (fam >= 1) {
+      x[2] = sigma[3 * i];
+      x[3] = sigma[3 * i + 1];
+      x[4] = sigma[3 * i + 2];
+    }
+    if (fam == 2) {
+      x[5] = tau[2 * i];
+      x[6] = tau[2 * i + 1];
+      mgga_inputs_pol(x);
+    } else if (fam == 1) {
+      if (x[2] < ORACLE_SIGMA_MIN) x[2] = ORACLE_SIGMA_MIN;
+      if (x[4] < ORACLE_SIGMA_MIN) x[4] = ORACLE_SIGMA_MIN;
+    }
+    cplx y[7];
+    for (int k = 0; k < 7; k++) y[k] = x[k];
+    exc[i] += creal(edens_mgga_pol(id, y)) / r;
+    const int nv = fam == 2 ? 7 : (fam == 1 ? 5 : 2);
+    for (int k = 0; k < nv; k++) {
+      double h = 1e-20 * (fabs(x[k]) > 1e-300 ? fabs(x[k]) : 1.0);
+      for (int q = 0; q < 7; q++) y[q] = x[q];
+      y[k] += I * h;
+      double dv = cimag(edens_mgga_pol(id, y)) / h;
+      if (k < 2) vrho[2 * i + k] += dv;
+      else if (k < 5) vsigma[3 * i + k - 2] += dv;
+      else vtau[2 * i + k - 5] += dv;
+    }
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------- XC build -- */
 
 typedef struct {
@@ -1527,6 +1639,8 @@ static void process_block_u(const block_cfg *cfg, int m, long n, const double *p
   double *rr = vs + 3 * n;            /* [2n] */
   double *ss = rr + 2 * n;            /* [3n] */
   double *G = ss + 3 * n;             /* [m][n] */
+  double *tt = G + mn, *vt = tt + 2 * n; /* meta-GGA: tau_s, vtau_s interleaved a,b */
+  const int mgga = cfg->deriv == 2;
   const int ld = 2 * m;
   memset(rho, 0, sizeof(double) * 8 * n);
   for (int sp = 0; sp < 2; sp++) {
@@ -1553,6 +1667,29 @@ static void process_block_u(const block_cfg *cfg, int m, long n, const double *p
       }
     }
   }
+  /* evaluate_density<2, U> (density.cpp:104-166): tau_s = (1/2) sum_k
+   * rowsum((d_k phi D_s) o d_k phi) per spin block */
+  if (mgga) {
+    memset(tt, 0, sizeof(double) * 2 * n);
+    const double *dk3[3] = {px, py, pz};
+    for (int sp = 0; sp < 2; sp++)
+      for (int k = 0; k < 3; k++) {
+        memset(T, 0, sizeof(double) * mn);
+        for (int nu = 0; nu < m; nu++) {
+          double *t = T + (long)nu * n;
+          for (int mu = 0; mu < m; mu++) {
+            const double dm = Dloc[sp * m + mu + (long)nu * ld];
+            const double *f = dk3[k] + (long)mu * n;
+            for (long p = 0; p < n; p++) t[p] += dm * f[p];
+          }
+        }
+        for (int nu = 0; nu < m; nu++) {
+          const double *t = T + (long)nu * n, *f = dk3[k] + (long)nu * n;
+          for (long p = 0; p < n; p++) tt[2 * p + sp] += f[p] * t[p];
+        }
+      }
+    for (long p = 0; p < 2 * n; p++) tt[p] *= 0.5;
+  }
   const double *ra = rho, *rb = rho + 4 * n;
   double na = 0.0, nb = 0.0, rmax = -1e300;
   for (long p = 0; p < n; p++) {
@@ -1574,12 +1711,19 @@ static void process_block_u(const block_cfg *cfg, int m, long n, const double *p
     ss[3 * p + 1] = ra[n + p] * rb[n + p] + ra[2 * n + p] * rb[2 * n + p] + ra[3 * n + p] * rb[3 * n + p];
     ss[3 * p + 2] = rb[n + p] * rb[n + p] + rb[2 * n + p] * rb[2 * n + p] + rb[3 * n + p] * rb[3 * n + p];
   }
-  for (int f = 0; f < cfg->nfunc; f++) oracle_functional_eval_pol(cfg->ids[f], n, rr, ss, exc, vr, vs);
+  if (mgga) {
+    for (long p = 0; p < 2 * n; p++) vt[p] = 0.0;
+    for (int f = 0; f < cfg->nfunc; f++)
+      oracle_functional_eval_mgga_pol(cfg->ids[f], n, rr, ss, tt, exc, vr, vs, vt);
+  } else {
+    for (int f = 0; f < cfg->nfunc; f++) oracle_functional_eval_pol(cfg->ids[f], n, rr, ss, exc, vr, vs);
+  }
   double e = 0.0;
   for (long p = 0; p < n; p++) { /* Result::weight_by, then exc . (rho_a + rho_b) */
     exc[p] *= wts[p];
     for (int k = 0; k < 2; k++) vr[2 * p + k] *= wts[p];
     for (int k = 0; k < 3; k++) vs[3 * p + k] *= wts[p];
+    if (mgga) for (int k = 0; k < 2; k++) vt[2 * p + k] *= wts[p];
     e += exc[p] * ra[p] + exc[p] * rb[p];
   }
   *energy += e;
@@ -1618,6 +1762,21 @@ static void process_block_u(const block_cfg *cfg, int m, long n, const double *p
         Kblk[sp * m + nu + (long)mu * ld] += s;
       }
     }
+    /* xc_potential_matrix<Unrestricted, 2> (xc_potential_matrix.cpp:111-137):
+     * V_s += sum_k d_k phi^T (vtau_s / 2 o d_k phi) */
+    if (mgga) {
+      const double *dk3[3] = {px, py, pz};
+      for (int k = 0; k < 3; k++)
+        for (int nu = 0; nu < m; nu++) {
+          const double *fn = dk3[k] + (long)nu * n;
+          for (int mu = 0; mu < m; mu++) {
+            const double *fm = dk3[k] + (long)mu * n;
+            double s = 0.0;
+            for (long p = 0; p < n; p++) s += fm[p] * (0.5 * vt[2 * p + sp]) * fn[p];
+            Kblk[sp * m + mu + (long)nu * ld] += s;
+          }
+        }
+    }
   }
 }
 
@@ -1630,7 +1789,7 @@ int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double 
   int deriv = 0;
   for (int f = 0; f < nfunc; f++) {
     int fam = oracle_functional_family(func_ids[f]);
-    if (fam < 0 || fam > 1) return -1;
+    if (fam < 0) return -1;
     if (fam > deriv) deriv = fam;
   }
   const int nbf = oracle_nbf(b), nsh = b->nshell;
@@ -1680,7 +1839,7 @@ int oracle_xc_build_unrestricted(const oracle_basis *b, long npts, const double 
     double *phi = malloc(sizeof(double) * 4 * (size_t)nbf * maxn);
     double *Dloc = malloc(sizeof(double) * 2 * (size_t)nbf * nbf);
     double *Kblk = malloc(sizeof(double) * 2 * (size_t)nbf * nbf);
-    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 20 * maxn));
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 24 * maxn));
 #pragma omp for schedule(dynamic, 4)
     for (long li = 0; li < nblk; li++) {
       long n = lcnt[li];

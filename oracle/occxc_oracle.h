@@ -103,6 +103,14 @@ int oracle_functional_eval_mgga(int id, long n, const double *rho, const double 
                                 const double *tau, double *exc, double *vrho,
                                 double *vsigma, double *vtau);
 double oracle_functional_energy_density_mgga(int id, double rho, double sigma, double tau);
+/* Spin-polarised meta-GGA (libxc's polarised layout): rho[2n], sigma[3n],
+ * tau[2n] = (tau_a, tau_b); ACCUMULATES exc[n], vrho[2n], vsigma[3n],
+ * vtau[2n].  LDA and GGA ids are accepted too (tau ignored). */
+int oracle_functional_eval_mgga_pol(int id, long n, const double *rho, const double *sigma,
+                                    const double *tau, double *exc, double *vrho,
+                                    double *vsigma, double *vtau);
+/* in = (rho_a, rho_b, s_aa, s_ab, s_bb, tau_a, tau_b) */
+double oracle_functional_energy_density_mgga_pol(int id, const double *in);
 
 /* The same functionals for a spin-polarised density, libxc's layout:
  * rho[2n] = (rho_a, rho_b) per point, sigma[3n] = (s_aa, s_ab, s_bb);

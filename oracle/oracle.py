@@ -69,6 +69,9 @@ def lib():
                                                    _dp, _dp, _dp]
         L.oracle_xc_gradient_unrestricted.argtypes = L.oracle_xc_gradient.argtypes
         L.oracle_functional_eval_mgga.argtypes = [ctypes.c_int, ctypes.c_long] + [_dp] * 7
+        L.oracle_functional_eval_mgga_pol.argtypes = [ctypes.c_int, ctypes.c_long] + [_dp] * 7
+        L.oracle_functional_energy_density_mgga_pol.argtypes = [ctypes.c_int, _dp]
+        L.oracle_functional_energy_density_mgga_pol.restype = ctypes.c_double
         L.oracle_vv10.argtypes = [ctypes.POINTER(_Basis), ctypes.c_long, _dp, _dp, _dp,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
         L.oracle_vv10_kernel.argtypes = [ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_double,
@@ -268,6 +271,26 @@ def functional_eval_pol(fid, rho, sigma=None):
 def energy_density_pol(fid, ra, rb, saa=0.0, sab=0.0, sbb=0.0):
     return lib().oracle_functional_energy_density_pol(fid, float(ra), float(rb), float(saa),
                                                       float(sab), float(sbb))
+
+
+def functional_eval_mgga_pol(fid, rho, sigma, tau):
+    """Polarised meta-GGA: rho (n, 2), sigma (n, 3), tau (n, 2) -> exc (n),
+    vrho (n, 2), vsigma (n, 3), vtau (n, 2)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1, 2)
+    n = rho.shape[0]
+    if lib().oracle_functional_family(fid) < 0:
+        raise ValueError(f"unknown functional id {fid}")
+    sig = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1, 3)
+    tau = np.ascontiguousarray(tau, dtype=np.float64).reshape(-1, 2)
+    exc, vrho, vsig, vtau = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3)), np.zeros((n, 2))
+    lib().oracle_functional_eval_mgga_pol(fid, n, _d(rho), _d(sig), _d(tau), _d(exc), _d(vrho),
+                                          _d(vsig), _d(vtau))
+    return exc, vrho, vsig, vtau
+
+
+def energy_density_mgga_pol(fid, ra, rb, saa, sab, sbb, ta, tb):
+    x = np.array([ra, rb, saa, sab, sbb, ta, tb], dtype=np.float64)
+    return lib().oracle_functional_energy_density_mgga_pol(fid, _d(x))
 
 
 def xc_build_unrestricted(basis, xyz, w, func_ids, D, density_threshold=1e-10,

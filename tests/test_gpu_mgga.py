@@ -148,5 +148,3 @@ def test_mgga_rejections():
     D = random_density(basis.nbf, 5, 3)
     with pytest.raises(Exception):
         dft.compute_xc_gradient(np.vstack([D, D]) / 2)  # unrestricted
-    with pytest.raises(Exception):
-        dft.compute_vxc(np.vstack([D, D]) / 2)

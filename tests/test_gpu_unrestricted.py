@@ -68,14 +68,14 @@ def _case(atoms, basis_name, method, pure, seed, precision="fp64", screen=True):
             max(abs(ga - na), abs(gb - nb)))
 
 
-@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp", "tpss"])
 def test_uks_water_631g(method):
     de, dv, dn = _case(molecules.water(), "6-31g", method, False, 31)
     print(f"UKS water/6-31G {method}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
     assert de <= TOL_E and dv <= TOL_V and dn <= TOL_N
 
 
-@pytest.mark.parametrize("method", ["pbe", "blyp", "svwn5"])
+@pytest.mark.parametrize("method", ["pbe", "blyp", "svwn5", "tpss"])
 def test_uks_water_dimer_def2svp(method):
     de, dv, dn = _case(molecules.water_cluster(2, seed=1), "def2-svp", method, True, 33)
     print(f"UKS (H2O)2/def2-SVP {method}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
@@ -164,3 +164,25 @@ def test_uks_one_electron_beta_density_zero(method):
     assert abs(dft.exchange_correlation_energy() - exc) <= TOL_E
     assert np.abs(V - K).max() <= TOL_V
     assert abs(ga - na) <= TOL_N and gb == 0.0 and nb == 0.0
+
+
+def test_uks_tpss_fp32_cache_and_closed_shell_limit():
+    """Unrestricted meta-GGA (compute_K_dft<2, Unrestricted>) with the FP32
+    AO cache, and D_a = D_b = D/2 against the restricted TPSS build."""
+    de, dv, dn = _case(molecules.water_cluster(2, seed=1), "def2-svp", "tpss", True, 39,
+                       precision="fp32cache")
+    print(f"UKS (H2O)2/def2-SVP tpss fp32cache: |dExc|={de:.2e} max|dVxc|={dv:.2e} "
+          f"|dN|={dn:.2e}")
+    assert de <= TOL_E and dv <= 1e-6 and dn <= 1e-6
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 5, 41)
+    dft = DFT("tpss", basis, pts, wts)
+    V = dft.compute_vxc(D)
+    e = dft.exchange_correlation_energy()
+    Vu = dft.compute_vxc(np.vstack([D, D]) / 2)
+    eu = dft.exchange_correlation_energy()
+    n = basis.nbf
+    assert abs(e - eu) < 1e-10
+    assert np.abs(Vu[:n] - V).max() < 1e-10 and np.abs(Vu[n:] - V).max() < 1e-10

@@ -177,11 +177,150 @@ def test_brute_force_tiny():
     np.testing.assert_allclose(Ko, K, atol=1e-12)
 
 
-def test_gradient_and_unrestricted_reject_mgga(water_case):
+def test_unrestricted_gradient_rejects_mgga(water_case):
     basis, ob, pts, w = water_case
     D = random_density(basis.nbf, 5, 3)
     with pytest.raises(ValueError):
-        oracle.xc_build_unrestricted(ob, pts, w, TPSS, np.vstack([D, D]))
+        oracle.xc_gradient_unrestricted(ob, basis.shell_atom, 3, pts, w, TPSS,
+                                        np.vstack([D, D]) / 2)
+
+
+def random_mgga_points_pol(rng, n):
+    """Open-shell points: spin ratios up to 1e3 either way, tau_s above
+    tau_W,s, gradients of both spins in independent directions."""
+    ra = 10 ** rng.uniform(-4, 1, n)
+    rb = ra * 10 ** rng.uniform(-3, 3, n)
+    ga = rng.normal(size=(n, 3)) * (ra ** (4 / 3))[:, None] * 10 ** rng.uniform(-1, 0.5, (n, 1))
+    gb = rng.normal(size=(n, 3)) * (rb ** (4 / 3))[:, None] * 10 ** rng.uniform(-1, 0.5, (n, 1))
+    sig = np.stack([(ga * ga).sum(1), (ga * gb).sum(1), (gb * gb).sum(1)], 1)
+    ta = sig[:, 0] / (8 * ra) * 10 ** rng.uniform(0.05, 2.0, n)
+    tb = sig[:, 2] / (8 * rb) * 10 ** rng.uniform(0.05, 2.0, n)
+    return np.stack([ra, rb], 1), sig, np.stack([ta, tb], 1)
+
+
+@pytest.mark.parametrize("fid", TPSS)
+def test_polarised_closed_shell_limit(fid, rng):
+    """(rho/2, rho/2, sigma/4 x 3, tau/2, tau/2) gives the closed-shell exc,
+    vrho_s = vrho, (v_aa + v_ab + v_bb)/4 = vsigma and vtau_s = vtau."""
+    r, s, t = random_mgga_points(rng, 300)
+    e, vr, vs, vt = oracle.functional_eval_mgga(fid, r, s, t)
+    ep, vrp, vsp, vtp = oracle.functional_eval_mgga_pol(
+        fid, np.stack([r, r], 1) / 2, np.stack([s, s, s], 1) / 4, np.stack([t, t], 1) / 2)
+    np.testing.assert_allclose(ep, e, rtol=1e-13)
+    for k in range(2):
+        np.testing.assert_allclose(vrp[:, k], vr, rtol=1e-12)
+        np.testing.assert_allclose(vtp[:, k], vt, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(vsp.sum(1) / 4, vs, rtol=1e-11)
+
+
+def test_polarised_tpss_exchange_spin_scaling(rng):
+    """E_x[na, nb] = (E_x[2 na] + E_x[2 nb]) / 2 with the closed-shell TPSS
+    exchange at (2 n_s, 4 s_ss, 2 t_s)."""
+    rho, sig, tau = random_mgga_points_pol(rng, 200)
+    for i in range(len(rho)):
+        ra, rb = rho[i]
+        e = oracle.energy_density_mgga_pol(202, ra, rb, sig[i, 0], sig[i, 1], sig[i, 2],
+                                           tau[i, 0], tau[i, 1])
+        ref = 0.5 * (oracle.energy_density_mgga(202, 2 * ra, 4 * sig[i, 0], 2 * tau[i, 0]) +
+                     oracle.energy_density_mgga(202, 2 * rb, 4 * sig[i, 2], 2 * tau[i, 1]))
+        assert e == pytest.approx(ref, rel=1e-13)
+
+
+def test_polarised_tpss_correlation_limits(rng):
+    """One-electron densities (n_b = 0, tau = tau_W) have zero TPSS
+    correlation (the self-interaction-free property of the paper's eq. 12);
+    fully polarised with z < 1 it is eps_PBE (1 - z^2)(1 + d eps (1-z^2) z^3)
+    with C dropped out; the spin-swap symmetry holds everywhere."""
+    n = 50
+    ra = 10 ** rng.uniform(-3, 1, n)
+    ga = rng.normal(size=(n, 3)) * (ra ** (4 / 3))[:, None]
+    saa = (ga * ga).sum(1)
+    z0 = np.zeros(n)
+    e, _, _, _ = oracle.functional_eval_mgga_pol(231, np.stack([ra, z0], 1),
+                                                 np.stack([saa, z0, z0], 1),
+                                                 np.stack([saa / (8 * ra), z0], 1))
+    epbe = np.array([oracle.energy_density_pol(130, a, 0.0, s, 0.0, 0.0) / a
+                     for a, s in zip(ra, saa)])
+    # z = tau_W / tau is 1 up to the rounding of saa / (8 ra)
+    assert np.all(np.abs(e) <= 1e-14 * np.abs(epbe))
+    zz = 10 ** rng.uniform(-1, -0.01, n)
+    e, _, _, _ = oracle.functional_eval_mgga_pol(231, np.stack([ra, z0], 1),
+                                                 np.stack([saa, z0, z0], 1),
+                                                 np.stack([saa / (8 * ra) / zz, z0], 1))
+    f0 = epbe * (1 - zz ** 2)
+    np.testing.assert_allclose(e, f0 * (1 + 2.8 * f0 * zz ** 3), rtol=1e-12)
+    rho, sig, tau = random_mgga_points_pol(rng, 100)
+    for i in range(len(rho)):
+        a = oracle.energy_density_mgga_pol(231, rho[i, 0], rho[i, 1], *sig[i], *tau[i])
+        b = oracle.energy_density_mgga_pol(231, rho[i, 1], rho[i, 0], sig[i, 2], sig[i, 1],
+                                           sig[i, 0], tau[i, 1], tau[i, 0])
+        assert a == pytest.approx(b, rel=1e-13)
+
+
+@pytest.mark.parametrize("fid", TPSS)
+def test_polarised_complex_step_matches_finite_difference(fid, rng):
+    rho, sig, tau = random_mgga_points_pol(rng, 30)
+    _, vr, vs, vt = oracle.functional_eval_mgga_pol(fid, rho, sig, tau)
+    for i in range(len(rho)):
+        x = np.concatenate([rho[i], sig[i], tau[i]])
+        grad = np.concatenate([vr[i], vs[i], vt[i]])
+        e0 = abs(oracle.energy_density_mgga_pol(fid, *x))
+        for k in range(7):
+            if fid == 202 and k == 3:
+                assert grad[k] == 0.0  # exchange does not see s_ab
+                continue
+            h = 1e-4 * abs(x[k])
+
+            def f(d):
+                y = x.copy()
+                y[k] += d
+                return oracle.energy_density_mgga_pol(fid, *y)
+            fd = (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+            assert grad[k] == pytest.approx(fd, rel=1e-6, abs=1e-12 * e0 / h)
+
+
+@pytest.fixture(scope="module")
+def open_shell_dimer():
+    atoms = molecules.water_cluster(2, seed=1)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    g = MolecularGrid(atoms, GridSettings(radial_points=41, angular_points=146))
+    pts, w = grid_ref.molecular_grid(g)
+    return basis, oracle.OracleBasis.from_flat(basis.flat()), pts, w
+
+
+def test_unrestricted_mgga_closed_shell_equals_restricted(open_shell_dimer):
+    basis, ob, pts, w = open_shell_dimer
+    n = basis.nbf
+    D = random_density(n, 10, 3)
+    K, e, nel = oracle.xc_build(ob, pts, w, TPSS, D)
+    Ku, eu, (na, nb) = oracle.xc_build_unrestricted(ob, pts, w, TPSS, np.vstack([D / 2, D / 2]))
+    assert eu == pytest.approx(e, abs=1e-11)
+    assert np.abs(Ku[:n] - K).max() < 1e-11 and np.abs(Ku[n:] - K).max() < 1e-11
+    assert na == pytest.approx(nel / 2, abs=1e-12) and nb == pytest.approx(nel / 2, abs=1e-12)
+
+
+def test_unrestricted_mgga_potential_is_the_derivative(open_shell_dimer):
+    """dExc/dD_s(Delta) = 2 tr(V_s Delta) per spin on an open-shell density,
+    which pins the per-spin tau terms of the unrestricted Vxc; screened ==
+    dense."""
+    basis, ob, pts, w = open_shell_dimer
+    n = basis.nbf
+    Du = np.vstack([random_density(n, 10, 5) / 2, random_density(n, 9, 6) / 2])
+    Ku, eu, _ = oracle.xc_build_unrestricted(ob, pts, w, TPSS, Du)
+    Ks, es, _ = oracle.xc_build_unrestricted(ob, pts, w, TPSS, Du, screened=False)
+    assert abs(eu - es) < 1e-8 and np.abs(Ku - Ks).max() < 1e-8
+    r = np.random.default_rng(9)
+    for sp in (0, 1):
+        Delta = r.normal(size=(n, n)) * 0.01
+        Delta = Delta + Delta.T
+        eps = 1e-4
+        Dp, Dm = Du.copy(), Du.copy()
+        Dp[sp * n:(sp + 1) * n] += eps * Delta
+        Dm[sp * n:(sp + 1) * n] -= eps * Delta
+        fd = (oracle.xc_build_unrestricted(ob, pts, w, TPSS, Dp)[1] -
+              oracle.xc_build_unrestricted(ob, pts, w, TPSS, Dm)[1]) / (2 * eps)
+        assert fd == pytest.approx(2 * np.sum(Ku[sp * n:(sp + 1) * n] * Delta), rel=1e-6,
+                                   abs=1e-9)
 
 
 @pytest.fixture(scope="module")
@@ -221,3 +360,55 @@ def test_gpu_mgga_header_matches_oracle(mgga_shim, fid, rng):
         assert v == pytest.approx(v0[i], rel=1e-10, abs=1e-14)
         assert w == pytest.approx(w0[i], rel=1e-10, abs=1e-14 * abs(w0[i]) + 1e-300)
         assert x == pytest.approx(x0[i], rel=1e-10, abs=1e-14)
+
+
+@pytest.fixture(scope="module")
+def mgga_pol_shim(tmp_path_factory):
+    """The GPU kernels' polarised meta-GGA header compiled for the host."""
+    import ctypes
+    import pathlib
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    d = tmp_path_factory.mktemp("mgga_pol_shim")
+    src = d / "shim.cpp"
+    src.write_text('#include "xc_functionals_mgga.hpp"\n'
+                   'extern "C" int shim(int id, const double *x, double *o) {\n'
+                   '  for (int k = 0; k < 8; ++k) o[k] = 0;\n'
+                   '  return occxc::xc_point_mgga_pol(id, x[0], x[1], x[2], x[3], x[4], x[5], '
+                   'x[6], o[0], o + 1, o + 3, o + 6) ? 0 : -1; }\n')
+    so = d / "shim.so"
+    subprocess.run(["g++", "-O2", "-fPIC", "-shared", f"-I{root / 'occ_amd' / 'csrc'}",
+                    str(src), "-o", str(so)], check=True)
+    L = ctypes.CDLL(str(so))
+    L.shim.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_double)]
+    return L
+
+
+@pytest.mark.parametrize("fid", TPSS)
+def test_gpu_mgga_pol_header_matches_oracle(mgga_pol_shim, fid, rng):
+    import ctypes
+    rho, sig, tau = random_mgga_points_pol(rng, 300)
+    # one spin empty, and tau below tau_W (limited) on a few points
+    rho[:10, 1] = 0.0
+    sig[:10, 1:] = 0.0
+    tau[:10, 1] = 0.0
+    tau[10:20] *= 0.3
+    e0, v0, s0, t0 = oracle.functional_eval_mgga_pol(fid, rho, sig, tau)
+    out = np.zeros(8)
+    for i in range(len(rho)):
+        x = np.concatenate([rho[i], sig[i], tau[i]])
+        assert mgga_pol_shim.shim(fid, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+        assert np.all(np.isfinite(out))
+        n = rho[i].sum()
+        assert out[0] == pytest.approx(e0[i] * n, rel=1e-11, abs=1e-300)
+        ref = np.concatenate([v0[i], s0[i], t0[i]])
+        got = out[1:].copy()
+        if i < 10:
+            # with n_b = 0 and s_bb = 0, d/ds_bb passes through C(zeta, xi)'s
+            # (1 - zeta)^(-4/3) (held at the zeta threshold, ~1e21) times
+            # eps_PBE - eps~ = 0: both forms round that 0 x 1e21 differently.
+            # The build multiplies vsigma_bb by grad rho_b = 0, so it is unused.
+            got[4] = ref[4]
+        np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-13 * np.abs(ref).max())
