@@ -121,7 +121,7 @@ def main():
         if args.workload == "tpssgrad32":
             args.modes = ",".join(m for m in args.modes.split(",") if m in ("fp64", "fp32cache"))
         return run_gradient(args)
-    if args.workload == "uks32":
+    if args.workload in ("uks32", "utpss32"):
         return run_unrestricted(args)
     if args.workload == "serve":
         return run_serving(args)
@@ -372,12 +372,15 @@ def run_unrestricted(args):
     doublet cation: alpha C (768 x 160) and beta C (768 x 159) seeded Gaussian,
     orthonormalised, D_s = C_s C_s^T / 2 (OCC's density_matrix_unrestricted).
     A step is one build from D (2 nbf x nbf) in HBM to both spins' Vxc in HBM,
-    Exc, N_alpha, N_beta."""
+    Exc, N_alpha, N_beta.  utpss32 is the same doublet with the TPSS meta-GGA
+    (compute_K_dft<2, Unrestricted>: per spin, rho and the three tau
+    directions in the density stage, the GGA and three tau terms in Vxc)."""
     from occ_amd import AOBasis
     from occ_amd.dft import XCContext, functional_ids
     from occ_amd.device import DeviceArray, synchronize
     from occ_amd.grid import GridSettings, MolecularGrid
-    atoms, bname, pure, method, nocc, desc = workload("h2o32")
+    mgga = args.workload == "utpss32"
+    atoms, bname, pure, method, nocc, desc = workload("tpss32" if mgga else "h2o32")
     basis = AOBasis.load(atoms, bname, pure=pure)
     nbf = basis.nbf
     pts, wts = MolecularGrid(atoms, GridSettings(radial_points=76, angular_points=302)).build(
@@ -439,9 +442,11 @@ def run_unrestricted(args):
     vxc_ms = tim["vxc"][0] / max(tim["vxc"][1], 1)
     kname, kms = ("density", dens_ms) if dens_ms >= vxc_ms else ("vxc", vxc_ms)
     flops = R["info"]["contraction_flops"]   # per stage: one 2 n m^2 GEMM per spin
+    if mgga:
+        flops *= 4
     achieved = flops / (kms * 1e-3) / 1e12
     out = {
-        "metric": "unrestricted XC builds/s (FP64)",
+        "metric": ("unrestricted meta-GGA" if mgga else "unrestricted") + " XC builds/s (FP64)",
         "value": R["per_s"], "unit": "UKS XC builds/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[head], "data": "synthetic",
@@ -451,8 +456,10 @@ def run_unrestricted(args):
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved,
                      "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS[head],
-                     "traffic": measured_traffic("uks32", head, kname), "kernel_ms": kms,
-                     "work": "sum over leaves of 2 n_b m_b^2 per spin (two GEMMs per stage)"},
+                     "traffic": measured_traffic(args.workload, head, kname), "kernel_ms": kms,
+                     "work": "sum over leaves of 2 n_b m_b^2 per spin (two GEMMs per stage)"
+                             + (", times four for rho and the three tau directions"
+                                if mgga else "")},
         "accuracy": {m: {k: results[m].get(k) for k in ("dexc", "dvxc", "dnelec")}
                      for m in results},
         "modes_ms": {m: results[m]["ms_per_step"] for m in results},

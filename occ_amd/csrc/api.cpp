@@ -1019,6 +1019,33 @@ void run_gradient(occxc_context *c, const double *d_D, double *d_grad, hipStream
 // compute_K_dft<d, Unrestricted> (dft.h:190-388).  Each spin density is
 // contracted by the density kernel in store mode, the polarised functional
 // stage forms both spins' coefficients, and the Vxc kernels run once per spin.
+// The unrestricted meta-GGA density stage (evaluate_density<2, U>,
+// density.cpp:104-166): each spin's rho and grad rho, each spin's three
+// per-direction tau passes on the derivative planes, then the polarised
+// meta-GGA functional stage.
+static void mgga_density_u(occxc_context *c, hipStream_t s) {
+  const BuildArgs &a = c->args;
+  const int prec = c->opt.precision;
+  const double *D2[2] = {c->d_D2a.get(), c->d_D2b.get()};
+  double *rho[2] = {c->d_rho_a.get(), c->d_rho_b.get()};
+  double *tk[2] = {c->d_mtau.get(), c->d_mtau_b.get()};
+  for (int sp = 0; sp < 2; ++sp) {
+    BuildArgs x = a;
+    x.rho_out = rho[sp];
+    run_density_store(x, c->d_ao.get(), prec, D2[sp], s);
+    for (int k = 0; k < 3; ++k) {
+      BuildArgs y = a;
+      y.deriv = 0;
+      y.leaves = c->d_leaves_k.get() + (size_t)k * c->nleaves;
+      y.rho_out = tk[sp] + (size_t)k * c->ptot;
+      run_density_store(y, c->d_ao.get(), prec, D2[sp], s);
+    }
+  }
+  launch_functional_mgga_pol_leaves(a, rho[0], rho[1], tk[0], tk[1], c->d_coef_b.get(),
+                                    c->d_coef_tau.get(), c->d_coef_tau_b.get(),
+                                    c->d_part_u.get(), s);
+}
+
 void run_build_u(occxc_context *c, hipStream_t s) {
   const BuildArgs &a = c->args;
   const int prec = c->opt.precision;
@@ -1031,6 +1058,18 @@ void run_build_u(occxc_context *c, hipStream_t s) {
       launch_zero(c->d_K_b.get(), c->mtot, s);
     }
   });
+  if (c->mgga) {
+    t.run(OCCXC_STAGE_DENSITY, [&] { mgga_density_u(c, s); });
+    t.run(OCCXC_STAGE_VXC, [&] {
+      BuildArgs x = a;
+      x.coef_tau = c->d_coef_tau.get();
+      run_vxc_mgga(c->plan, x, c->d_leaves_k.get(), c->d_ao.get(), prec, s);
+      x.coef = c->d_coef_b.get();
+      x.coef_tau = c->d_coef_tau_b.get();
+      x.K = c->d_K_b.get();
+      run_vxc_mgga(c->plan, x, c->d_leaves_k.get(), c->d_ao.get(), prec, s);
+    });
+  } else {
   t.run(OCCXC_STAGE_DENSITY, [&] {
     BuildArgs x = a;
     x.rho_out = c->d_rho_a.get();
@@ -1049,6 +1088,7 @@ void run_build_u(occxc_context *c, hipStream_t s) {
     run_vxc(c->plan, b, c->d_ao.get(), prec, s);
     if (two_pass) run_vxc_reduce(b, c->d_V_b.get(), s);
   });
+  }
   t.run(OCCXC_STAGE_FINALIZE, [&] {
     launch_finalize_u(two_pass ? nullptr : c->d_K.get(), c->d_K_b.get(), c->d_V.get(),
                       c->d_V_b.get(), c->d_mols.get(), c->maxnbf, c->d_part_u.get(),
@@ -1107,6 +1147,11 @@ void uks_setup(occxc_context *c) {
   c->d_part_u.alloc(3 * (size_t)std::max(c->nleaves, 1));
   c->d_result_u.alloc(3);
   c->device_bytes += 8.0 * (7 * nn + 12 * (size_t)c->ptot + 3 * (size_t)c->nleaves + 3);
+  if (c->mgga) {
+    c->d_mtau_b.alloc(3 * (size_t)c->ptot);
+    c->d_coef_tau_b.alloc(c->ptot);
+    c->device_bytes += 8.0 * 4 * (size_t)c->ptot;
+  }
   c->uks_ready = true;
 }
 
@@ -1390,8 +1435,6 @@ occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, doubl
                                       double *exc, double *nelec) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && Vxc && exc && nelec, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
-    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_build_unrestricted needs a single-molecule context");
     OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
@@ -1426,8 +1469,6 @@ occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d
                                              double *d_Vxc, double *d_result, void *stream) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && d_D && d_Vxc && d_result, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
-    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_build_unrestricted needs a single-molecule context");
     OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
@@ -1459,7 +1500,7 @@ occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D,
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA functionals: restricted densities only");
+                  "meta-GGA gradients: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");

@@ -82,6 +82,9 @@ struct occxc_context {
   // point at the d/dx, d/dy, d/dz planes of the AO cache
   bool mgga = false;
   occxc::DevBuf<double> d_mrho, d_mtau, d_coef_tau;
+  // unrestricted meta-GGA: the beta spin's tau passes and c_tau (alpha uses
+  // d_mtau / d_coef_tau)
+  occxc::DevBuf<double> d_mtau_b, d_coef_tau_b;
   occxc::DevBuf<occxc::LeafDev> d_leaves_k;
   // VV10 (allocated on the first call): input-order point planes and F, U, W
   occxc::DevBuf<double> d_nlc;

@@ -608,6 +608,69 @@ __global__ __launch_bounds__(256) void k_functional_mgga(BuildArgs a,
   }
 }
 
+// Unrestricted meta-GGA per-leaf functional stage (one thread per point,
+// blockDim = 256), process_grid_block<2, Unrestricted> (dft_kernels.cpp:8-70):
+// rho_s and grad rho_s from the spin densities' store passes ([4][ptot]
+// each), tau_s = (1/2) sum_k tk_s[k] (density.cpp:130-166), the polarised
+// functionals (tau interleaved per point as dft_kernels.h:51-57 hands it to
+// libxc), weight_by, and the coefficients of xc_potential_matrix<
+// Unrestricted, 2> (xc_potential_matrix.cpp:61-137) per spin: those of
+// k_functional_pol in a.coef / coef_b and c_tau,s = w vtau_s / 4 in
+// ctau_a / ctau_b (V_s = K_s + K_s^T).  part[3 leaf] = {Exc, N_a, N_b}.
+__global__ __launch_bounds__(256) void k_functional_mgga_pol(
+    BuildArgs a, const double *__restrict__ rho_a, const double *__restrict__ rho_b,
+    const double *__restrict__ tka, const double *__restrict__ tkb, double *__restrict__ coef_b,
+    double *__restrict__ ctau_a, double *__restrict__ ctau_b, double *__restrict__ part) {
+  __shared__ double s_red[4];
+  const int leaf = blockIdx.x;
+  const LeafDev L = a.leaves[leaf];
+  const int p = threadIdx.x;
+  const bool live = p < L.np, real = p < L.n;
+  const long long q = L.pt_off + p, P = a.ptot;
+  double ra = 0, rb = 0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0}, ta = 0, tb = 0, w = 0;
+  if (live) {
+    ra = rho_a[q];
+    rb = rho_b[q];
+    for (int k = 0; k < 3; ++k) {
+      ga[k] = rho_a[(k + 1) * P + q];
+      gb[k] = rho_b[(k + 1) * P + q];
+    }
+    ta = 0.5 * (tka[q] + tka[P + q] + tka[2 * P + q]);
+    tb = 0.5 * (tkb[q] + tkb[P + q] + tkb[2 * P + q]);
+    w = a.pw[q];
+  }
+  const double na = block_sum_256(real ? ra * w : 0.0, s_red);
+  const double nb = block_sum_256(real ? rb * w : 0.0, s_red);
+  const double rmax = block_max_256(real ? fmax(ra, rb) : -1e300, s_red);
+  const bool active = !(rmax < a.dthr);
+  double e = 0.0, vr[2] = {0, 0}, vs[3] = {0, 0, 0}, vt[2] = {0, 0};
+  if (active && live && ra + rb >= kDensThreshold) {
+    const double saa = fmax(ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2], kSigmaMin);
+    const double sab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    const double sbb = fmax(gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2], kSigmaMin);
+    for (int f = 0; f < a.nfunc; ++f)
+      xc_point_mgga_pol(a.func_ids[f], ra, rb, saa, sab, sbb, ta, tb, e, vr, vs, vt);
+  }
+  const double exc = block_sum_256(active && real ? w * e : 0.0, s_red);
+  if (live) {
+    double *ca = a.coef;
+    ca[q] = active ? 0.5 * w * vr[0] : 0.0;
+    coef_b[q] = active ? 0.5 * w * vr[1] : 0.0;
+    for (int k = 0; k < 3; ++k) {
+      ca[(k + 1) * P + q] = active ? w * (2.0 * vs[0] * ga[k] + vs[1] * gb[k]) : 0.0;
+      coef_b[(k + 1) * P + q] = active ? w * (2.0 * vs[2] * gb[k] + vs[1] * ga[k]) : 0.0;
+    }
+    ctau_a[q] = active ? 0.25 * w * vt[0] : 0.0;
+    ctau_b[q] = active ? 0.25 * w * vt[1] : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    part[3 * leaf] = active ? exc : 0.0;
+    part[3 * leaf + 1] = na;
+    part[3 * leaf + 2] = nb;
+    a.leaf_active[leaf] = active ? 1 : 0;
+  }
+}
+
 // Meta-GGA (and LDA / GGA) functionals over points, ADDING to the outputs.
 __global__ void k_functional_mgga_points(int id, long long n, const double *__restrict__ rho,
                                          const double *__restrict__ sigma,
@@ -964,6 +1027,16 @@ void launch_functional_mgga_leaves(const BuildArgs &a, const double *rho4, const
                                    double *coef_tau, hipStream_t s) {
   if (a.nleaves == 0) return;
   k_functional_mgga<<<a.nleaves, 256, 0, s>>>(a, rho4, tk, coef_tau);
+  check_launch();
+}
+
+void launch_functional_mgga_pol_leaves(const BuildArgs &a, const double *rho_a,
+                                       const double *rho_b, const double *tka, const double *tkb,
+                                       double *coef_b, double *ctau_a, double *ctau_b,
+                                       double *part, hipStream_t s) {
+  if (a.nleaves == 0) return;
+  k_functional_mgga_pol<<<a.nleaves, 256, 0, s>>>(a, rho_a, rho_b, tka, tkb, coef_b, ctau_a,
+                                                   ctau_b, part);
   check_launch();
 }
 

@@ -191,6 +191,12 @@ void launch_functional_pol(int id, long long n, const double *rho, const double 
 // coefficients into a.coef and coef_tau, {Exc, N_elec} into a.leaf_part
 void launch_functional_mgga_leaves(const BuildArgs &a, const double *rho4, const double *tk,
                                    double *coef_tau, hipStream_t s);
+// unrestricted: both spins' coefficients, c_tau per spin, {Exc, N_a, N_b}
+// per leaf into part
+void launch_functional_mgga_pol_leaves(const BuildArgs &a, const double *rho_a,
+                                       const double *rho_b, const double *tka, const double *tkb,
+                                       double *coef_b, double *ctau_a, double *ctau_b,
+                                       double *part, hipStream_t s);
 void launch_functional_mgga(int id, long long n, const double *rho, const double *sigma,
                             const double *tau, double *exc, double *vrho, double *vsigma,
                             double *vtau, hipStream_t s);

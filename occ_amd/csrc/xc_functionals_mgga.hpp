@@ -1,5 +1,6 @@
 // xc_functionals_mgga.hpp -- TPSS meta-GGA exchange and correlation in FP64
-// for the restricted build, evaluated per grid point on the GPU.
+// for the restricted and unrestricted builds, evaluated per grid point on the
+// GPU.
 //
 // Replaces libxc 6.2.2's unpolarised xc_mgga_exc_vxc for mgga_x_tpss and
 // mgga_c_tpss as called from DensityFunctional::evaluate with Family::MGGA
@@ -86,6 +87,86 @@ OCCXC_HD inline bool xc_point_mgga(int id, double rho, double sigma, double tau,
   vr = out.d[0];
   vs = out.d[1];
   vt = out.d[2];
+  return true;
+}
+
+// Spin-polarised TPSS (same paper): exchange by exact spin scaling,
+// E_x[na, nb] = E_x[2 na]/2 + E_x[2 nb]/2 with (2 n_s, 4 s_ss, 2 t_s);
+// correlation by eqs. 11-14 in full: zeta = (na - nb)/n, |grad zeta|^2 =
+// 4 (nb^2 s_aa - 2 na nb s_ab + na^2 s_bb) / n^4, xi^2 = |grad zeta|^2 /
+// (4 (3 pi^2 n)^(2/3)), C(zeta, xi) = (0.53 + 0.87 zeta^2 + 0.50 zeta^4 +
+// 2.26 zeta^6) / (1 + xi^2 ((1+zeta)^(-4/3) + (1-zeta)^(-4/3))/2)^4, the
+// spin channels' max(eps_PBE(n_s, 0, s_ss), eps_PBE) weighted n_s / n, and
+// z = tau_W / tau of the total density (held at 1 above 1).
+OCCXC_HD inline D5 ex_tpss_pol(const D5 &ra, const D5 &rb, const D5 &saa, const D5 &sbb,
+                               const D5 &ta, const D5 &tb) {
+  D5 e = d5c(0.0);
+  if (ra.v >= kDensThreshold) e = e + 0.5 * ex_tpss(2.0 * ra, 4.0 * saa, 2.0 * ta);
+  if (rb.v >= kDensThreshold) e = e + 0.5 * ex_tpss(2.0 * rb, 4.0 * sbb, 2.0 * tb);
+  return e;
+}
+
+OCCXC_HD inline D5 ec_tpss_pol(const D5 &ra, const D5 &rb, const D5 &saa, const D5 &sab,
+                               const D5 &sbb, const D5 &ta, const D5 &tb) {
+  const double d = 2.8;
+  const D5 n = ra + rb, zeta = (ra - rb) / n;
+  const D5 sig = saa + 2.0 * sab + sbb;
+  const D5 epbe = ec_pbe_pol(ra, rb, sig) / n;
+  D5 par = d5c(0.0);
+  if (ra.v >= kDensThreshold) {
+    const D5 ea = ec_pbe_pol(ra, d5c(0.0), saa) / ra;
+    par = par + ra / n * (ea.v > epbe.v ? ea : epbe);
+  }
+  if (rb.v >= kDensThreshold) {
+    const D5 eb = ec_pbe_pol(rb, d5c(0.0), sbb) / rb;
+    par = par + rb / n * (eb.v > epbe.v ? eb : epbe);
+  }
+  D5 z = sig / (8.0 * n * (ta + tb));
+  if (z.v > 1.0) z = d5c(1.0);
+  const D5 n2 = n * n;
+  const D5 gz2 = 4.0 * (rb * rb * saa - 2.0 * ra * rb * sab + ra * ra * sbb) / (n2 * n2);
+  const D5 xi2 = gz2 / (4.0 * d5_pow(3.0 * kPi * kPi * n, 2.0 / 3.0));
+  const D5 q = zeta * zeta;
+  const D5 c0 = 0.53 + 0.87 * q + 0.50 * q * q + 2.26 * q * q * q;
+  const D5 den = 1.0 + xi2 * (opz_pow(1.0 + zeta, -4.0 / 3.0) + opz_pow(1.0 - zeta, -4.0 / 3.0)) / 2.0;
+  const D5 den2 = den * den;
+  const D5 C = c0 / (den2 * den2);
+  const D5 z2 = z * z;
+  const D5 f0 = (1.0 + C * z2) * epbe - (1.0 + C) * z2 * par;
+  return n * f0 * (1.0 + d * f0 * z2 * z);
+}
+
+// One functional at one polarised point of a meta-GGA build, ACCUMULATED
+// into e (= n eps), vr[2], vs[3] (aa, ab, bb), vt[2] like xc_point_pol.  The
+// seven partials come from two passes over the five dual slots: (rho_a,
+// rho_b, s_aa, s_ab, s_bb), then (tau_a, tau_b).  Inputs are limited per
+// spin as for the closed-shell form (tau_s >= 1e-20, s_ss <= 8 n_s tau_s).
+// LDA and GGA ids go to xc_point_pol.  Returns false for an unknown id.
+OCCXC_HD inline bool xc_point_mgga_pol(int id, double ra, double rb, double saa, double sab,
+                                       double sbb, double ta, double tb, double &e, double *vr,
+                                       double *vs, double *vt) {
+  if (id != XC_MGGA_X_TPSS && id != XC_MGGA_C_TPSS) return xc_point_pol(id, ra, rb, saa, sab, sbb, e, vr, vs);
+  ta = fmax(ta, kTauMin);
+  tb = fmax(tb, kTauMin);
+  saa = fmin(fmax(saa, kSigmaMin), 8.0 * ra * ta);
+  sbb = fmin(fmax(sbb, kSigmaMin), 8.0 * rb * tb);
+  const bool x = id == XC_MGGA_X_TPSS;
+  {
+    const D5 a = d5v(ra, 0), b = d5v(rb, 1), s0 = d5v(saa, 2), s1 = d5v(sab, 3), s2 = d5v(sbb, 4);
+    const D5 out = x ? ex_tpss_pol(a, b, s0, s2, d5c(ta), d5c(tb))
+                     : ec_tpss_pol(a, b, s0, s1, s2, d5c(ta), d5c(tb));
+    e += out.v;
+    vr[0] += out.d[0];
+    vr[1] += out.d[1];
+    vs[0] += out.d[2];
+    vs[1] += out.d[3];
+    vs[2] += out.d[4];
+  }
+  const D5 a = d5c(ra), b = d5c(rb), s0 = d5c(saa), s1 = d5c(sab), s2 = d5c(sbb);
+  const D5 out = x ? ex_tpss_pol(a, b, s0, s2, d5v(ta, 0), d5v(tb, 1))
+                   : ec_tpss_pol(a, b, s0, s1, s2, d5v(ta, 0), d5v(tb, 1));
+  vt[0] += out.d[0];
+  vt[1] += out.d[1];
   return true;
 }
 
