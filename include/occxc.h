@@ -59,7 +59,7 @@ enum {
   OCCXC_GGA_X_B88 = 106,
   OCCXC_GGA_C_PBE = 130,
   OCCXC_GGA_C_LYP = 131,
-  OCCXC_MGGA_X_TPSS = 202,    /* meta-GGA: restricted builds only */
+  OCCXC_MGGA_X_TPSS = 202,    /* meta-GGA: builds (both spin cases), restricted gradient */
   OCCXC_MGGA_C_TPSS = 231
 };
 
@@ -141,7 +141,9 @@ occxc_status occxc_build(occxc_context *ctx, const double *D, double *Vxc,
  * (qm::block::a / b; OCC's density_matrix_unrestricted, orb.h:68-82, holds
  * C_s C_s^T / 2 per spin), each block read in full; Vxc is returned in the
  * same 2nbf x nbf layout; nelec[2] = {N_alpha, N_beta}, which OCC logs.
- * FP64 and FP32-cache contexts of one molecule. */
+ * Meta-GGA ids add per-spin tau (density.cpp:130-166) and the per-spin tau
+ * terms of xc_potential_matrix<Unrestricted, 2> (xc_potential_matrix.cpp:
+ * 111-137).  FP64 and FP32-cache contexts of one molecule. */
 occxc_status occxc_build_unrestricted(occxc_context *ctx, const double *D, double *Vxc,
                                       double *exc, double *nelec);
 
