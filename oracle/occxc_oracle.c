@@ -1916,6 +1916,8 @@ static void process_block_gradient_u(const block_cfg *cfg, int m, long n, const 
   double *exc = rho + 8 * n, *vr = exc + n, *vs = vr + 2 * n, *rr = vs + 3 * n, *ss = rr + 2 * n;
   double *A = ss + 3 * n;           /* [m][n] */
   double *V = A + mn;               /* [2][3][m x m] */
+  double *tt = V + 6L * m * m, *vt = tt + 2 * n; /* meta-GGA: tau_s, vtau_s (a,b per point) */
+  const int mgga = cfg->deriv == 2;
   memset(rho, 0, sizeof(double) * 8 * n);
   for (int sp = 0; sp < 2; sp++) { /* evaluate_density<d, U> */
     double *r = rho + 4L * n * sp;
@@ -1938,6 +1940,27 @@ static void process_block_gradient_u(const block_cfg *cfg, int m, long n, const 
         }
     }
   }
+  /* evaluate_density<2, U> (density.cpp:130-166): tau_s per spin block */
+  if (mgga) {
+    memset(tt, 0, sizeof(double) * 2 * n);
+    for (int sp = 0; sp < 2; sp++)
+      for (int k = 0; k < 3; k++) {
+        memset(T, 0, sizeof(double) * mn);
+        for (int nu = 0; nu < m; nu++) {
+          double *t = T + (long)nu * n;
+          for (int mu = 0; mu < m; mu++) {
+            const double dm = Dloc[sp * m + mu + (long)nu * ld];
+            const double *f = pd[k] + (long)mu * n;
+            for (long p = 0; p < n; p++) t[p] += dm * f[p];
+          }
+        }
+        for (int nu = 0; nu < m; nu++) {
+          const double *t = T + (long)nu * n, *f = pd[k] + (long)nu * n;
+          for (long p = 0; p < n; p++) tt[2 * p + sp] += f[p] * t[p];
+        }
+      }
+    for (long p = 0; p < 2 * n; p++) tt[p] *= 0.5;
+  }
   const double *ra = rho, *rb = rho + 4 * n;
   double rmax = -1e300;
   for (long p = 0; p < n; p++) {
@@ -1955,10 +1978,17 @@ static void process_block_gradient_u(const block_cfg *cfg, int m, long n, const 
     ss[3 * p + 1] = ra[n + p] * rb[n + p] + ra[2 * n + p] * rb[2 * n + p] + ra[3 * n + p] * rb[3 * n + p];
     ss[3 * p + 2] = rb[n + p] * rb[n + p] + rb[2 * n + p] * rb[2 * n + p] + rb[3 * n + p] * rb[3 * n + p];
   }
-  for (int f = 0; f < cfg->nfunc; f++) oracle_functional_eval_pol(cfg->ids[f], n, rr, ss, exc, vr, vs);
+  if (mgga) {
+    for (long p = 0; p < 2 * n; p++) vt[p] = 0.0;
+    for (int f = 0; f < cfg->nfunc; f++)
+      oracle_functional_eval_mgga_pol(cfg->ids[f], n, rr, ss, tt, exc, vr, vs, vt);
+  } else {
+    for (int f = 0; f < cfg->nfunc; f++) oracle_functional_eval_pol(cfg->ids[f], n, rr, ss, exc, vr, vs);
+  }
   for (long p = 0; p < n; p++) {
     for (int k = 0; k < 2; k++) vr[2 * p + k] *= wts[p];
     for (int k = 0; k < 3; k++) vs[3 * p + k] *= wts[p];
+    if (mgga) for (int k = 0; k < 2; k++) vt[2 * p + k] *= wts[p];
   }
   memset(V, 0, sizeof(double) * 6 * m * m);
   for (int sp = 0; sp < 2; sp++) {
@@ -2023,6 +2053,27 @@ static void process_block_gradient_u(const block_cfg *cfg, int m, long n, const 
         }
       }
     }
+    /* accumulate_gradient_contribution_U<2> (dft_gradient_kernels.cpp:
+     * 379-408 alpha, 441-470 beta): aow_j = (vtau_s / 2) d_j phi,
+     * V_k += d_kj phi^T aow_j, not symmetrised */
+    if (mgga) {
+      for (int j = 0; j < 3; j++) {
+        for (int nu = 0; nu < m; nu++) {
+          const long o = (long)nu * n;
+          for (long p = 0; p < n; p++) A[o + p] = 0.5 * vt[2 * p + sp] * pd[j][o + p];
+        }
+        for (int k = 0; k < 3; k++)
+          for (int nu = 0; nu < m; nu++) {
+            const double *an = A + (long)nu * n;
+            for (int mu = 0; mu < m; mu++) {
+              const double *hm = ph[k][j] + (long)mu * n;
+              double acc = 0.0;
+              for (long p = 0; p < n; p++) acc += hm[p] * an[p];
+              Vs[k][mu + (long)nu * m] += acc;
+            }
+          }
+      }
+    }
   }
   /* dft_gradient_kernels.cpp:561-582: 2 (V_a D_a + V_b D_b), skipping
    * |D_a + D_b| < 1e-12 */
@@ -2047,7 +2098,7 @@ int oracle_xc_gradient_unrestricted(const oracle_basis *b, const int *shell_atom
   int deriv = 0;
   for (int f = 0; f < nfunc; f++) {
     int fam = oracle_functional_family(func_ids[f]);
-    if (fam < 0 || fam > 1) return -1;
+    if (fam < 0) return -1;
     if (fam > deriv) deriv = fam;
   }
   const int nsh = b->nshell;
@@ -2100,7 +2151,7 @@ int oracle_xc_gradient_unrestricted(const oracle_basis *b, const int *shell_atom
     double *phi = malloc(sizeof(double) * 10 * (size_t)nbf * maxn);
     double *tmp = malloc(sizeof(double) * 10 * (ORACLE_MAX_L + 1) * (ORACLE_MAX_L + 2) / 2 * maxn);
     double *Dloc = malloc(sizeof(double) * 2 * (size_t)nbf * nbf);
-    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 20 * maxn +
+    double *work = malloc(sizeof(double) * ((size_t)2 * nbf * maxn + 24 * maxn +
                                             6 * (size_t)nbf * nbf));
 #pragma omp for schedule(dynamic, 4)
     for (long li = 0; li < nblk; li++) {

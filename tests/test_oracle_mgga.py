@@ -177,14 +177,6 @@ def test_brute_force_tiny():
     np.testing.assert_allclose(Ko, K, atol=1e-12)
 
 
-def test_unrestricted_gradient_rejects_mgga(water_case):
-    basis, ob, pts, w = water_case
-    D = random_density(basis.nbf, 5, 3)
-    with pytest.raises(ValueError):
-        oracle.xc_gradient_unrestricted(ob, basis.shell_atom, 3, pts, w, TPSS,
-                                        np.vstack([D, D]) / 2)
-
-
 def random_mgga_points_pol(rng, n):
     """Open-shell points: spin ratios up to 1e3 either way, tau_s above
     tau_W,s, gradients of both spins in independent directions."""
