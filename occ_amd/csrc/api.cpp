@@ -1111,6 +1111,29 @@ void run_gradient_u(occxc_context *c, double *d_grad, hipStream_t s) {
   t.run(OCCXC_STAGE_PREPARE, [&] {
     launch_prepare_density_u(c->d_Du.get(), c->d_D2a.get(), c->d_D2b.get(), c->nbf, s);
   });
+  if (c->mgga) {
+    // process_grid_block_gradient<2, Unrestricted>: the unrestricted
+    // meta-GGA density stage, per spin the GGA and tau partial sets, one
+    // reduction of all four
+    const bool fp32 = prec != OCCXC_PREC_FP64;
+    t.run(OCCXC_STAGE_DENSITY, [&] { mgga_density_u(c, s); });
+    t.run(OCCXC_STAGE_GRADIENT, [&] {
+      BuildArgs x = a;
+      x.D2 = c->d_D2a.get();
+      x.coef_tau = c->d_coef_tau.get();
+      launch_xc_gradient_partials(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart.get(), s);
+      launch_xc_gradient_tau(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart3.get(), s);
+      x.D2 = c->d_D2b.get();
+      x.coef = c->d_coef_b.get();
+      x.coef_tau = c->d_coef_tau_b.get();
+      launch_xc_gradient_partials(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart2.get(), s);
+      launch_xc_gradient_tau(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart4.get(), s);
+      launch_grad_reduce(c->d_gpart.get(), c->d_gpart2.get(), c->grad_natoms,
+                         c->d_gatom_off.get(), c->d_gatom_ent.get(), d_grad, s,
+                         c->d_gpart3.get(), c->d_gpart4.get());
+    });
+    return;
+  }
   t.run(OCCXC_STAGE_DENSITY, [&] {
     BuildArgs x = a;
     x.rho_out = c->d_rho_a.get();
@@ -1499,8 +1522,6 @@ occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D,
                                             const int *shell_atom, double *grad) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
-    OCCXC_REQUIRE(!ctx->mgga, OCCXC_ERR_UNSUPPORTED,
-                  "meta-GGA gradients: restricted densities only");
     OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
                   "occxc_xc_gradient needs a single-molecule context");
     OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
@@ -1520,6 +1541,11 @@ occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D,
     DeviceGuard g(ctx->device);
     gradient_setup(ctx, natoms, shell_atom);
     uks_setup(ctx);
+    if (ctx->mgga && ctx->d_gpart3.size() < ctx->d_gpart.size()) {
+      ctx->d_gpart3.alloc(ctx->d_gpart.size());
+      ctx->d_gpart4.alloc(ctx->d_gpart.size());
+      ctx->device_bytes += ctx->d_gpart3.bytes() + ctx->d_gpart4.bytes();
+    }
     if (ctx->d_gpart2.size() < ctx->d_gpart.size()) {
       ctx->d_gpart2.alloc(ctx->d_gpart.size());
       ctx->device_bytes += ctx->d_gpart2.bytes();

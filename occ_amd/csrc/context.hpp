@@ -65,6 +65,8 @@ struct occxc_context {
   // shell-to-atom map of the last call
   std::vector<int> bf_shell;
   occxc::DevBuf<double> d_ao2, d_gpart, d_gpart2, d_grad;
+  // unrestricted meta-GGA gradient: each spin's tau partials
+  occxc::DevBuf<double> d_gpart3, d_gpart4;
   occxc::DevBuf<int> d_gatom_off, d_gatom_ent;
   bool ao2_ready = false;
   std::vector<int> grad_shell_atom;

@@ -289,6 +289,8 @@ __global__ __launch_bounds__(256, 2) void k_xc_grad_tau(BuildArgs a, const S *__
 // order (per-thread strided sums, then a fixed tree).
 __global__ __launch_bounds__(256) void k_grad_reduce(const double *__restrict__ gpart,
                                                      const double *__restrict__ gpart2,
+                                                     const double *__restrict__ gpart3,
+                                                     const double *__restrict__ gpart4,
                                                      const int *__restrict__ atom_off,
                                                      const int *__restrict__ atom_ent,
                                                      double *__restrict__ grad) {
@@ -305,6 +307,11 @@ __global__ __launch_bounds__(256) void k_grad_reduce(const double *__restrict__ 
       v[0] += gpart2[e];
       v[1] += gpart2[e + 1];
       v[2] += gpart2[e + 2];
+    }
+    if (gpart3) {
+      v[0] += gpart3[e] + gpart4[e];
+      v[1] += gpart3[e + 1] + gpart4[e + 1];
+      v[2] += gpart3[e + 2] + gpart4[e + 2];
     }
   }
 #pragma unroll
@@ -353,8 +360,9 @@ void launch_xc_gradient_tau(const BuildArgs &a, const void *ao, bool ao_fp32,
 }
 
 void launch_grad_reduce(const double *gpart, const double *gpart2, int natoms,
-                        const int *atom_off, const int *atom_ent, double *grad, hipStream_t s) {
-  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, gpart2, atom_off, atom_ent, grad);
+                        const int *atom_off, const int *atom_ent, double *grad, hipStream_t s,
+                        const double *gpart3, const double *gpart4) {
+  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, gpart2, gpart3, gpart4, atom_off, atom_ent, grad);
   check("k_grad_reduce");
 }
 
@@ -374,7 +382,8 @@ void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const 
     }
     check("k_xc_grad");
   }
-  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, nullptr, atom_off, atom_ent, grad);
+  k_grad_reduce<<<natoms, 256, 0, s>>>(gpart, nullptr, nullptr, nullptr, atom_off, atom_ent,
+                                       grad);
   check("k_grad_reduce");
 }
 

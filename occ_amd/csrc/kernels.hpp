@@ -165,8 +165,11 @@ void launch_xc_gradient_partials(const BuildArgs &a, const void *ao, bool ao_fp3
 // ao2) into gpart, reduced together with k_xc_grad's by launch_grad_reduce
 void launch_xc_gradient_tau(const BuildArgs &a, const void *ao, bool ao_fp32,
                             const double *ao2, double *gpart, hipStream_t s);
+// grad = -2 sum of up to four partial sets (gpart2, gpart3/gpart4 optional;
+// gpart3 and gpart4 come together)
 void launch_grad_reduce(const double *gpart, const double *gpart2, int natoms,
-                        const int *atom_off, const int *atom_ent, double *grad, hipStream_t s);
+                        const int *atom_off, const int *atom_ent, double *grad, hipStream_t s,
+                        const double *gpart3 = nullptr, const double *gpart4 = nullptr);
 void launch_xc_gradient(const BuildArgs &a, const void *ao, bool ao_fp32, const double *ao2,
                         double *gpart, int natoms, const int *atom_off, const int *atom_ent,
                         double *grad, hipStream_t s);

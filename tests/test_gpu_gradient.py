@@ -160,7 +160,7 @@ def test_gradient_argument_checks():
     assert st == 1 and b"single-molecule" in L.occxc_last_error_message()
 
 
-@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp", "tpss"])
 def test_unrestricted_gradient_matches_oracle(method):
     """compute_xc_gradient_impl<d, Unrestricted> on the GPU against the oracle
     (tolerance as above); its closed-shell limit equals the restricted GPU

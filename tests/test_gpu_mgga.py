@@ -144,7 +144,3 @@ def test_mgga_rejections():
     pts, wts = _grid(atoms, 21, 50)
     with pytest.raises(Exception):
         XCContext(basis, pts, wts, TPSS, precision="fp32")
-    dft = DFT("tpss", basis, pts, wts)
-    D = random_density(basis.nbf, 5, 3)
-    with pytest.raises(Exception):
-        dft.compute_xc_gradient(np.vstack([D, D]) / 2)  # unrestricted
