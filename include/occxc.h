@@ -59,7 +59,7 @@ enum {
   OCCXC_GGA_X_B88 = 106,
   OCCXC_GGA_C_PBE = 130,
   OCCXC_GGA_C_LYP = 131,
-  OCCXC_MGGA_X_TPSS = 202,    /* meta-GGA: builds (both spin cases), restricted gradient */
+  OCCXC_MGGA_X_TPSS = 202,    /* meta-GGA: builds and gradients, both spin cases */
   OCCXC_MGGA_C_TPSS = 231
 };
 
@@ -190,8 +190,9 @@ occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int
  * Unrestricted> (dft.h:598-749) with process_grid_block_gradient<d,
  * Unrestricted> (src/dft/dft_gradient_kernels.cpp:473-583) and the polarised
  * functionals.  D as occxc_build_unrestricted (2nbf x nbf, alpha block over
- * beta block), the rest as occxc_xc_gradient.  FP64 and FP32-cache contexts
- * of one molecule. */
+ * beta block), the rest as occxc_xc_gradient.  Meta-GGA ids add each spin's
+ * tau terms (accumulate_gradient_contribution_U<2>, dft_gradient_kernels.cpp:
+ * 306-470).  FP64 and FP32-cache contexts of one molecule. */
 occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D, int natoms,
                                             const int *shell_atom, double *grad);
 
