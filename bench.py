@@ -117,8 +117,8 @@ def main():
 
     if args.workload == "batch512":
         return run_batch(args)
-    if args.workload in ("grad32", "tpssgrad32"):
-        if args.workload == "tpssgrad32":
+    if args.workload in ("grad32", "tpssgrad32", "ugrad32", "utpssgrad32"):
+        if args.workload in ("tpssgrad32", "utpssgrad32", "ugrad32"):
             args.modes = ",".join(m for m in args.modes.split(",") if m in ("fp64", "fp32cache"))
         return run_gradient(args)
     if args.workload in ("uks32", "utpss32"):
@@ -273,20 +273,28 @@ def run_gradient(args):
     """XC nuclear gradient (compute_xc_gradient) of the (H2O)32 / def2-SVP /
     PBE workload: a step is one gradient from a resident context (D in ->
     3 x 96 gradient out: D2, the density stage and the gradient kernels); the
-    first call, which evaluates the AO Hessian once, is a warm-up."""
+    first call, which evaluates the AO Hessian once, is a warm-up.  ugrad32 /
+    utpssgrad32: the unrestricted gradient (PBE / TPSS) of the doublet cation
+    of the unrestricted build (D_s = C_s C_s^T / 2, 160 alpha, 159 beta)."""
     from occ_amd import AOBasis
     from occ_amd.dft import XCContext, functional_ids
     from occ_amd.device import synchronize
     from occ_amd.grid import GridSettings, MolecularGrid
-    atoms, bname, pure, method, nocc, desc = workload(
-        "tpss32" if args.workload == "tpssgrad32" else "h2o32")
+    uks = args.workload in ("ugrad32", "utpssgrad32")
+    mg = args.workload in ("tpssgrad32", "utpssgrad32")
+    atoms, bname, pure, method, nocc, desc = workload("tpss32" if mg else "h2o32")
     basis = AOBasis.load(atoms, bname, pure=pure)
     nbf = basis.nbf
     grid = MolecularGrid(atoms, GridSettings(radial_points=76, angular_points=302))
     pts, wts = grid.build(device=0)
-    rng = np.random.default_rng(0)
-    C, _ = np.linalg.qr(rng.normal(size=(nbf, nocc)))
-    D = np.asfortranarray(C @ C.T)
+    if uks:
+        D = np.asfortranarray(np.vstack([seeded_density(nbf, nocc, 0) / 2,
+                                         seeded_density(nbf, nocc - 1, 1) / 2]))
+        desc = "unrestricted " + desc + ", doublet cation (160 alpha, 159 beta)"
+    else:
+        rng = np.random.default_rng(0)
+        C, _ = np.linalg.qr(rng.normal(size=(nbf, nocc)))
+        D = np.asfortranarray(C @ C.T)
     ids = functional_ids(method)
     sa, nat = basis.shell_atom, len(atoms)
     ref, cpu = None, None
@@ -298,7 +306,8 @@ def run_gradient(args):
         times = []
         for _ in range(max(1, min(args.cpu_repeats, 3))):
             t0 = time.time()
-            ref = oracle.xc_gradient(ob, sa, nat, pts, wts, ids, D, nthreads=threads)
+            ref = (oracle.xc_gradient_unrestricted if uks else oracle.xc_gradient)(
+                ob, sa, nat, pts, wts, ids, D, nthreads=threads)
             times.append(time.time() - t0)
         dt = float(np.median(times))
         cpu = {"value": 1.0 / dt, "unit": "XC gradients/s", "cores": threads, "kind": "port",
@@ -309,21 +318,22 @@ def run_gradient(args):
                "cpu_model": cpu_model()}
         log(f"cpu oracle gradient: {dt:.2f}s on {threads} threads")
     from occ_amd.device import DeviceArray
-    d_D = DeviceArray(nbf * nbf)
+    d_D = DeviceArray(D.size)
     d_D.upload(D)
     d_G = DeviceArray(3 * nat)
     results = {}
     for mode in [m for m in args.modes.split(",") if m]:
         ctx = XCContext(basis, pts, wts, ids, precision=mode)
         info = ctx.info()
+        grad = ctx.xc_gradient_unrestricted_device if uks else ctx.xc_gradient_device
         for _ in range(max(args.warmup, 1)):
-            ctx.xc_gradient_device(d_D.ptr, sa, nat, d_G.ptr)
+            grad(d_D.ptr, sa, nat, d_G.ptr)
         ctx.synchronize()
         ctx.set_timing(True)
         synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            ctx.xc_gradient_device(d_D.ptr, sa, nat, d_G.ptr)
+            grad(d_D.ptr, sa, nat, d_G.ptr)
         ctx.synchronize()
         synchronize()
         wall = time.perf_counter() - t0
@@ -343,17 +353,19 @@ def run_gradient(args):
     tim = R["timing"]
     gms = tim["gradient"][0] / max(tim["gradient"][1], 1)
     flops = R["info"]["contraction_flops"]   # T = Phi P and U = Aow P: 2 x 2 n m^2
-    mg = args.workload == "tpssgrad32"
     if mg:
         flops *= 2.5   # + T_x, T_y, T_z of the tau terms: 5 x 2 n m^2
+    if uks:
+        flops *= 2     # once per spin
     achieved = flops / (gms * 1e-3) / 1e12
     roof = {"bound": "mfma", "kernel": "gradient", "achieved": achieved,
             "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[head],
             "traffic": measured_traffic(args.workload, head, "gradient"), "kernel_ms": gms,
             "work": "sum over leaves of 4 n_b m_b^2 (T = Phi P and U = Aow P, screened m_b)"
-                    + (" + 6 n_b m_b^2 (the three tau GEMMs)" if mg else "")}
+                    + (" + 6 n_b m_b^2 (the three tau GEMMs)" if mg else "")
+                    + (", per spin" if uks else "")}
     out = {
-        "metric": "XC nuclear gradients/s (FP64)",
+        "metric": ("unrestricted " if uks else "") + "XC nuclear gradients/s (FP64)",
         "value": R["per_s"], "unit": "XC gradients/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[head], "data": "synthetic",

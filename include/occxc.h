@@ -196,6 +196,14 @@ occxc_status occxc_xc_gradient_device(occxc_context *ctx, const double *d_D, int
 occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D, int natoms,
                                             const int *shell_atom, double *grad);
 
+/* Same unrestricted gradient on device-resident buffers (d_D 2nbf*nbf as
+ * above, d_grad 3*natoms), enqueued on `stream` (NULL: the context's own
+ * stream); shell_atom stays a host array.  Mirrors occxc_xc_gradient_device;
+ * D is not checked for non-finite entries on this path. */
+occxc_status occxc_xc_gradient_unrestricted_device(occxc_context *ctx, const double *d_D,
+                                                   int natoms, const int *shell_atom,
+                                                   double *d_grad, void *stream);
+
 /* --- batches: many molecules in one context, one launch per stage ----- */
 
 /* A context over nmol molecules (each its own basis and grid; shared

@@ -96,6 +96,9 @@ SIGNATURES = {
                                              ctypes.c_void_p]),
     "occxc_xc_gradient_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                 _ip, ctypes.c_void_p, ctypes.c_void_p]),
+    "occxc_xc_gradient_unrestricted_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                             ctypes.c_int, _ip, ctypes.c_void_p,
+                                                             ctypes.c_void_p]),
     "occxc_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(CBasis),
                                           ctypes.POINTER(CGrid), ctypes.POINTER(CFunctional),
                                           ctypes.POINTER(COptions),

@@ -1518,42 +1518,65 @@ occxc_status occxc_build_unrestricted_device(occxc_context *ctx, const double *d
   });
 }
 
+// Checks and set-up shared by the two unrestricted gradient entry points:
+// the context and shell map, the Hessian cache and per-atom lists, the
+// unrestricted buffers and the extra partial sets.
+static void gradient_u_prepare(occxc_context *ctx, int natoms, const int *shell_atom) {
+  OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
+                "occxc_xc_gradient needs a single-molecule context");
+  OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
+  for (int k = 0; k < ctx->nsh; ++k)
+    OCCXC_REQUIRE(shell_atom[k] >= 0 && shell_atom[k] < natoms, OCCXC_ERR_INVALID_ARGUMENT,
+                  "shell_atom: atom index out of range");
+  OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
+                    ctx->opt.precision == OCCXC_PREC_FP32_CACHE,
+                OCCXC_ERR_UNSUPPORTED,
+                "unrestricted gradients need an FP64 or FP32-cache context");
+  OCCXC_REQUIRE(ctx->plan.use_mfma && ctx->plan.max_mp <= kStoreMaxMp, OCCXC_ERR_UNSUPPORTED,
+                "unrestricted gradients need the matrix-core kernels and leaves of at most " +
+                    std::to_string(kStoreMaxMp) + " AOs");
+  gradient_setup(ctx, natoms, shell_atom);
+  uks_setup(ctx);
+  if (ctx->mgga && ctx->d_gpart3.size() < ctx->d_gpart.size()) {
+    ctx->d_gpart3.alloc(ctx->d_gpart.size());
+    ctx->d_gpart4.alloc(ctx->d_gpart.size());
+    ctx->device_bytes += ctx->d_gpart3.bytes() + ctx->d_gpart4.bytes();
+  }
+  if (ctx->d_gpart2.size() < ctx->d_gpart.size()) {
+    ctx->d_gpart2.alloc(ctx->d_gpart.size());
+    ctx->device_bytes += ctx->d_gpart2.bytes();
+  }
+}
+
 occxc_status occxc_xc_gradient_unrestricted(occxc_context *ctx, const double *D, int natoms,
                                             const int *shell_atom, double *grad) {
   return guarded([&] {
     OCCXC_REQUIRE(ctx && D && shell_atom && grad, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
-    OCCXC_REQUIRE(ctx->nmol == 1, OCCXC_ERR_INVALID_ARGUMENT,
-                  "occxc_xc_gradient needs a single-molecule context");
-    OCCXC_REQUIRE(natoms > 0, OCCXC_ERR_INVALID_ARGUMENT, "natoms must be positive");
-    for (int k = 0; k < ctx->nsh; ++k)
-      OCCXC_REQUIRE(shell_atom[k] >= 0 && shell_atom[k] < natoms, OCCXC_ERR_INVALID_ARGUMENT,
-                    "shell_atom: atom index out of range");
-    OCCXC_REQUIRE(ctx->opt.precision == OCCXC_PREC_FP64 ||
-                      ctx->opt.precision == OCCXC_PREC_FP32_CACHE,
-                  OCCXC_ERR_UNSUPPORTED,
-                  "unrestricted gradients need an FP64 or FP32-cache context");
-    OCCXC_REQUIRE(ctx->plan.use_mfma && ctx->plan.max_mp <= kStoreMaxMp, OCCXC_ERR_UNSUPPORTED,
-                  "unrestricted gradients need the matrix-core kernels and leaves of at most " +
-                      std::to_string(kStoreMaxMp) + " AOs");
     const size_t nn = (size_t)ctx->nbf * ctx->nbf;
     for (size_t i = 0; i < 2 * nn; ++i)
       OCCXC_REQUIRE(std::isfinite(D[i]), OCCXC_ERR_INVALID_ARGUMENT, "D: non-finite entry");
     DeviceGuard g(ctx->device);
-    gradient_setup(ctx, natoms, shell_atom);
-    uks_setup(ctx);
-    if (ctx->mgga && ctx->d_gpart3.size() < ctx->d_gpart.size()) {
-      ctx->d_gpart3.alloc(ctx->d_gpart.size());
-      ctx->d_gpart4.alloc(ctx->d_gpart.size());
-      ctx->device_bytes += ctx->d_gpart3.bytes() + ctx->d_gpart4.bytes();
-    }
-    if (ctx->d_gpart2.size() < ctx->d_gpart.size()) {
-      ctx->d_gpart2.alloc(ctx->d_gpart.size());
-      ctx->device_bytes += ctx->d_gpart2.bytes();
-    }
+    gradient_u_prepare(ctx, natoms, shell_atom);
     ctx->d_Du.upload(D, 2 * nn, ctx->stream);
     run_gradient_u(ctx, ctx->d_grad.get(), ctx->stream);
     ctx->d_grad.download(grad, 3 * (size_t)natoms, ctx->stream);
     OCCXC_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+occxc_status occxc_xc_gradient_unrestricted_device(occxc_context *ctx, const double *d_D,
+                                                   int natoms, const int *shell_atom,
+                                                   double *d_grad, void *stream) {
+  return guarded([&] {
+    OCCXC_REQUIRE(ctx && d_D && shell_atom && d_grad, OCCXC_ERR_INVALID_ARGUMENT,
+                  "null pointer");
+    DeviceGuard g(ctx->device);
+    gradient_u_prepare(ctx, natoms, shell_atom);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    const size_t nn = (size_t)ctx->nbf * ctx->nbf;
+    OCCXC_HIP(hipMemcpyAsync(ctx->d_Du.get(), d_D, 2 * nn * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    run_gradient_u(ctx, d_grad, s);
   });
 }
 

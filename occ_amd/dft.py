@@ -184,6 +184,15 @@ class XCContext:
                                                        ctypes.c_void_p(d_grad),
                                                        ctypes.c_void_p(stream or 0)))
 
+    def xc_gradient_unrestricted_device(self, d_D, shell_atom, natoms, d_grad, stream=None):
+        """occxc_xc_gradient_unrestricted_device: D (2 nbf x nbf) and the
+        3 x natoms gradient in device memory, enqueued without host
+        synchronisation."""
+        self._sa = _lib.i32(shell_atom)
+        _lib.check(_lib.lib().occxc_xc_gradient_unrestricted_device(
+            self._h, ctypes.c_void_p(d_D), int(natoms), _lib.iptr(self._sa),
+            ctypes.c_void_p(d_grad), ctypes.c_void_p(stream or 0)))
+
     STAGES = ("prepare", "density", "vxc", "finalize", "ao", "gradient", "nlc")
 
     def set_timing(self, enable=True):

@@ -226,3 +226,24 @@ def test_tpss_gradient_device_entry_and_build_after():
     V1, e1, _ = ctx.build(D)
     assert abs(e1 - e0) <= 1e-12 and np.abs(V1 - V0).max() <= 1e-14
     ctx.close()
+
+
+@pytest.mark.parametrize("method", ["pbe", "tpss"])
+def test_unrestricted_gradient_device_entry(method):
+    """occxc_xc_gradient_unrestricted_device gives the host entry's gradient,
+    twice in a row."""
+    from occ_amd.device import DeviceArray
+    atoms = molecules.water()
+    basis = AOBasis.load(atoms, "6-31g", pure=False)
+    pts, wts = _grid(atoms)
+    n = basis.nbf
+    Du = np.vstack([random_density(n, 5, 71) / 2, random_density(n, 4, 72) / 2])
+    ctx = XCContext(basis, pts, wts, functional_ids(method))
+    G = ctx.xc_gradient_unrestricted(Du, basis.shell_atom, 3)
+    dD, dG = DeviceArray(2 * n * n), DeviceArray(9)
+    dD.upload(np.asfortranarray(Du))
+    for _ in range(2):
+        ctx.xc_gradient_unrestricted_device(dD.ptr, basis.shell_atom, 3, dG.ptr)
+        ctx.synchronize()
+        assert np.abs(dG.download((3, 3)) - G).max() <= 1e-14
+    ctx.close()
