@@ -369,6 +369,18 @@ occxc_status occxc_functional_evaluate_mgga(int id, long long n, const double *r
                                             double *exc, double *vrho, double *vsigma,
                                             double *vtau, int device);
 
+/* DensityFunctional::evaluate (src/dft/functional.cpp:49-57) with
+ * Family::MGGA on a spin-polarised density, libxc's polarised layout:
+ * rho[2n] = (rho_a, rho_b), sigma[3n] = (s_aa, s_ab, s_bb), tau[2n] =
+ * (tau_a, tau_b) per point (dft_kernels.h:51-57); ACCUMULATES exc[n] (per
+ * particle), vrho[2n], vsigma[3n], vtau[2n] (vlaplacian is zero for every
+ * functional here).  LDA and GGA ids are accepted (tau / sigma ignored as
+ * their family allows).  Runs on GPU `device`. */
+occxc_status occxc_functional_evaluate_mgga_polarized(int id, long long n, const double *rho,
+                                                      const double *sigma, const double *tau,
+                                                      double *exc, double *vrho, double *vsigma,
+                                                      double *vtau, int device);
+
 /* The same for a spin-polarised density (DensityFunctional with
  * polarized = true, functional.cpp:18-65), libxc's layout: rho[2n] =
  * (rho_a, rho_b) per point, sigma[3n] = (s_aa, s_ab, s_bb); ACCUMULATES

@@ -87,6 +87,9 @@ SIGNATURES = {
                                                            _dp, _dp, _dp, _dp, ctypes.c_int]),
     "occxc_functional_evaluate_mgga": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, _dp, _dp,
                                                       _dp, _dp, _dp, _dp, _dp, ctypes.c_int]),
+    "occxc_functional_evaluate_mgga_polarized": (ctypes.c_int, [ctypes.c_int, ctypes.c_longlong,
+                                                                _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                                                ctypes.c_int]),
     "occxc_lmg_radial_grid": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
                                              ctypes.c_double, ctypes.c_int, _dp, _dp, _ip]),
     "occxc_nlc_vv10": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double, _dp,

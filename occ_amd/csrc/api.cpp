@@ -1759,6 +1759,50 @@ occxc_status occxc_functional_evaluate_mgga(int id, long long n, const double *r
   });
 }
 
+occxc_status occxc_functional_evaluate_mgga_polarized(int id, long long n, const double *rho,
+                                                      const double *sigma, const double *tau,
+                                                      double *exc, double *vrho, double *vsigma,
+                                                      double *vtau, int device) {
+  return guarded([&] {
+    const int fam = xc_family_mgga(id);
+    OCCXC_REQUIRE(fam >= 0, OCCXC_ERR_UNSUPPORTED, "functional id " + std::to_string(id) + " not found");
+    OCCXC_REQUIRE(n >= 0 && (n == 0 || (rho && exc && vrho)), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_mgga_polarized: null pointer");
+    OCCXC_REQUIRE(fam == 0 || n == 0 || (sigma && vsigma), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_mgga_polarized: GGA and meta-GGA need sigma and vsigma");
+    OCCXC_REQUIRE(fam < 2 || n == 0 || (tau && vtau), OCCXC_ERR_INVALID_ARGUMENT,
+                  "functional_evaluate_mgga_polarized: meta-GGA needs tau and vtau");
+    if (n == 0) return;
+    DeviceGuard g(device);
+    const bool gga = fam >= 1, mg = fam == 2;
+    DevBuf<double> dr(2 * n), ds(gga ? 3 * n : 1), dt(mg ? 2 * n : 1), de(n), dv(2 * n),
+        dw(gga ? 3 * n : 1), dx(mg ? 2 * n : 1);
+    dr.upload(rho, 2 * n);
+    if (gga) ds.upload(sigma, 3 * n);
+    if (mg) dt.upload(tau, 2 * n);
+    launch_zero(de.get(), n, nullptr);
+    launch_zero(dv.get(), 2 * n, nullptr);
+    launch_zero(dw.get(), gga ? 3 * n : 1, nullptr);
+    launch_zero(dx.get(), mg ? 2 * n : 1, nullptr);
+    launch_functional_mgga_pol(id, n, dr.get(), ds.get(), dt.get(), de.get(), dv.get(),
+                               dw.get(), dx.get(), nullptr);
+    std::vector<double> e(n), v(2 * n), w(gga ? 3 * n : 0), x(mg ? 2 * n : 0);
+    de.download(e.data(), n);
+    dv.download(v.data(), 2 * n);
+    if (gga) dw.download(w.data(), 3 * n);
+    if (mg) dx.download(x.data(), 2 * n);
+    OCCXC_HIP(hipDeviceSynchronize());
+    for (long long i = 0; i < n; ++i) {
+      exc[i] += e[i];
+      for (int k = 0; k < 2; ++k) vrho[2 * i + k] += v[2 * i + k];
+      if (gga)
+        for (int k = 0; k < 3; ++k) vsigma[3 * i + k] += w[3 * i + k];
+      if (mg)
+        for (int k = 0; k < 2; ++k) vtau[2 * i + k] += x[2 * i + k];
+    }
+  });
+}
+
 occxc_status occxc_functional_evaluate_polarized(int id, long long n, const double *rho,
                                                  const double *sigma, double *exc, double *vrho,
                                                  double *vsigma, int device) {

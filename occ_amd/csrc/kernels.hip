@@ -732,6 +732,43 @@ __global__ void k_functional_pol_points(int id, long long n, const double *__res
     for (int k = 0; k < 3; ++k) vsigma[3 * i + k] += vs[k];
 }
 
+// Polarised meta-GGA (and LDA / GGA) functionals over points, ADDING to the
+// outputs in libxc's polarised layout: rho[2n], sigma[3n], tau[2n] in;
+// exc[n] (per particle), vrho[2n], vsigma[3n], vtau[2n] out.
+__global__ void k_functional_mgga_pol_points(int id, long long n, const double *__restrict__ rho,
+                                             const double *__restrict__ sigma,
+                                             const double *__restrict__ tau,
+                                             double *__restrict__ exc, double *__restrict__ vrho,
+                                             double *__restrict__ vsigma,
+                                             double *__restrict__ vtau) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int fam = xc_family_mgga(id);
+  const double ra = rho[2 * i], rb = rho[2 * i + 1];
+  if (!(ra + rb >= kDensThreshold)) return;
+  double saa = 0, sab = 0, sbb = 0, ta = 0, tb = 0;
+  if (fam >= 1) {
+    saa = fmax(sigma[3 * i], kSigmaMin);
+    sab = sigma[3 * i + 1];
+    sbb = fmax(sigma[3 * i + 2], kSigmaMin);
+  }
+  if (fam == 2) {
+    ta = tau[2 * i];
+    tb = tau[2 * i + 1];
+  }
+  double e = 0.0, vr[2] = {0, 0}, vs[3] = {0, 0, 0}, vt[2] = {0, 0};
+  xc_point_mgga_pol(id, ra, rb, saa, sab, sbb, ta, tb, e, vr, vs, vt);
+  exc[i] += e / (ra + rb);
+  vrho[2 * i] += vr[0];
+  vrho[2 * i + 1] += vr[1];
+  if (fam >= 1)
+    for (int k = 0; k < 3; ++k) vsigma[3 * i + k] += vs[k];
+  if (fam == 2) {
+    vtau[2 * i] += vt[0];
+    vtau[2 * i + 1] += vt[1];
+  }
+}
+
 __global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V,
                                   const MolDev *__restrict__ mols) {
   const MolDev M = mols[blockIdx.y];
@@ -1066,6 +1103,15 @@ void launch_functional_pol(int id, long long n, const double *rho, const double 
                            double *exc, double *vrho, double *vsigma, hipStream_t s) {
   if (n == 0) return;
   k_functional_pol_points<<<grid_for(n, 256), 256, 0, s>>>(id, n, rho, sigma, exc, vrho, vsigma);
+  check_launch();
+}
+
+void launch_functional_mgga_pol(int id, long long n, const double *rho, const double *sigma,
+                                const double *tau, double *exc, double *vrho, double *vsigma,
+                                double *vtau, hipStream_t s) {
+  if (n == 0) return;
+  k_functional_mgga_pol_points<<<grid_for(n, 256), 256, 0, s>>>(id, n, rho, sigma, tau, exc,
+                                                                 vrho, vsigma, vtau);
   check_launch();
 }
 

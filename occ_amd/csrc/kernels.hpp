@@ -189,6 +189,10 @@ void launch_finalize_u(const double *Ka, const double *Kb, double *Va, double *V
                        double *result, hipStream_t s);
 void launch_functional_pol(int id, long long n, const double *rho, const double *sigma,
                            double *exc, double *vrho, double *vsigma, hipStream_t s);
+// polarised meta-GGA over points (ADDING; rho[2n], sigma[3n], tau[2n])
+void launch_functional_mgga_pol(int id, long long n, const double *rho, const double *sigma,
+                                const double *tau, double *exc, double *vrho, double *vsigma,
+                                double *vtau, hipStream_t s);
 // restricted meta-GGA: the per-leaf functional stage from the stored rho /
 // grad rho (rho4, [4][ptot]) and per-direction tau passes (tk, [3][ptot]);
 // coefficients into a.coef and coef_tau, {Exc, N_elec} into a.leaf_part

@@ -420,6 +420,18 @@ class DensityFunctional:
             _lib.dptr(vrho), _lib.dptr(vsig), _lib.dptr(vtau), device))
         return exc, vrho, vsig, vtau
 
+    def evaluate_mgga_polarized(self, rho, sigma, tau, device=0):
+        """Family::MGGA on a spin-polarised density: rho (n, 2), sigma (n, 3),
+        tau (n, 2) -> exc (n), vrho (n, 2), vsigma (n, 3), vtau (n, 2)."""
+        rho = _lib.f64(rho).reshape(-1, 2)
+        n = rho.shape[0]
+        sig, t = _lib.f64(sigma).reshape(-1, 3), _lib.f64(tau).reshape(-1, 2)
+        exc, vrho, vsig, vtau = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3)), np.zeros((n, 2))
+        _lib.check(_lib.lib().occxc_functional_evaluate_mgga_polarized(
+            self.id, n, _lib.dptr(rho), _lib.dptr(sig), _lib.dptr(t), _lib.dptr(exc),
+            _lib.dptr(vrho), _lib.dptr(vsig), _lib.dptr(vtau), device))
+        return exc, vrho, vsig, vtau
+
     def evaluate_polarized(self, rho, sigma=None, device=0):
         """Spin-polarised: rho (n, 2), sigma (n, 3) -> exc (n), vrho (n, 2),
         vsigma (n, 3) (None for LDA)."""

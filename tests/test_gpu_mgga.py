@@ -144,3 +144,19 @@ def test_mgga_rejections():
     pts, wts = _grid(atoms, 21, 50)
     with pytest.raises(Exception):
         XCContext(basis, pts, wts, TPSS, precision="fp32")
+
+
+@pytest.mark.parametrize("name,fid", [("mgga_x_tpss", 202), ("mgga_c_tpss", 231)])
+def test_polarised_tpss_matches_oracle(rng, name, fid):
+    """occxc_functional_evaluate_mgga_polarized against the oracle's
+    polarised TPSS on open-shell points (spin ratios up to 1e3, tau above
+    tau_W, a tenth of them with tau below tau_W, which the limits raise)."""
+    from test_oracle_mgga import random_mgga_points_pol
+    rho, sig, tau = random_mgga_points_pol(rng, 4000)
+    tau[:400] *= 0.3
+    exc, vr, vs, vt = DensityFunctional(name).evaluate_mgga_polarized(rho, sig, tau)
+    e0, r0, s0, t0 = oracle.functional_eval_mgga_pol(fid, rho, sig, tau)
+    np.testing.assert_allclose(exc, e0, rtol=1e-11, atol=1e-300)
+    np.testing.assert_allclose(vr, r0, rtol=1e-9, atol=1e-13 * np.abs(r0).max())
+    np.testing.assert_allclose(vs, s0, rtol=1e-9, atol=1e-13 * np.abs(s0).max())
+    np.testing.assert_allclose(vt, t0, rtol=1e-9, atol=1e-13 * np.abs(t0).max())
