@@ -140,7 +140,8 @@ OCCXC_HD inline D5 ec_tpss_pol(const D5 &ra, const D5 &rb, const D5 &saa, const 
 // into e (= n eps), vr[2], vs[3] (aa, ab, bb), vt[2] like xc_point_pol.  The
 // seven partials come from two passes over the five dual slots: (rho_a,
 // rho_b, s_aa, s_ab, s_bb), then (tau_a, tau_b).  Inputs are limited per
-// spin as for the closed-shell form (tau_s >= 1e-20, s_ss <= 8 n_s tau_s).
+// spin as for the closed-shell form (tau_s >= 1e-20, s_ss <= 8 n_s tau_s),
+// then |s_ab| <= (s_aa + s_bb) / 2.
 // LDA and GGA ids go to xc_point_pol.  Returns false for an unknown id.
 OCCXC_HD inline bool xc_point_mgga_pol(int id, double ra, double rb, double saa, double sab,
                                        double sbb, double ta, double tb, double &e, double *vr,
@@ -150,6 +151,10 @@ OCCXC_HD inline bool xc_point_mgga_pol(int id, double ra, double rb, double saa,
   tb = fmax(tb, kTauMin);
   saa = fmin(fmax(saa, kSigmaMin), 8.0 * ra * ta);
   sbb = fmin(fmax(sbb, kSigmaMin), 8.0 * rb * tb);
+  // |grad n_a . grad n_b| <= (s_aa + s_bb) / 2 after the per-spin limits, so
+  // that |grad n|^2 and |grad zeta|^2 stay non-negative
+  const double sab_max = 0.5 * (saa + sbb);
+  sab = fmin(fmax(sab, -sab_max), sab_max);
   const bool x = id == XC_MGGA_X_TPSS;
   {
     const D5 a = d5v(ra, 0), b = d5v(rb, 1), s0 = d5v(saa, 2), s1 = d5v(sab, 3), s2 = d5v(sbb, 4);

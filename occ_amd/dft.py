@@ -422,14 +422,25 @@ class DensityFunctional:
 
     def evaluate_mgga_polarized(self, rho, sigma, tau, device=0):
         """Family::MGGA on a spin-polarised density: rho (n, 2), sigma (n, 3),
-        tau (n, 2) -> exc (n), vrho (n, 2), vsigma (n, 3), vtau (n, 2)."""
+        tau (n, 2) -> exc (n), vrho (n, 2), vsigma (n, 3), vtau (n, 2).  For an
+        LDA id sigma and tau may be None (vsigma and vtau are then None), for
+        a GGA id tau may be None (vtau is then None)."""
         rho = _lib.f64(rho).reshape(-1, 2)
         n = rho.shape[0]
-        sig, t = _lib.f64(sigma).reshape(-1, 3), _lib.f64(tau).reshape(-1, 2)
-        exc, vrho, vsig, vtau = np.zeros(n), np.zeros((n, 2)), np.zeros((n, 3)), np.zeros((n, 2))
+        gga = self.id > 100
+        mgga = self.id in MGGA_IDS
+        sig = _lib.f64(sigma).reshape(-1, 3) if gga else None
+        t = _lib.f64(tau).reshape(-1, 2) if mgga else None
+        exc, vrho = np.zeros(n), np.zeros((n, 2))
+        vsig = np.zeros((n, 3)) if gga else None
+        vtau = np.zeros((n, 2)) if mgga else None
+
+        def ptr(a):
+            return _lib.dptr(a) if a is not None else None
+
         _lib.check(_lib.lib().occxc_functional_evaluate_mgga_polarized(
-            self.id, n, _lib.dptr(rho), _lib.dptr(sig), _lib.dptr(t), _lib.dptr(exc),
-            _lib.dptr(vrho), _lib.dptr(vsig), _lib.dptr(vtau), device))
+            self.id, n, _lib.dptr(rho), ptr(sig), ptr(t), _lib.dptr(exc), _lib.dptr(vrho),
+            ptr(vsig), ptr(vtau), device))
         return exc, vrho, vsig, vtau
 
     def evaluate_polarized(self, rho, sigma=None, device=0):

@@ -913,6 +913,10 @@ static void mgga_inputs_pol(double *x) {
     if (*t < ORACLE_TAU_MIN) *t = ORACLE_TAU_MIN;
     if (*sg > 8.0 * x[s] * *t) *sg = 8.0 * x[s] * *t;
   }
+  /* |grad n_a . grad n_b| <= (s_aa + s_bb) / 2 after the per-spin limits */
+  double sab_max = 0.5 * (x[2] + x[4]);
+  if (x[3] > sab_max) x[3] = sab_max;
+  if (x[3] < -sab_max) x[3] = -sab_max;
 }
 
 double oracle_functional_energy_density_mgga_pol(int id, const double *in) {

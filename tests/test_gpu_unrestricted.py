@@ -143,7 +143,7 @@ def test_uks_rejections():
     assert st == 1
 
 
-@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp"])
+@pytest.mark.parametrize("method", ["svwn5", "pbe", "blyp", "tpss"])
 def test_uks_one_electron_beta_density_zero(method):
     """A one-electron system (hydrogen atom, D_beta = 0): rho_beta vanishes
     everywhere and zeta = 1; the build stays finite and matches the oracle."""

@@ -404,3 +404,38 @@ def test_gpu_mgga_pol_header_matches_oracle(mgga_pol_shim, fid, rng):
             # The build multiplies vsigma_bb by grad rho_b = 0, so it is unused.
             got[4] = ref[4]
         np.testing.assert_allclose(got, ref, rtol=1e-9, atol=1e-13 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("fid", TPSS)
+def test_polarised_tpss_limits_sigma_ab(mgga_pol_shim, fid, rng):
+    """Where tau_s < tau_W,s the per-spin limits lower s_aa and s_bb to
+    8 n_s tau_s; s_ab is then held to +-(s_aa + s_bb) / 2 (as libxc does), so
+    |grad n|^2 = s_aa + 2 s_ab + s_bb and |grad zeta|^2 stay non-negative.
+    The results are finite, equal those at the limited s_ab, and the GPU
+    header equals the oracle there."""
+    import ctypes
+    rho, sig, tau = random_mgga_points_pol(rng, 200)
+    # parallel or antiparallel spin gradients: |s_ab| = sqrt(s_aa s_bb)
+    sgn = np.where(np.arange(200) % 2 == 0, 1.0, -1.0)
+    sig[:, 1] = sgn * np.sqrt(sig[:, 0] * sig[:, 2])
+    tau *= 0.05                                   # every point below tau_W
+    saa = np.minimum(sig[:, 0], 8 * rho[:, 0] * tau[:, 0])
+    sbb = np.minimum(sig[:, 2], 8 * rho[:, 1] * tau[:, 1])
+    lim = 0.5 * (saa + sbb)
+    assert np.mean(np.abs(sig[:, 1]) > lim) > 0.1   # the s_ab limit acts
+    held = sig.copy()
+    held[:, 1] = np.clip(sig[:, 1], -lim, lim)
+    e0, v0, s0, t0 = oracle.functional_eval_mgga_pol(fid, rho, sig, tau)
+    e1, v1, s1, t1 = oracle.functional_eval_mgga_pol(fid, rho, held, tau)
+    assert np.all(np.isfinite(e0)) and np.all(np.isfinite(v0)) and np.all(np.isfinite(s0))
+    np.testing.assert_array_equal(e0, e1)
+    np.testing.assert_array_equal(v0, v1)
+    out = np.zeros(8)
+    for i in range(len(rho)):
+        x = np.concatenate([rho[i], sig[i], tau[i]])
+        assert mgga_pol_shim.shim(fid, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+        assert np.all(np.isfinite(out))
+        assert out[0] == pytest.approx(e0[i] * rho[i].sum(), rel=1e-11, abs=1e-300)
+        ref = np.concatenate([v0[i], s0[i], t0[i]])
+        np.testing.assert_allclose(out[1:], ref, rtol=1e-9, atol=1e-13 * np.abs(ref).max())
