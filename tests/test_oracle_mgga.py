@@ -233,8 +233,10 @@ def test_polarised_tpss_correlation_limits(rng):
                                                  np.stack([saa / (8 * ra), z0], 1))
     epbe = np.array([oracle.energy_density_pol(130, a, 0.0, s, 0.0, 0.0) / a
                      for a, s in zip(ra, saa)])
-    # z = tau_W / tau is 1 up to the rounding of saa / (8 ra)
-    assert np.all(np.abs(e) <= 1e-14 * np.abs(epbe))
+    # z = tau_W / tau is 1 up to the rounding of saa / (8 ra) and the 1e-20
+    # floor that tau_b = 0 is raised to (it enters tau_a + tau_b)
+    ta = saa / (8 * ra)
+    assert np.all(np.abs(e) <= (1e-14 + 4e-20 / ta) * np.abs(epbe))
     zz = 10 ** rng.uniform(-1, -0.01, n)
     e, _, _, _ = oracle.functional_eval_mgga_pol(231, np.stack([ra, z0], 1),
                                                  np.stack([saa, z0, z0], 1),
