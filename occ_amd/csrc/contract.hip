@@ -37,6 +37,9 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+// four doubles at a 16-byte (not 32-byte) aligned LDS address
+typedef double d4u __attribute__((ext_vector_type(4))) __attribute__((aligned(16)));
+typedef double d2u __attribute__((ext_vector_type(2))) __attribute__((aligned(16)));
 
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -1381,6 +1384,285 @@ __global__ __launch_bounds__(256) void k_vxc_reduce_final(BuildArgs a, double *_
   }
 }
 
+// ------------------------------------------------------------ tiled path --
+// Work items are (leaf, AO block) tiles (kernels.hpp: DenItem, VxcItem), so a
+// leaf wider than one block is spread over many workgroups and each
+// wavefront works on a 2D block of up to 4 x 4 16x16 MFMA tiles (FP64
+// arithmetic; S is the AO storage type).
+
+constexpr int kTKc = 32;         // AOs per density k-chunk
+constexpr int kTKld = kTKc + 4;  // LDS row stride (doubles) of the [row][k] density operands
+constexpr int kTVp = 16;         // points per Vxc chunk
+constexpr int kTVld = kTVp + 2;  // LDS row stride (doubles) of the [row][point] Vxc operands
+
+// Density item: T(p, nu) = sum_mu Phi(p, mu) D2(mu, nu) for the item's points p
+// and columns nu, over all of the leaf's AOs mu in k-chunks staged through LDS
+// (Phi transposed to [point][mu], D2 gathered through the AO map to
+// [nu][mu]); wavefront (wp, wc) owns half the points and half the columns.
+// Then the column-block partials of rho = sum_nu Phi T and
+// grad rho = 2 sum_nu grad Phi T (density.cpp:32-43); the leaf's only item
+// runs the functional epilogue at once, the others store their partials.
+template <typename S, bool GGA>
+__global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__restrict__ ao,
+                                                     const DenItem *__restrict__ items,
+                                                     double *__restrict__ part) {
+  __shared__ double s_phi[kTileAo * kTKld];  // [point][k]; after the k loop s_q and s_h
+  __shared__ double s_d[kTileAo * kTKld];    // [column][k]
+  __shared__ double s_red[4];
+  static_assert(4 * kMaxLeafPts + 4 * kTileAo <= kTileAo * kTKld, "epilogue sums fit s_phi");
+  double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(s_phi);
+  double(*s_h)[kTileAo] = reinterpret_cast<double(*)[kTileAo]>(s_phi + 4 * kMaxLeafPts);
+  const DenItem it = items[blockIdx.x];
+  if (it.leaf < 0) return;
+  const LeafDev L = a.leaves[it.leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wp = wave & 1, wc = wave >> 1;
+  const int hp = it.np >> 1, hc = it.wc >> 1;  // points / columns per wavefront
+  const int RT = hp >> 4, CT = hc >> 4;        // 16x16 tiles per wavefront
+  // f64 MFMA: feeding point 4(i%4) + i/4 as row i gives lane group g the
+  // accumulator rows of points 4g .. 4g+3 (one vector load in the epilogue)
+  const int pa = 4 * (l16 & 3) + (l16 >> 2);
+  const S *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const double *Dm = a.D2 + MD.moff;
+  const int *map = a.ao_map + L.map_off;
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int pg = it.np >> 2;  // 4-point groups per AO row
+  for (int k0 = 0; k0 < L.mp; k0 += kTKc) {
+    __syncthreads();
+    for (int gi = t; gi < kTKc * pg; gi += 256) {
+      const int mu = gi / pg, p = 4 * (gi - mu * pg);
+      const d4 v = load4_as<double>(phi + (long long)(k0 + mu) * L.np + it.p0 + p);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) s_phi[(p + e) * kTKld + mu] = v[e];
+    }
+    for (int gi = t; gi < it.wc * (kTKc / 4); gi += 256) {
+      const int col = gi / (kTKc / 4), kk = 4 * (gi - col * (kTKc / 4));
+      const double *dc = Dm + (long long)map[it.c0 + col] * MD.nbf;
+      const int4 mk = *reinterpret_cast<const int4 *>(map + k0 + kk);
+      *reinterpret_cast<d4 *>(s_d + col * kTKld + kk) = d4{dc[mk.x], dc[mk.y], dc[mk.z], dc[mk.w]};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kTKc; ks += 16) {
+      d4 av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (i < RT) av[i] = *reinterpret_cast<const d4 *>(s_phi + (wp * hp + 16 * i + pa) * kTKld + ks + 4 * g);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < CT) bv[j] = *reinterpret_cast<const d4 *>(s_d + (wc * hc + 16 * j + l16) * kTKld + ks + 4 * g);
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (i < RT && j < CT)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][s4], bv[j][s4], acc[i][j], 0, 0, 0);
+    }
+  }
+  __syncthreads();  // s_phi is reused for the sums below
+  // acc[i][j][r] = T(point wp*hp + 16 i + 4 g + r, column wc*hc + 16 j + l16)
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i >= RT) break;
+    const int pl = wp * hp + 16 * i + 4 * g;  // item-local point of r = 0
+    double q[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[r][k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= CT) break;
+      const long long off = (long long)(it.c0 + wc * hc + 16 * j + l16) * L.np + it.p0 + pl;
+      const d4 f = load4_as<double>(phi + off);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q[r][0] += f[r] * acc[i][j][r];
+      if constexpr (GGA) {
+        const d4 fx = load4_as<double>(phi + cs + off);
+        const d4 fy = load4_as<double>(phi + 2 * cs + off);
+        const d4 fz = load4_as<double>(phi + 3 * cs + off);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double tt = 2.0 * acc[i][j][r];
+          q[r][1] += fx[r] * tt;
+          q[r][2] += fy[r] * tt;
+          q[r][3] += fz[r] * tt;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int k = 0; k < (GGA ? 4 : 1); ++k) {
+        const double v = sum16(q[r][k]);
+        if (l16 == 0) {
+          if (wc == 0) s_q[k][pl + r] = v;
+          else s_h[k][pl + r] = v;
+        }
+      }
+  }
+  __syncthreads();
+  const int nq = GGA ? 4 : 1;
+  if (it.part < 0) {
+    // the leaf's only item: its points are the leaf's points
+    for (int u = t; u < nq * it.np; u += 256) {
+      const int k = u / it.np, p = u - k * it.np;
+      s_q[k][p] += s_h[k][p];
+    }
+    if (!GGA)
+      for (int p = t; p < it.np; p += 256) s_q[1][p] = s_q[2][p] = s_q[3][p] = 0.0;
+    __syncthreads();
+    density_epilogue(a, L, it.leaf, s_q, s_red);
+  } else {
+    double *dst = part + (long long)it.part * 4 * kTileAo;
+    for (int u = t; u < nq * it.np; u += 256) {
+      const int k = u / it.np, p = u - k * it.np;
+      dst[k * kTileAo + p] = s_q[k][p] + s_h[k][p];
+    }
+  }
+}
+
+// Leaves with more than one density item: their partials summed in
+// (point block, column block) order, then the functional epilogue.
+__global__ __launch_bounds__(256) void k_den_finish(BuildArgs a, const DenFinish *__restrict__ fin,
+                                                     const double *__restrict__ part) {
+  __shared__ double s_q[4][kMaxLeafPts];
+  __shared__ double s_red[4];
+  const DenFinish F = fin[blockIdx.x];
+  const LeafDev L = a.leaves[F.leaf];
+  const int nq = a.deriv ? 4 : 1;
+  for (int u = threadIdx.x; u < 4 * L.np; u += blockDim.x) {
+    const int k = u / L.np, p = u - k * L.np;
+    double v = 0.0;
+    if (k < nq) {
+      const int pb = p / kTileAo, pl = p - pb * kTileAo;
+      const double *src = part + (long long)(F.part0 + pb * F.ncb) * 4 * kTileAo + k * kTileAo + pl;
+      for (int cb = 0; cb < F.ncb; ++cb) v += src[(long long)cb * 4 * kTileAo];
+    }
+    s_q[k][p] = v;
+  }
+  __syncthreads();
+  density_epilogue(a, L, F.leaf, s_q, s_red);
+}
+
+// Vxc item, block pair (r, c) of a leaf: the leaf's points are streamed kTVp
+// at a time; each chunk stages Phi and A = c0 Phi + c . grad Phi
+// (xc_potential_matrix.cpp:19-41) of the block rows in LDS ([row][point]),
+// and wavefront (wr, wc) accumulates its quadrant of
+//   r == c:  K_rr = Phi_r^T A_r
+//   r <  c:  V_rc = Phi_r^T A_c + A_r^T Phi_c
+// in registers.  Entry (mu, nu) is added to K(map[nu], map[mu]) with FP64
+// atomics; Vxc = K + K^T afterwards counts every entry of V_loc once.
+template <typename S, bool GGA>
+__global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__restrict__ ao,
+                                                     const VxcItem *__restrict__ items) {
+  __shared__ double s_pr[kTileAo * kTVld], s_ar[kTileAo * kTVld];
+  __shared__ double s_pc[kTileAo * kTVld], s_ac[kTileAo * kTVld];
+  __shared__ double s_c[2][4][kTVp];
+  const VxcItem it = items[blockIdx.x];
+  if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
+  const LeafDev L = a.leaves[it.leaf];
+  const MolDev MD = a.mols[L.mol];
+  double *Km = a.K + MD.moff;
+  const int *map = a.ao_map + L.map_off;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wr = wave & 1, wcw = wave >> 1;
+  const bool diag = it.r0 == it.c0;
+  const int hr = it.wr >> 1, hc = it.wc >> 1;
+  const int RT = hr >> 4, CT = hc >> 4;
+  const S *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const int nrow = diag ? it.wr : it.wr + it.wc;
+  const double *cf = a.coef + L.pt_off;
+  constexpr int NQ = GGA ? 4 : 1;
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  if (t < NQ * kTVp) s_c[0][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + t % kTVp];
+  int buf = 0;
+  for (int p0 = 0; p0 < L.np; p0 += kTVp, buf ^= 1) {
+    __syncthreads();
+    // the next chunk's coefficients (read after the barrier below)
+    if (p0 + kTVp < L.np && t < NQ * kTVp)
+      s_c[buf ^ 1][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + p0 + kTVp + t % kTVp];
+    for (int gi = t; gi < nrow * (kTVp / 4); gi += 256) {
+      const int row = gi >> 2, p = 4 * (gi & 3);
+      const bool rb = row < it.wr;
+      const int mu = rb ? it.r0 + row : it.c0 + row - it.wr;
+      const long long o = (long long)mu * L.np + p0 + p;
+      const d4 f = load4_as<double>(phi + o);
+      d4 A;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) A[e] = s_c[buf][0][p + e] * f[e];
+      if constexpr (GGA) {
+        const d4 fx = load4_as<double>(phi + cs + o);
+        const d4 fy = load4_as<double>(phi + 2 * cs + o);
+        const d4 fz = load4_as<double>(phi + 3 * cs + o);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          A[e] += s_c[buf][1][p + e] * fx[e] + s_c[buf][2][p + e] * fy[e] + s_c[buf][3][p + e] * fz[e];
+      }
+      const int lr = rb ? row : row - it.wr;
+      *reinterpret_cast<d4u *>((rb ? s_pr : s_pc) + lr * kTVld + p) = f;
+      *reinterpret_cast<d4u *>((rb ? s_ar : s_ac) + lr * kTVld + p) = A;
+    }
+    __syncthreads();
+    // two k-steps (points) per operand load keeps the operands at 32
+    // registers next to the 128 of the accumulators
+    auto product = [&](const double *xa, const double *yb) {
+#pragma unroll
+      for (int sp = 0; sp < 4; sp += 2) {
+        d2u xv[4], yv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (i < RT) xv[i] = *reinterpret_cast<const d2u *>(xa + (wr * hr + 16 * i + l16) * kTVld + 4 * g + sp);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (j < CT) yv[j] = *reinterpret_cast<const d2u *>(yb + (wcw * hc + 16 * j + l16) * kTVld + 4 * g + sp);
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              if (i < RT && j < CT)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i][s2], yv[j][s2], acc[i][j], 0, 0, 0);
+      }
+    };
+    product(s_pr, diag ? s_ar : s_ac);
+    if (!diag) product(s_ar, s_pc);
+  }
+  // acc[i][j][r] = (mu = r0 + wr*hr + 16 i + g + 4 r, nu = c0 + wcw*hc + 16 j + l16)
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j >= CT) break;
+    const int nu = it.c0 + wcw * hc + 16 * j + l16;
+    if (nu >= L.m) continue;
+    double *kc = Km + map[nu];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= RT) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
+        if (mu < L.m) atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+      }
+    }
+  }
+}
+
 void check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -1392,6 +1674,80 @@ void check_launch(const char *what) {
 static size_t vxc_smem(size_t elem) {
   const size_t pad = 16 / elem;
   return sizeof(double) * 4 * kMaxLeafPts + elem * 64 * (128 + pad);
+}
+
+// Blocks of a leaf's padded AOs (or points): n32 units of 32 split into
+// ceil(n32 / 4) blocks of at most 4 units, the first ones one unit larger.
+static std::vector<std::pair<int, int>> tile_blocks(int n) {
+  std::vector<std::pair<int, int>> b;
+  const int units = n / 32, nb = (units + 3) / 4;
+  int at = 0;
+  for (int k = 0; k < nb; ++k) {
+    const int u = units / nb + (k < units % nb ? 1 : 0);
+    b.push_back({at, 32 * u});
+    at += 32 * u;
+  }
+  return b;
+}
+
+// Item lists of the tiled path.  The active leaves (Morton order) are cut
+// into 8 contiguous runs of about equal work (sum of np mp^2), one per XCD;
+// workgroup i runs on XCD i % 8, so entry 8 k + x of a list is the k-th item
+// of run x (padding entries have leaf = -1) and each XCD's L2 sees a compact
+// run of leaves.
+static void make_tile_plan(ContractPlan &p, const std::vector<LeafDev> &leaves) {
+  constexpr int kXcd = 8;
+  const int nl = (int)leaves.size();
+  double total = 0.0;
+  for (const auto &L : leaves) total += (double)L.np * L.mp * L.mp;
+  std::vector<std::vector<DenItem>> den(kXcd);
+  std::vector<std::vector<VxcItem>> vxc(kXcd);
+  std::vector<DenFinish> fin;
+  int nparts = 0;
+  double acc = 0.0;
+  for (int li = 0; li < nl; ++li) {
+    const LeafDev &L = leaves[li];
+    const int x = std::min(kXcd - 1, (int)(acc / std::max(total, 1.0) * kXcd));
+    acc += (double)L.np * L.mp * L.mp;
+    const auto cb = tile_blocks(L.mp);
+    std::vector<std::pair<int, int>> pb;
+    for (int p0 = 0; p0 < L.np; p0 += kTileAo) pb.push_back({p0, std::min(kTileAo, L.np - p0)});
+    const bool single = cb.size() == 1 && pb.size() == 1;
+    if (!single) fin.push_back(DenFinish{li, nparts, (int)cb.size(), (int)pb.size()});
+    for (size_t ip = 0; ip < pb.size(); ++ip)
+      for (size_t ic = 0; ic < cb.size(); ++ic)
+        den[x].push_back(DenItem{li, single ? -1 : nparts + (int)(ip * cb.size() + ic),
+                                 (short)pb[ip].first, (short)pb[ip].second,
+                                 (short)cb[ic].first, (short)cb[ic].second});
+    if (!single) nparts += (int)(cb.size() * pb.size());
+    for (size_t r = 0; r < cb.size(); ++r)
+      for (size_t c = r; c < cb.size(); ++c)
+        vxc[x].push_back(VxcItem{li, (short)cb[r].first, (short)cb[r].second,
+                                 (short)cb[c].first, (short)cb[c].second, 0});
+  }
+  auto interleave = [&](auto &lists, auto pad) {
+    size_t len = 0;
+    for (auto &l : lists) len = std::max(len, l.size());
+    std::vector<decltype(pad)> out(len * kXcd, pad);
+    for (int x = 0; x < kXcd; ++x)
+      for (size_t k = 0; k < lists[x].size(); ++k) out[k * kXcd + x] = lists[x][k];
+    return out;
+  };
+  const auto d = interleave(den, DenItem{-1, -1, 0, 0, 0, 0});
+  const auto v = interleave(vxc, VxcItem{-1, 0, 0, 0, 0, 0});
+  p.n_den_items = (int)d.size();
+  p.n_vxc_items = (int)v.size();
+  p.n_den_finish = (int)fin.size();
+  p.n_den_parts = nparts;
+  p.den_items.alloc(std::max<size_t>(d.size(), 1));
+  p.den_items.upload(d.data(), d.size());
+  p.vxc_items.alloc(std::max<size_t>(v.size(), 1));
+  p.vxc_items.upload(v.data(), v.size());
+  p.den_finish.alloc(std::max<size_t>(fin.size(), 1));
+  p.den_finish.upload(fin.data(), fin.size());
+  p.den_part.alloc((size_t)std::max(nparts, 1) * 4 * kTileAo);
+  OCCXC_HIP(hipStreamSynchronize(nullptr));  // the host lists are freed on return
+  p.bytes += p.den_items.bytes() + p.vxc_items.bytes() + p.den_finish.bytes() + p.den_part.bytes();
 }
 
 int vxc_variant_setting() {
@@ -1420,6 +1776,10 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   // or (2) as 0 with the K_loc blocks stored and reduced in a second pass
   p.vxc_variant = vxc_variant_setting();
   p.two_pass = p.vxc_variant == 2 && precision != 2;  // not with the split-BF16 Vxc
+  const char *tv = std::getenv("OCCXC_TILED");
+  p.tiled = p.use_mfma && (precision == 0 || precision == 3) && !p.two_pass &&
+            !(tv && std::atoi(tv) == 0);
+  if (p.tiled) make_tile_plan(p, leaves);
   if (precision == 1) {
     p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
     p.d2l.alloc((size_t)std::max<long long>(mtot, 1));
@@ -1484,6 +1844,23 @@ void run_density_store(const BuildArgs &a, const void *ao, int precision, const 
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s) {
   if (a.nleaves == 0) return;
+  if (plan.tiled) {
+    auto launch = [&](auto s_tag) {
+      using S = decltype(s_tag);
+      const S *x = static_cast<const S *>(ao);
+      (a.deriv ? k_den_tile<S, true> : k_den_tile<S, false>)
+          <<<plan.n_den_items, 256, 0, s>>>(a, x, plan.den_items.get(), plan.den_part.get());
+      check_launch("k_den_tile");
+      if (plan.n_den_finish > 0) {
+        k_den_finish<<<plan.n_den_finish, 256, 0, s>>>(a, plan.den_finish.get(),
+                                                       plan.den_part.get());
+        check_launch("k_den_finish");
+      }
+    };
+    if (precision == 0) launch(double{});
+    else launch(float{});
+    return;
+  }
   // variant 0: LDS-staged kernel for leaves of up to kBlkMaxMp AOs, the
   // per-wavefront kernel for the rest (each skips the other's leaves)
   const bool blk = plan.density_variant == 0;
@@ -1584,6 +1961,17 @@ void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s) {
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s) {
   if (a.nleaves == 0) return;
+  if (plan.tiled) {
+    auto launch = [&](auto s_tag) {
+      using S = decltype(s_tag);
+      (a.deriv ? k_vxc_tile<S, true> : k_vxc_tile<S, false>)
+          <<<plan.n_vxc_items, 256, 0, s>>>(a, static_cast<const S *>(ao), plan.vxc_items.get());
+      check_launch("k_vxc_tile");
+    };
+    if (precision == 0) launch(double{});
+    else launch(float{});
+    return;
+  }
   if (precision == 2) {
     k_vxc_bf16<<<a.nleaves, 256, vxc_bf16_smem(), s>>>(a, static_cast<const float *>(ao));
     check_launch("k_vxc_bf16");

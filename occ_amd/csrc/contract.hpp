@@ -22,6 +22,15 @@ struct ContractPlan {
   int vxc_variant = 0;     // 0 symmetric tile pairs (small leaves), 1 general kernel,
                            // 2 as 0 but two-pass (stores + tile-pair reduction)
   bool two_pass = false;
+  // tiled contractions (FP64 and FP32-cache contexts; OCCXC_TILED=0 turns
+  // them off): item lists interleaved over the XCDs, the multi-item leaves
+  // and their partial sums
+  bool tiled = false;
+  int n_den_items = 0, n_vxc_items = 0, n_den_finish = 0, n_den_parts = 0;
+  DevBuf<DenItem> den_items;
+  DevBuf<VxcItem> vxc_items;
+  DevBuf<DenFinish> den_finish;
+  DevBuf<double> den_part;  // [n_den_parts][4][kTileAo]
   DevBuf<float> d2f;       // FP32 rounding of the doubled density (FP32 mode)
   DevBuf<uint16_t> d2l;    // BF16 rounding of the remainder (FP32 mode)
   size_t bytes = 0;

@@ -69,6 +69,39 @@ struct TileEntry {
   unsigned mask_a, mask_b;
 };
 
+// Tiled contractions: a leaf's padded AOs are cut into blocks of at most
+// kTileAo (multiples of 32) and its padded points into blocks of at most
+// kTileAo.  A density item computes T = Phi D_loc for one (point block,
+// column block) pair, contracting over all of the leaf's AOs, and the partial
+// rho / grad rho of its columns; a Vxc item computes one block pair (r <= c)
+// of V_loc.  Items with leaf < 0 are padding (the lists are interleaved over
+// the 8 XCDs).
+constexpr int kTileAo = 128;
+
+struct DenItem {
+  int leaf;
+  int part;        // partial slot [4][kTileAo] (-1: the leaf's only item, functional inline)
+  short p0, np;    // point block
+  short c0, wc;    // column block (local AOs)
+};
+
+// A leaf with more than one density item: its partial slots are
+// part0 + pb * ncb + cb (point block pb, column block cb), summed in that
+// order before the functional.
+struct DenFinish {
+  int leaf;
+  int part0;
+  int ncb;
+  int npb;
+};
+
+struct VxcItem {
+  int leaf;
+  short r0, wr;    // row block (local AOs)
+  short c0, wc;    // column block (c0 >= r0)
+  int pad_;
+};
+
 // One molecule of a context: its nbf x nbf blocks (D2, K, Vxc) start at moff
 // in the concatenated per-context arrays; its active leaves are
 // [leaf_begin, leaf_end).
