@@ -1390,7 +1390,7 @@ __global__ __launch_bounds__(256) void k_vxc_reduce_final(BuildArgs a, double *_
 // wavefront works on a 2D block of up to 4 x 4 16x16 MFMA tiles (FP64
 // arithmetic; S is the AO storage type).
 
-constexpr int kTKc = 32;         // AOs per density k-chunk
+constexpr int kTKc = 16;         // AOs per density k-chunk
 constexpr int kTKld = kTKc + 4;  // LDS row stride (doubles) of the [row][k] density operands
 constexpr int kTVp = 16;         // points per Vxc chunk
 constexpr int kTVld = kTVp + 2;  // LDS row stride (doubles) of the [row][point] Vxc operands
@@ -1399,7 +1399,8 @@ constexpr int kTVld = kTVp + 2;  // LDS row stride (doubles) of the [row][point]
 // and columns nu, over all of the leaf's AOs mu in k-chunks staged through LDS
 // (Phi transposed to [point][mu], D2 gathered through the AO map to
 // [nu][mu]); wavefront (wp, wc) owns half the points and half the columns.
-// Then the column-block partials of rho = sum_nu Phi T and
+// The loads of chunk k+1 are issued before the MFMAs of chunk k.  Then the
+// column-block partials of rho = sum_nu Phi T and
 // grad rho = 2 sum_nu grad Phi T (density.cpp:32-43); the leaf's only item
 // runs the functional epilogue at once, the others store their partials.
 template <typename S, bool GGA>
@@ -1408,6 +1409,7 @@ __global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__res
                                                      double *__restrict__ part) {
   __shared__ double s_phi[kTileAo * kTKld];  // [point][k]; after the k loop s_q and s_h
   __shared__ double s_d[kTileAo * kTKld];    // [column][k]
+  __shared__ int s_map[kMaxLocalAo];
   __shared__ double s_red[4];
   static_assert(4 * kMaxLeafPts + 4 * kTileAo <= kTileAo * kTKld, "epilogue sums fit s_phi");
   double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(s_phi);
@@ -1427,45 +1429,76 @@ __global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__res
   const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
   const double *Dm = a.D2 + MD.moff;
-  const int *map = a.ao_map + L.map_off;
+  {
+    const int *map = a.ao_map + L.map_off;
+    for (int i = t; i < L.mp; i += 256) s_map[i] = map[i];
+  }
   d4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  const int pg = it.np >> 2;  // 4-point groups per AO row
+  // per chunk: kTKc x np Phi values in groups of 4 points, wc x kTKc D values
+  // in groups of 4 AOs; at most 512 groups of each, two per thread
+  const int nphi = kTKc * (it.np >> 2), nd = it.wc * (kTKc / 4);
+  d4 rp[2], rd[2];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int gi = t + 256 * u;
+      if (gi < nphi) {
+        const int mu = gi & (kTKc - 1), p = 4 * (gi / kTKc);
+        rp[u] = load4_as<double>(phi + (long long)(k0 + mu) * L.np + it.p0 + p);
+      }
+      if (gi < nd) {
+        const int col = gi >> 2, kk = 4 * (gi & 3);
+        const double *dc = Dm + (long long)s_map[it.c0 + col] * MD.nbf;
+        rd[u] = d4{dc[s_map[k0 + kk]], dc[s_map[k0 + kk + 1]], dc[s_map[k0 + kk + 2]],
+                   dc[s_map[k0 + kk + 3]]};
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int gi = t + 256 * u;
+      if (gi < nphi) {
+        // AO fastest over the lanes: the transposed writes of 16 lanes fall in
+        // consecutive LDS words
+        const int mu = gi & (kTKc - 1), p = 4 * (gi / kTKc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s_phi[(p + e) * kTKld + mu] = rp[u][e];
+      }
+      if (gi < nd) {
+        const int col = gi >> 2, kk = 4 * (gi & 3);
+        *reinterpret_cast<d4 *>(s_d + col * kTKld + kk) = rd[u];
+      }
+    }
+  };
+  __syncthreads();  // s_map
+  fetch(0);
   for (int k0 = 0; k0 < L.mp; k0 += kTKc) {
+    __syncthreads();  // the previous chunk's operands are consumed
+    store();
     __syncthreads();
-    for (int gi = t; gi < kTKc * pg; gi += 256) {
-      const int mu = gi / pg, p = 4 * (gi - mu * pg);
-      const d4 v = load4_as<double>(phi + (long long)(k0 + mu) * L.np + it.p0 + p);
+    if (k0 + kTKc < L.mp) fetch(k0 + kTKc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) s_phi[(p + e) * kTKld + mu] = v[e];
-    }
-    for (int gi = t; gi < it.wc * (kTKc / 4); gi += 256) {
-      const int col = gi / (kTKc / 4), kk = 4 * (gi - col * (kTKc / 4));
-      const double *dc = Dm + (long long)map[it.c0 + col] * MD.nbf;
-      const int4 mk = *reinterpret_cast<const int4 *>(map + k0 + kk);
-      *reinterpret_cast<d4 *>(s_d + col * kTKld + kk) = d4{dc[mk.x], dc[mk.y], dc[mk.z], dc[mk.w]};
-    }
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < kTKc; ks += 16) {
-      d4 av[4], bv[4];
+    for (int sp = 0; sp < kTKc / 4; sp += 2) {
+      d2u av[4], bv[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (i < RT) av[i] = *reinterpret_cast<const d4 *>(s_phi + (wp * hp + 16 * i + pa) * kTKld + ks + 4 * g);
+        if (i < RT) av[i] = *reinterpret_cast<const d2u *>(s_phi + (wp * hp + 16 * i + pa) * kTKld + 4 * g + sp);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (j < CT) bv[j] = *reinterpret_cast<const d4 *>(s_d + (wc * hc + 16 * j + l16) * kTKld + ks + 4 * g);
+        if (j < CT) bv[j] = *reinterpret_cast<const d2u *>(s_d + (wc * hc + 16 * j + l16) * kTKld + 4 * g + sp);
 #pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4)
+      for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
           for (int j = 0; j < 4; ++j)
             if (i < RT && j < CT)
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][s4], bv[j][s4], acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][s2], bv[j][s2], acc[i][j], 0, 0, 0);
     }
   }
   __syncthreads();  // s_phi is reused for the sums below
@@ -1480,22 +1513,34 @@ __global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__res
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[r][k] = 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (j >= CT) break;
-      const long long off = (long long)(it.c0 + wc * hc + 16 * j + l16) * L.np + it.p0 + pl;
-      const d4 f = load4_as<double>(phi + off);
+    for (int jj = 0; jj < 4; jj += 2) {
+      // both column tiles' loads in flight before either is used
+      d4 f[2], fx[2], fy[2], fz[2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) q[r][0] += f[r] * acc[i][j][r];
-      if constexpr (GGA) {
-        const d4 fx = load4_as<double>(phi + cs + off);
-        const d4 fy = load4_as<double>(phi + 2 * cs + off);
-        const d4 fz = load4_as<double>(phi + 3 * cs + off);
+      for (int jq = 0; jq < 2; ++jq) {
+        if (jj + jq >= CT) break;
+        const long long off = (long long)(it.c0 + wc * hc + 16 * (jj + jq) + l16) * L.np + it.p0 + pl;
+        f[jq] = load4_as<double>(phi + off);
+        if constexpr (GGA) {
+          fx[jq] = load4_as<double>(phi + cs + off);
+          fy[jq] = load4_as<double>(phi + 2 * cs + off);
+          fz[jq] = load4_as<double>(phi + 3 * cs + off);
+        }
+      }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const double tt = 2.0 * acc[i][j][r];
-          q[r][1] += fx[r] * tt;
-          q[r][2] += fy[r] * tt;
-          q[r][3] += fz[r] * tt;
+      for (int jq = 0; jq < 2; ++jq) {
+        const int j = jj + jq;
+        if (j >= CT) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r][0] += f[jq][r] * acc[i][j][r];
+        if constexpr (GGA) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const double tt = 2.0 * acc[i][j][r];
+            q[r][1] += fx[jq][r] * tt;
+            q[r][2] += fy[jq][r] * tt;
+            q[r][3] += fz[jq][r] * tt;
+          }
         }
       }
     }
@@ -1663,6 +1708,192 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
   }
 }
 
+// Persistent, wavefront-specialised form of the Vxc items: one 512-thread
+// workgroup per CU walks the items blockIdx.x, blockIdx.x + gridDim.x, ...
+// (the same XCD for all of them, gridDim.x being a multiple of 8) chunk by
+// chunk.  Wavefronts 4-7 stage the next chunk (Phi and A of the item's block
+// rows) into one of two LDS buffers while wavefronts 0-3 run the MFMAs of the
+// current chunk from the other.  After an item's last chunk the MFMA
+// wavefronts issue its scatter atomics and go on with the next item while
+// they drain.  Skipped leaves and padding entries take no steps.
+struct VxcCursor {
+  int ii, p0;
+  VxcItem it;
+  LeafDev L;
+};
+
+__device__ __forceinline__ void vxc_seek(const BuildArgs &a, const VxcItem *items, int n,
+                                         VxcCursor &c) {
+  while (c.ii < n) {
+    c.it = items[c.ii];
+    if (c.it.leaf >= 0 && a.leaf_active[c.it.leaf]) {
+      c.L = a.leaves[c.it.leaf];
+      return;
+    }
+    c.ii += gridDim.x;
+  }
+}
+
+__device__ __forceinline__ void vxc_next(const BuildArgs &a, const VxcItem *items, int n,
+                                         VxcCursor &c) {
+  c.p0 += kTVp;
+  if (c.p0 >= c.L.np) {
+    c.p0 = 0;
+    c.ii += gridDim.x;
+    vxc_seek(a, items, n, c);
+  }
+}
+
+constexpr int kWsBuf = 4 * kTileAo * kTVld;  // doubles per staging buffer
+static size_t vxc_ws_smem() { return sizeof(double) * 2 * kWsBuf; }
+
+template <typename S, bool GGA>
+__global__ __launch_bounds__(512, 1) void k_vxc_ws(BuildArgs a, const S *__restrict__ ao,
+                                                   const VxcItem *__restrict__ items, int n) {
+  extern __shared__ double smem[];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const bool producer = wave >= 4;
+  VxcCursor cc;
+  cc.ii = blockIdx.x;
+  cc.p0 = 0;
+  vxc_seek(a, items, n, cc);
+  if (cc.ii >= n) return;
+  // producers: Phi and A = c0 Phi + c . grad Phi of a step's block rows.  A
+  // producer thread takes the groups pt, pt + 256, ... (at most 4: 2 x 128
+  // rows of 4 groups), which share one 4-point group, so its coefficients are
+  // loaded once, and all its AO loads are issued before any is used.
+  auto stage = [&](const VxcCursor &c, double *B) {
+    const int pt = t - 256;
+    const bool diag = c.it.r0 == c.it.c0;
+    const int ng = (diag ? c.it.wr : c.it.wr + c.it.wc) * (kTVp / 4);
+    const S *phi = ao + c.L.ao_off;
+    const long long cs = (long long)c.L.mp * c.L.np;
+    const int p = 4 * (pt & 3);
+    const double *cf = a.coef + c.L.pt_off + c.p0 + p;
+    d4 cq[4];
+#pragma unroll
+    for (int k = 0; k < (GGA ? 4 : 1); ++k) cq[k] = *reinterpret_cast<const d4 *>(cf + k * a.ptot);
+    d4 f[4][GGA ? 4 : 1];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int gi = pt + 256 * u;
+      if (gi < ng) {
+        const int row = gi >> 2;
+        const int mu = row < c.it.wr ? c.it.r0 + row : c.it.c0 + row - c.it.wr;
+        const S *src = phi + (long long)mu * c.L.np + c.p0 + p;
+#pragma unroll
+        for (int k = 0; k < (GGA ? 4 : 1); ++k) f[u][k] = load4_as<double>(src + k * cs);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int gi = pt + 256 * u;
+      if (gi < ng) {
+        const int row = gi >> 2;
+        const bool rb = row < c.it.wr;
+        d4 A = cq[0] * f[u][0];
+        if constexpr (GGA) A += cq[1] * f[u][1] + cq[2] * f[u][2] + cq[3] * f[u][3];
+        const int lr = rb ? row : row - c.it.wr;
+        double *P = B + (rb ? 0 : 2 * kTileAo * kTVld) + lr * kTVld + p;
+        *reinterpret_cast<d4u *>(P) = f[u][0];
+        *reinterpret_cast<d4u *>(P + kTileAo * kTVld) = A;
+      }
+    }
+  };
+  // the two roles run separate loops (so the accumulators and the staging
+  // registers are never live together) with the same barrier sequence: one
+  // barrier per step, the producers one step ahead
+  if (producer) {
+    stage(cc, smem);
+    __syncthreads();
+    int b = 0;
+    while (true) {
+      VxcCursor nx = cc;
+      vxc_next(a, items, n, nx);
+      if (nx.ii < n) stage(nx, smem + (b ^ 1) * kWsBuf);
+      __syncthreads();
+      if (nx.ii >= n) break;
+      cc = nx;
+      b ^= 1;
+    }
+    return;
+  }
+  __syncthreads();
+  const int wr = wave & 1, wcw = (wave >> 1) & 1;
+  d4 acc[4][4];
+  int b = 0;
+  while (true) {
+    VxcCursor nx = cc;
+    vxc_next(a, items, n, nx);
+    {
+      const VxcItem &it = cc.it;
+      const bool diag = it.r0 == it.c0;
+      const int hr = it.wr >> 1, hc = it.wc >> 1;
+      const int RT = hr >> 4, CT = hc >> 4;
+      if (cc.p0 == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      }
+      const double *B = smem + b * kWsBuf;
+      const double *s_pr = B, *s_ar = B + kTileAo * kTVld;
+      const double *s_pc = B + 2 * kTileAo * kTVld, *s_ac = B + 3 * kTileAo * kTVld;
+      auto product = [&](const double *xa, const double *yb) {
+#pragma unroll
+        for (int sp = 0; sp < 4; sp += 2) {
+          d2u xv[4], yv[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (i < RT) xv[i] = *reinterpret_cast<const d2u *>(xa + (wr * hr + 16 * i + l16) * kTVld + 4 * g + sp);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (j < CT) yv[j] = *reinterpret_cast<const d2u *>(yb + (wcw * hc + 16 * j + l16) * kTVld + 4 * g + sp);
+#pragma unroll
+          for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int j = 0; j < 4; ++j)
+                if (i < RT && j < CT)
+                  acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i][s2], yv[j][s2], acc[i][j], 0, 0, 0);
+        }
+      };
+      product(s_pr, diag ? s_ar : s_ac);
+      if (!diag) product(s_ar, s_pc);
+      if (nx.ii != cc.ii) {
+        // the item's last chunk: scatter (mu = r0 + wr*hr + 16 i + g + 4 r,
+        // nu = c0 + wcw*hc + 16 j + l16) into K(map[nu], map[mu])
+        const LeafDev &L = cc.L;
+        const MolDev MD = a.mols[L.mol];
+        double *Km = a.K + MD.moff;
+        const int *map = a.ao_map + L.map_off;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (j >= CT) break;
+          const int nu = it.c0 + wcw * hc + 16 * j + l16;
+          if (nu >= L.m) continue;
+          double *kc = Km + map[nu];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (i >= RT) break;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
+              if (mu < L.m) atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (nx.ii >= n) break;
+    cc = nx;
+    b ^= 1;
+  }
+}
+
 void check_launch(const char *what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -1779,7 +2010,24 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   const char *tv = std::getenv("OCCXC_TILED");
   p.tiled = p.use_mfma && (precision == 0 || precision == 3) && !p.two_pass &&
             !(tv && std::atoi(tv) == 0);
-  if (p.tiled) make_tile_plan(p, leaves);
+  if (p.tiled) {
+    make_tile_plan(p, leaves);
+    int dev = 0, cus = 0;
+    OCCXC_HIP(hipGetDevice(&dev));
+    OCCXC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    p.persistent_wgs = std::max(8, cus / 8 * 8);
+    // the persistent form is slower than one workgroup per item on the
+    // single-molecule clusters (staging latency with one workgroup per CU);
+    // OCCXC_VXC_WS=1 selects it
+    const char *ws = std::getenv("OCCXC_VXC_WS");
+    p.vxc_ws = ws && std::atoi(ws) != 0;
+    for (const void *k : {reinterpret_cast<const void *>(&k_vxc_ws<double, true>),
+                          reinterpret_cast<const void *>(&k_vxc_ws<double, false>),
+                          reinterpret_cast<const void *>(&k_vxc_ws<float, true>),
+                          reinterpret_cast<const void *>(&k_vxc_ws<float, false>)})
+      OCCXC_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)vxc_ws_smem()));
+  }
   if (precision == 1) {
     p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
     p.d2l.alloc((size_t)std::max<long long>(mtot, 1));
@@ -1964,6 +2212,13 @@ void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int p
   if (plan.tiled) {
     auto launch = [&](auto s_tag) {
       using S = decltype(s_tag);
+      if (plan.vxc_ws) {
+        (a.deriv ? k_vxc_ws<S, true> : k_vxc_ws<S, false>)
+            <<<plan.persistent_wgs, 512, vxc_ws_smem(), s>>>(a, static_cast<const S *>(ao),
+                                                             plan.vxc_items.get(), plan.n_vxc_items);
+        check_launch("k_vxc_ws");
+        return;
+      }
       (a.deriv ? k_vxc_tile<S, true> : k_vxc_tile<S, false>)
           <<<plan.n_vxc_items, 256, 0, s>>>(a, static_cast<const S *>(ao), plan.vxc_items.get());
       check_launch("k_vxc_tile");

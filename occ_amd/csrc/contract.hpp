@@ -26,6 +26,8 @@ struct ContractPlan {
   // them off): item lists interleaved over the XCDs, the multi-item leaves
   // and their partial sums
   bool tiled = false;
+  bool vxc_ws = false;     // persistent wavefront-specialised Vxc (OCCXC_VXC_WS=1) instead of one item per workgroup
+  int persistent_wgs = 256;  // workgroups of the persistent kernels (a multiple of 8)
   int n_den_items = 0, n_vxc_items = 0, n_den_finish = 0, n_den_parts = 0;
   DevBuf<DenItem> den_items;
   DevBuf<VxcItem> vxc_items;
