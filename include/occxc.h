@@ -111,6 +111,8 @@ typedef struct {
   double ao_bytes;           /* bytes of the resident AO values */
   double ao_write_bytes;     /* algorithmic bytes of one AO evaluation */
   double device_bytes;       /* total device footprint of the context */
+  int max_local_nbf;         /* largest m_b over the active leaves */
+  int pad_;
 } occxc_context_info;
 
 /* --- the XC build ---------------------------------------------------- */

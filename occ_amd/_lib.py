@@ -50,7 +50,8 @@ class CContextInfo(ctypes.Structure):
                 ("nbf", ctypes.c_int), ("nshell", ctypes.c_int),
                 ("mean_local_nbf", ctypes.c_double), ("contraction_flops", ctypes.c_double),
                 ("ao_bytes", ctypes.c_double), ("ao_write_bytes", ctypes.c_double),
-                ("device_bytes", ctypes.c_double)]
+                ("device_bytes", ctypes.c_double), ("max_local_nbf", ctypes.c_int),
+                ("pad_", ctypes.c_int)]
 
 
 class CServerOptions(ctypes.Structure):

@@ -1321,6 +1321,9 @@ occxc_status occxc_context_get_info(const occxc_context *ctx, occxc_context_info
     info->ao_bytes = (double)ctx->ao_elems * ctx->ao_elem_size;
     info->ao_write_bytes = ctx->ao_write_bytes;
     info->device_bytes = (double)ctx->device_bytes;
+    info->max_local_nbf = 0;
+    for (const auto &L : ctx->leaves) info->max_local_nbf = std::max(info->max_local_nbf, L.m);
+    info->pad_ = 0;
   });
 }
 

@@ -248,10 +248,16 @@ def test_density_variants_agree(monkeypatch):
     basis = AOBasis.load(atoms, "def2-svp", pure=True)
     pts, wts = _grid(atoms)
     D = random_density(basis.nbf, 10, 51)
-    variants = ("0", "1", "2", "4")
+    # the per-leaf kernels (OCCXC_TILED=0) in each density variant, then the
+    # tiled kernels (the default)
+    variants = ("0", "1", "2", "4", "tiled")
     out = []
     for v in variants:
-        monkeypatch.setenv("OCCXC_DENSITY_VARIANT", v)
+        if v == "tiled":
+            monkeypatch.delenv("OCCXC_TILED", raising=False)
+        else:
+            monkeypatch.setenv("OCCXC_TILED", "0")
+            monkeypatch.setenv("OCCXC_DENSITY_VARIANT", v)
         dft = DFT("pbe", basis, pts, wts)
         out.append((dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density()))
     for k in range(1, len(variants)):
@@ -269,6 +275,7 @@ def test_density_kernels_split_by_leaf_width(monkeypatch):
     pts, wts = _grid(atoms, 11, 26)
     D = random_density(basis.nbf, 40, 53)
     out = []
+    monkeypatch.setenv("OCCXC_TILED", "0")
     for v in ("2", "1"):
         monkeypatch.setenv("OCCXC_DENSITY_VARIANT", v)
         dft = DFT("pbe", basis, pts, wts)
@@ -372,16 +379,20 @@ def test_vxc_variants_agree(monkeypatch, precision):
     pts, wts = _grid(atoms)
     D = random_density(basis.nbf, 12, 71)
     out = []
-    for v in ("0", "1", "2"):
-        monkeypatch.setenv("OCCXC_VXC_VARIANT", v)
+    for v in ("0", "1", "2", "tiled", "ws"):
+        # per-leaf kernels (OCCXC_TILED=0) in each Vxc variant, then the tiled
+        # kernels one item per workgroup and in the persistent form
+        monkeypatch.setenv("OCCXC_TILED", "1" if v in ("tiled", "ws") else "0")
+        monkeypatch.setenv("OCCXC_VXC_WS", "1" if v == "ws" else "0")
+        monkeypatch.setenv("OCCXC_VXC_VARIANT", v if v in ("0", "1", "2") else "0")
         dft = DFT("pbe", basis, pts, wts, precision=precision)
         out.append((dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density()))
         if v == "2":
             # the two-pass form sums in a fixed order: builds repeat bit for bit
             assert np.array_equal(dft.compute_vxc(D), out[-1][0])
     tol = 1e-12 if precision == "fp64" else 2e-6
-    assert np.abs(out[0][0] - out[1][0]).max() <= tol
-    assert np.abs(out[0][0] - out[2][0]).max() <= tol
+    for k in range(1, len(out)):
+        assert np.abs(out[0][0] - out[k][0]).max() <= tol
     assert np.abs(out[2][0] - out[2][0].T).max() == 0.0
     assert np.abs(out[0][0] - out[0][0].T).max() == 0.0
     if precision == "fp64":

@@ -1,10 +1,13 @@
 """GPU: the native serving loop (occxc_server_*) -- continuous batching of
 builds from several SCFs, admission by device-memory footprint, and results
 that are the same alone and in the mix (Vxc to 1e-12: the atomics that sum
-Vxc have no fixed order; Exc and N_elec to 1e-12)."""
+Vxc have no fixed order; Exc and N_elec to 1e-12).  Every served build is
+also checked against the CPU oracle (|dExc| <= 1e-6, max|dVxc| <= 1e-7,
+|dN| <= 1e-8)."""
 import numpy as np
 import pytest
 
+import oracle
 from conftest import random_density
 from occ_amd import AOBasis, molecules
 from occ_amd.dft import DFT, XCContext
@@ -30,6 +33,16 @@ def _systems():
     return out
 
 
+def _oracle(basis, pts, w, D):
+    return oracle.xc_build(oracle.OracleBasis.from_flat(basis.flat()), pts, w, FUNCS, D)
+
+
+def _matches_oracle(out, ref):
+    V, e, n = out[:3]
+    K, exc, nel = ref
+    return np.abs(V - K).max() <= 1e-7 and abs(e - exc) <= 1e-6 and abs(n - nel) <= 1e-8
+
+
 def _close(a, b):
     Va, ea, na = a
     Vb, eb, nb = b
@@ -45,6 +58,7 @@ def test_server_results_same_alone_and_in_mix():
             for k, D in enumerate(Ds):
                 V, e, n, _ = srv.wait(srv.submit(scf, D))
                 alone[name, k] = (V, e, n)
+                assert _matches_oracle(alone[name, k], _oracle(basis, pts, w, D)), (name, k)
         # the serving path gives the plain context's result
         dft = DFT("pbe", basis, pts, w)
         V = dft.compute_vxc(Ds[0])
@@ -81,6 +95,7 @@ def test_server_admits_by_memory_budget():
         for k in range(2):
             V = dft.compute_vxc(Ds[k])
             ref[name, k] = (V, dft.exchange_correlation_energy(), dft.total_density())
+    sysmap = {name: (basis, pts, w, Ds) for name, basis, pts, w, Ds in systems}
     with XCServer(slots=2, memory_budget=budget) as srv:
         scfs = {name: srv.open(basis, pts, w, FUNCS) for name, basis, pts, w, _ in systems}
         for k in range(2):
@@ -88,6 +103,8 @@ def test_server_admits_by_memory_budget():
             for name, t in tickets:
                 out = srv.wait(t)
                 assert _close(ref[name, k], out[:3]), (name, k)
+                basis, pts, w, Ds = sysmap[name]
+                assert _matches_oracle(out, _oracle(basis, pts, w, Ds[k])), (name, k)
             st = srv.stats()
             assert st["resident_bytes"] <= budget
         st = srv.stats()
