@@ -1,11 +1,14 @@
 """XC-build benchmark on one MI355X.
 
-Workload (default): PBE XC build for a seeded (H2O)_32 cluster, def2-SVP
-(spherical, 768 AO), 75 radial (Euler-Maclaurin) x 302 Lebedev points per
-atom = 2,174,400 grid points, D = C C^T with C a seeded Gaussian 768 x 160
-matrix orthonormalised by QR.  A step is one XC build (D in HBM -> Vxc in
-HBM, Exc, N_elec) on a context whose grid, leaves and AO values are resident,
-as in an SCF iteration.
+Workload (default): B88+LYP XC build for a seeded (H2O)_128 cluster,
+def2-SVP (spherical, 3072 AO), 75 radial (Euler-Maclaurin) x 302 Lebedev
+points per atom = 8,697,600 grid points, D = C C^T with C a seeded Gaussian
+3072 x 640 matrix orthonormalised by QR -- the largest single-GPU
+configuration of the benchmark set.  A step is one XC build (D in HBM -> Vxc
+in HBM, Exc, N_elec) on a context whose grid, leaves and AO values are
+resident, as in an SCF iteration.  The default run also reports, under
+"extras", the (H2O)_32 / PBE build rate, one serving point (p50/p99 latency)
+and the batched 512-molecule rate against the plain FP64 path.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -78,7 +81,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="h2o32")
+    ap.add_argument("--workload", default="h2o128")
     ap.add_argument("--modes", default="fp64,fp32cache,fp32,bf16x3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle (no accuracy, "
                     "no cpu_baseline)")
@@ -91,7 +94,12 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=12)
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="molecules built one context at a time for the FP64 reference rate")
-    ap.add_argument("--cpu-single", type=int, default=1, help="also time one 1-thread build")
+    ap.add_argument("--cpu-single", type=int, default=1,
+                    help="also time one 1-thread build (skipped when it would take over 60 s)")
+    ap.add_argument("--extras", type=int, default=-1,
+                    help="add the (H2O)32 rate, a serving point and the batched rate "
+                         "(default: on for the default workload)")
+    ap.add_argument("--serve-extra-lambda", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device (a known-size kernel against "
                          "which profiler byte counters are scaled)")
@@ -116,7 +124,8 @@ def main():
     from occ_amd.grid import GridSettings, MolecularGrid
 
     if args.workload == "batch512":
-        return run_batch(args)
+        print(json.dumps(run_batch(args)), flush=True)
+        return
     if args.workload in ("grad32", "tpssgrad32", "ugrad32", "utpssgrad32"):
         if args.workload in ("tpssgrad32", "utpssgrad32", "ugrad32"):
             args.modes = ",".join(m for m in args.modes.split(",") if m in ("fp64", "fp32cache"))
@@ -124,7 +133,8 @@ def main():
     if args.workload in ("uks32", "utpss32"):
         return run_unrestricted(args)
     if args.workload == "serve":
-        return run_serving(args)
+        print(json.dumps(run_serving(args)), flush=True)
+        return
     if args.workload == "vv10_32":
         return run_vv10(args)
     # meta-GGA: FP64 and FP32-cache contexts only; each stage is four GEMMs
@@ -159,7 +169,7 @@ def main():
         dt = float(np.median(times))
         ref = (K, exc, nel)
         t1 = None
-        if args.cpu_single:
+        if args.cpu_single and dt * threads <= 60.0:
             t0 = time.time()
             oracle.xc_build(ob, pts, wts, ids, D, nthreads=1)
             t1 = time.time() - t0
@@ -266,7 +276,66 @@ def main():
         "modes_ms": {m: results[m]["ms_per_step"] for m in results},
         "cpu_baseline": cpu,
     }
+    if args.extras == 1 or (args.extras == -1 and args.workload == "h2o128"):
+        out["extras"] = run_extras(args)
     print(json.dumps(out), flush=True)
+
+
+def run_rate(name, mode, steps, warmup):
+    """Warm FP64 build rate of a single-molecule workload (no CPU oracle)."""
+    from occ_amd import AOBasis
+    from occ_amd.device import DeviceArray, synchronize
+    from occ_amd.dft import XCContext, functional_ids
+    from occ_amd.grid import GridSettings, MolecularGrid
+    atoms, bname, pure, method, nocc, desc = workload(name)
+    basis = AOBasis.load(atoms, bname, pure=pure)
+    nbf = basis.nbf
+    pts, wts = MolecularGrid(atoms, GridSettings(radial_points=76,
+                                                 angular_points=302)).build(device=0)
+    D = seeded_density(nbf, nocc, 0)
+    d_D, d_V, d_r = DeviceArray(nbf * nbf), DeviceArray(nbf * nbf), DeviceArray(2)
+    d_D.upload(D)
+    ctx = XCContext(basis, pts, wts, functional_ids(method), precision=mode)
+    for _ in range(warmup):
+        ctx.build_device(d_D.ptr, d_V.ptr, d_r.ptr)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.build_device(d_D.ptr, d_V.ptr, d_r.ptr)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    ctx.close()
+    for a in (d_D, d_V, d_r):
+        a.free()
+    log(f"{name} {mode}: {ms:.3f} ms/build")
+    return {"workload": desc, "precision": mode, "npts": len(wts), "nbf": nbf,
+            "ms_per_build": ms, "builds_per_s": 1e3 / ms, "grid_pts_per_s": len(wts) * 1e3 / ms}
+
+
+def run_extras(args):
+    """The other benchmark configurations, one line each: the (H2O)32 / PBE
+    build rate, one serving point and the batched launch."""
+    import copy
+    ex = {}
+    ex["h2o32_pbe"] = run_rate("h2o32", "fp64", args.steps, args.warmup)
+    a = copy.copy(args)
+    a.serve_lambdas = f"{args.serve_extra_lambda:g}"
+    srv = run_serving(a)
+    row = srv["sweep"][0]
+    ex["serving"] = {"workload": srv["config"]["workload"], "lambda_req_per_s": row["lambda_req_per_s"],
+                     "p50_ms": row["p50_ms"], "p99_ms": row["p99_ms"],
+                     "throughput_builds_per_s": row["throughput_builds_per_s"],
+                     "builds": row["builds"], "errors": row["errors"]}
+    a = copy.copy(args)
+    a.modes = "fp64,fp32"
+    b = run_batch(a)
+    ex["batch512"] = {"workload": b["config"]["workload"], "precision": b["config"]["precision"],
+                      "molecules_per_s": b["value"],
+                      "cold_molecules_per_s": b["cold_molecules_per_s"],
+                      "plain_fp64_molecules_per_s": b["plain_fp64_molecules_per_s"],
+                      "speedup_vs_plain_fp64": b["speedup_vs_plain_fp64"],
+                      "modes": b["modes"]}
+    return ex
 
 
 def run_gradient(args):
@@ -698,7 +767,7 @@ def run_batch(args):
                       "max_dexc_sample": worst[m]} for m in results},
         "grid_build_s": t_grid,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def run_serving(args):
@@ -791,7 +860,7 @@ def run_serving(args):
                    "pool": [p[0] for p in pool]},
         "sweep": sweep,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":
