@@ -440,6 +440,25 @@ occxc_status occxc_becke_partition(int natoms, const double *atom_xyz,
                                    const int *atom_of_point, double *weights,
                                    int device);
 
+/* Allocation-free form of occxc_becke_partition, with no counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Every
+ * array is a caller-owned DEVICE buffer on the current device: atom_xyz
+ * [3 natoms], radii [natoms], points [3 npts], atom_of_point [npts] and
+ * weights [npts] (scaled in place); work is queued on stream (null: the
+ * default stream) and the call returns without waiting.  The inter-atomic
+ * distance table lives in a fixed device array of 1024 x 1024 doubles
+ * (8 MiB per GPU, held until the process ends): natoms above 1024 is
+ * rejected with OCCXC_ERR_INVALID_ARGUMENT.  Inputs are not inspected on
+ * the host (coincident atoms, which occxc_becke_partition rejects, give
+ * non-finite weights; an atom index out of range gives weight 0).  All calls
+ * on one GPU share the table, so calls that overlap on one GPU race. */
+occxc_status occxc_becke_partition_noalloc(int natoms, const double *atom_xyz,
+                                           const double *radii, int ta_adjustment,
+                                           long long npts, const double *points,
+                                           const int *atom_of_point, double *weights,
+                                           void *stream);
+
 /* --- errors ------------------------------------------------------------ */
 
 /* Message of the last failing call on this thread ("" if none). */

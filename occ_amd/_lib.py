@@ -133,6 +133,11 @@ SIGNATURES = {
                                          _ip]),
     "occxc_becke_partition": (ctypes.c_int, [ctypes.c_int, _dp, _dp, ctypes.c_int,
                                              ctypes.c_longlong, _dp, _ip, _dp, ctypes.c_int]),
+    "occxc_becke_partition_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p,
+                                                     ctypes.c_void_p, ctypes.c_int,
+                                                     ctypes.c_longlong, ctypes.c_void_p,
+                                                     ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p]),
     "occxc_server_default_options": (None, [ctypes.POINTER(CServerOptions)]),
     "occxc_server_create": (ctypes.c_int, [ctypes.POINTER(CServerOptions),
                                            ctypes.POINTER(ctypes.c_void_p)]),

@@ -2056,6 +2056,20 @@ occxc_status occxc_becke_partition(int natoms, const double *atom_xyz, const dou
   });
 }
 
+occxc_status occxc_becke_partition_noalloc(int natoms, const double *atom_xyz,
+                                           const double *radii, int ta_adjustment,
+                                           long long npts, const double *points,
+                                           const int *atom_of_point, double *weights,
+                                           void *stream) {
+  return guarded([&] {
+    OCCXC_REQUIRE(atom_xyz && radii && npts >= 0 &&
+                      (npts == 0 || (points && atom_of_point && weights)),
+                  OCCXC_ERR_INVALID_ARGUMENT, "becke_partition_noalloc: bad arguments");
+    becke_partition_noalloc(natoms, atom_xyz, radii, ta_adjustment, npts, points, atom_of_point,
+                            weights, static_cast<hipStream_t>(stream));
+  });
+}
+
 const char *occxc_last_error_message(void) { return g_last_error.c_str(); }
 
 const char *occxc_version(void) { return "occxc 0.1 gfx950"; }

@@ -21,9 +21,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
+#include "common.hpp"
 #include "kernels.hpp"
 #include "xc_functionals.hpp"
 #include "xc_functionals_pol.hpp"
@@ -1156,6 +1158,68 @@ void launch_becke(int natoms, const double *atom_xyz, const double *radii, const
   k_becke<<<grid_for(npts, 128), 128, 0, s>>>(natoms, atom_xyz, radii, dist, ta, npts, pts,
                                               atom_of_point, w);
   check_launch();
+}
+
+// ---- allocation-free Becke partition (occxc_becke_partition_noalloc) ----
+// The inter-atomic distance table (natoms x natoms FP64, at most
+// kNoAllocMaxAtoms atoms: 8 MiB) lives in this file-scope device array.
+// Every GPU the library runs on holds its own copy until the process ends;
+// host code never passes the array's name to a kernel, only the address
+// hipGetSymbolAddress returned for the current device.  All calls on one GPU
+// share the table: calls that overlap on one GPU race.
+static __device__ __align__(256) double scratch[(size_t)kNoAllocMaxAtoms * kNoAllocMaxAtoms];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// the distances in the host restatement's operation order (no contraction),
+// so that both entry points see the same table
+__global__ void k_atom_distances(int natoms, const double *__restrict__ axyz,
+                                 double *__restrict__ dist) {
+#pragma clang fp contract(off)
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)natoms * natoms) return;
+  const int i = (int)(idx / natoms), j = (int)(idx - (long long)i * natoms);
+  double d = 0.0;
+  if (i != j) {
+    const double dx = axyz[3 * i] - axyz[3 * j], dy = axyz[3 * i + 1] - axyz[3 * j + 1],
+                 dz = axyz[3 * i + 2] - axyz[3 * j + 2];
+    d = sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  dist[idx] = d;
+}
+} // namespace
+
+double *noalloc_scratch(int device) {
+  OCCXC_REQUIRE(device >= 0 && device < kMaxScratchDevices, OCCXC_ERR_INVALID_ARGUMENT,
+                "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    OCCXC_HIP(hipGetDevice(&cur));
+    OCCXC_REQUIRE(cur == device, OCCXC_ERR_INTERNAL, "scratch set-up on a device that is not current");
+    void *p = nullptr;
+    OCCXC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+  }
+  return g_scratch_ptr[device];
+}
+
+void becke_partition_noalloc(int natoms, const double *atom_xyz, const double *radii, int ta,
+                             long long npts, const double *pts, const int *atom_of_point,
+                             double *w, hipStream_t s) {
+  OCCXC_REQUIRE(natoms >= 1 && natoms <= kNoAllocMaxAtoms, OCCXC_ERR_INVALID_ARGUMENT,
+                "becke_partition_noalloc: natoms must be 1.." + std::to_string(kNoAllocMaxAtoms) +
+                    " (the size of its fixed device scratch table)");
+  int dev = 0;
+  OCCXC_HIP(hipGetDevice(&dev));
+  double *dist = noalloc_scratch(dev);
+  const long long nn = (long long)natoms * natoms;
+  k_atom_distances<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(natoms, atom_xyz, dist);
+  check_launch();
+  launch_becke(natoms, atom_xyz, radii, dist, ta, npts, pts, atom_of_point, w, s);
 }
 
 } // namespace occxc

@@ -266,5 +266,14 @@ void launch_becke(int natoms, const double *atom_xyz, const double *radii,
                   const double *dist, int ta, long long npts,
                   const double *pts, const int *atom_of_point, double *w,
                   hipStream_t s);
+// Becke partition on caller-owned device buffers, its inter-atomic distance
+// table in a fixed device array (at most kNoAllocMaxAtoms atoms); allocates
+// nothing.  noalloc_scratch(device) resolves that array's address on a device
+// (the current one) once.
+constexpr int kNoAllocMaxAtoms = 1024;
+double *noalloc_scratch(int device);
+void becke_partition_noalloc(int natoms, const double *atom_xyz, const double *radii, int ta,
+                             long long npts, const double *pts, const int *atom_of_point,
+                             double *w, hipStream_t s);
 
 } // namespace occxc
