@@ -297,11 +297,14 @@ typedef struct {
   long long admissions_deferred, evictions;
   long long builds_submitted, builds_completed, builds_failed;
   double resident_bytes, budget_bytes;
+  long long builds_cancelled;
 } occxc_server_stats;
 
 void occxc_server_default_options(occxc_server_options *opt);
 occxc_status occxc_server_create(const occxc_server_options *opt, occxc_server **srv);
-/* Runs every pending build, frees every SCF, then stops the workers. */
+/* Cancels every build that has not started (its ticket completes with
+ * OCCXC_ERR_CANCELLED), lets the running ones finish, frees every SCF, then
+ * stops the workers. */
 occxc_status occxc_server_destroy(occxc_server *srv);
 /* Registers an SCF (inputs copied; opt->device is the server's). */
 occxc_status occxc_server_open(occxc_server *srv, const occxc_basis *basis,
@@ -315,7 +318,13 @@ occxc_status occxc_server_submit(occxc_server *srv, long long scf, const double 
  * to completion.  A ticket is waited for once. */
 occxc_status occxc_server_wait(occxc_server *srv, long long ticket, double *latency_ms);
 occxc_status occxc_server_poll(occxc_server *srv, long long ticket, int *done);
-/* The SCF's pending builds still run; then its context is freed. */
+/* Cancels a build that has not started: its ticket completes at once with
+ * OCCXC_ERR_CANCELLED and *cancelled is 1.  A build that is running or done
+ * is left alone (*cancelled = 0; its ticket reports the build's own result).
+ * No counterpart in the reference (one SCF per process, no request queue). */
+occxc_status occxc_server_cancel(occxc_server *srv, long long ticket, int *cancelled);
+/* Closes an SCF: its builds that have not started are cancelled
+ * (OCCXC_ERR_CANCELLED), a running one finishes, then its context is freed. */
 occxc_status occxc_server_close(occxc_server *srv, long long scf);
 occxc_status occxc_server_get_stats(occxc_server *srv, occxc_server_stats *stats);
 

@@ -64,7 +64,8 @@ class CServerStats(ctypes.Structure):
         "scfs_opened", "scfs_admitted", "scfs_closed", "scfs_resident", "scfs_waiting",
         "admissions_deferred", "evictions", "builds_submitted", "builds_completed",
         "builds_failed")] + [("resident_bytes", ctypes.c_double),
-                             ("budget_bytes", ctypes.c_double)]
+                             ("budget_bytes", ctypes.c_double),
+                             ("builds_cancelled", ctypes.c_longlong)]
 
 
 # every symbol declared in include/occxc.h: name -> (restype, argtypes)
@@ -149,6 +150,7 @@ SIGNATURES = {
                                            _dp, _llp]),
     "occxc_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _dp]),
     "occxc_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _ip]),
+    "occxc_server_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _ip]),
     "occxc_server_close": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "occxc_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CServerStats)]),
     "occxc_last_error_message": (ctypes.c_char_p, []),

@@ -610,8 +610,11 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   for (const LeafDev &L : c->leaves)
     c->mol_bytes[L.mol] += (double)c->ncomp * L.mp * L.np * c->ao_elem_size + 64.0 * L.np +
                            4.0 * (L.mp + L.nsig) + sizeof(LeafDev) +
-                           (two_pass ? 8.0 * L.mp * L.mp : 0.0);
-  const size_t planned = ao_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
+                           (two_pass ? 8.0 * L.mp * L.mp : tile_plan_bytes(L));
+  double tile_bytes = 0.0;
+  if (!two_pass)
+    for (const LeafDev &L : c->leaves) tile_bytes += tile_plan_bytes(L);
+  const size_t planned = ao_bytes + (size_t)tile_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
                          (c->mgga ? (size_t)c->ptot * 8 * 8 : 0) +
                          c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                          (two_pass ? (size_t)c->kloc_elems * 8 : 0);
@@ -662,6 +665,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.D2 = c->d_D2.get();
   a.K = c->d_K.get();
   a.coef = c->d_coef.get();
+  a.coef_planes = 4;
+  a.coef_tau_planes = 0;
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
 
@@ -844,6 +849,7 @@ static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, dou
   t.run(OCCXC_STAGE_VXC, [&] {
     BuildArgs x = a;
     x.coef_tau = c->d_coef_tau.get();
+    x.coef_tau_planes = 1;
     run_vxc_mgga(c->plan, x, c->d_leaves_k.get(), c->d_ao.get(), prec, s);
   });
   t.run(OCCXC_STAGE_FINALIZE, [&] {
@@ -996,6 +1002,7 @@ void run_gradient(occxc_context *c, const double *d_D, double *d_grad, hipStream
       launch_xc_gradient_partials(a, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart.get(), s);
       BuildArgs x = a;
       x.coef_tau = c->d_coef_tau.get();
+      x.coef_tau_planes = 1;
       launch_xc_gradient_tau(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart2.get(), s);
       launch_grad_reduce(c->d_gpart.get(), c->d_gpart2.get(), c->grad_natoms,
                          c->d_gatom_off.get(), c->d_gatom_ent.get(), d_grad, s);
@@ -1063,9 +1070,11 @@ void run_build_u(occxc_context *c, hipStream_t s) {
     t.run(OCCXC_STAGE_VXC, [&] {
       BuildArgs x = a;
       x.coef_tau = c->d_coef_tau.get();
+      x.coef_tau_planes = 1;
       run_vxc_mgga(c->plan, x, c->d_leaves_k.get(), c->d_ao.get(), prec, s);
       x.coef = c->d_coef_b.get();
       x.coef_tau = c->d_coef_tau_b.get();
+      x.coef_tau_planes = 1;
       x.K = c->d_K_b.get();
       run_vxc_mgga(c->plan, x, c->d_leaves_k.get(), c->d_ao.get(), prec, s);
     });
@@ -1121,11 +1130,13 @@ void run_gradient_u(occxc_context *c, double *d_grad, hipStream_t s) {
       BuildArgs x = a;
       x.D2 = c->d_D2a.get();
       x.coef_tau = c->d_coef_tau.get();
+      x.coef_tau_planes = 1;
       launch_xc_gradient_partials(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart.get(), s);
       launch_xc_gradient_tau(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart3.get(), s);
       x.D2 = c->d_D2b.get();
       x.coef = c->d_coef_b.get();
       x.coef_tau = c->d_coef_tau_b.get();
+      x.coef_tau_planes = 1;
       launch_xc_gradient_partials(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart2.get(), s);
       launch_xc_gradient_tau(x, c->d_ao.get(), fp32, c->d_ao2.get(), c->d_gpart4.get(), s);
       launch_grad_reduce(c->d_gpart.get(), c->d_gpart2.get(), c->grad_natoms,

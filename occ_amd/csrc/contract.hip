@@ -1901,6 +1901,13 @@ void check_launch(const char *what) {
 
 } // namespace
 
+void check_coef_planes(int need, int have, const char *what) {
+  if (need > have)
+    throw Error(OCCXC_ERR_INTERNAL, std::string(what) + ": the kernels access " +
+                                        std::to_string(need) + " coefficient planes, the buffer holds " +
+                                        std::to_string(have));
+}
+
 // LDS of k_vxc_mfma<T, 64>: coefficients + a 64 x (128 + pad) A chunk
 static size_t vxc_smem(size_t elem) {
   const size_t pad = 16 / elem;
@@ -1919,6 +1926,14 @@ static std::vector<std::pair<int, int>> tile_blocks(int n) {
     at += 32 * u;
   }
   return b;
+}
+
+double tile_plan_bytes(const LeafDev &L) {
+  const double ncb = (double)tile_blocks(L.mp).size();
+  const double npb = (double)((L.np + kTileAo - 1) / kTileAo);
+  const double parts = ncb * npb > 1.0 ? ncb * npb : 0.0;
+  return parts * 4 * kTileAo * sizeof(double) + ncb * npb * sizeof(DenItem) +
+         ncb * (ncb + 1) / 2 * sizeof(VxcItem) + sizeof(DenFinish);
 }
 
 // Item lists of the tiled path.  The active leaves (Morton order) are cut
@@ -2091,6 +2106,7 @@ void run_density_store(const BuildArgs &a, const void *ao, int precision, const 
 
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s) {
+  check_coef_planes(4, a.coef_planes, "density stage");
   if (a.nleaves == 0) return;
   if (plan.tiled) {
     auto launch = [&](auto s_tag) {
@@ -2208,6 +2224,7 @@ void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s) {
 
 void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
              hipStream_t s) {
+  check_coef_planes(a.deriv ? 4 : 1, a.coef_planes, "Vxc stage");
   if (a.nleaves == 0) return;
   if (plan.tiled) {
     auto launch = [&](auto s_tag) {
@@ -2264,6 +2281,8 @@ void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int p
 
 void run_vxc_mgga(const ContractPlan &plan, const BuildArgs &a, const LeafDev *leaves_k,
                   const void *ao, int precision, hipStream_t s) {
+  check_coef_planes(4, a.coef_planes, "meta-GGA Vxc stage (GGA term)");
+  check_coef_planes(1, a.coef_tau_planes, "meta-GGA Vxc stage (tau terms)");
   if (a.nleaves == 0) return;
   // leaves of up to kSymMaxMp AOs: the GGA term and the three tau terms in
   // one symmetric launch; the others: the general kernel on the GGA term and
@@ -2284,7 +2303,10 @@ void run_vxc_mgga(const ContractPlan &plan, const BuildArgs &a, const LeafDev *l
         y.deriv = 0;
         y.leaves = leaves_k + (size_t)k * a.nleaves;
         y.coef = const_cast<double *>(a.coef_tau);
+        y.coef_planes = a.coef_tau_planes;
         y.coef_tau = nullptr;
+        y.coef_tau_planes = 0;
+        check_coef_planes(y.deriv ? 4 : 1, y.coef_planes, "meta-GGA Vxc stage (tau plane)");
         k_vxc_mfma<S, double, 64><<<a.nleaves, 256, vxc_smem(sizeof(double)), s>>>(y, x, min_mp);
       }
     }

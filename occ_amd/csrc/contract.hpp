@@ -38,6 +38,10 @@ struct ContractPlan {
   size_t bytes = 0;
 };
 
+// Device bytes the tiled path's item lists and density partials take for
+// one leaf (an upper bound used before the plan exists)
+double tile_plan_bytes(const LeafDev &L);
+
 // Vxc kernel selection (OCCXC_VXC_VARIANT, default 0)
 int vxc_variant_setting();
 

@@ -147,7 +147,8 @@ struct BuildArgs {
   const double *px, *py, *pz, *pw;
   const double *D2;     // doubled, symmetrised densities (concatenated nbf x nbf)
   double *K;            // accumulated phi^T A (concatenated nbf x nbf)
-  double *coef;         // per-point Vxc coefficients: [4][ptot] (deriv 1), [ptot] (deriv 0)
+  double *coef;         // per-point Vxc coefficients [4][ptot] (the density stage writes 4
+                        // planes; the Vxc stage reads 4 for a GGA, 1 for an LDA)
   double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
   int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
   double *kloc;         // two-pass Vxc: per-leaf K_loc blocks (null: atomics into K)
@@ -159,7 +160,14 @@ struct BuildArgs {
   double *tpartial;     // [ntchunks][32 * 32] partial blocks
   double *rho_out;      // unrestricted: [4][ptot] rho, grad rho of one spin (density store)
   const double *coef_tau;  // restricted meta-GGA: [ptot] tau coefficients (null otherwise)
+  int coef_planes;      // planes of ptot doubles that coef holds
+  int coef_tau_planes;  // planes that coef_tau holds (0 without one)
 };
+
+// The launch wrappers check, before any launch, that the coefficient buffers
+// hold every plane their kernels read or write; a mismatch throws an
+// OCCXC_ERR_INTERNAL error.
+void check_coef_planes(int need, int have, const char *what);
 
 // device set-up of the screened layout (setup.hip)
 int hierarchy_leaf_count(long long n);

@@ -32,8 +32,10 @@
 #include "occxc.h"
 #include "common.hpp"
 #include "context.hpp"
+#include "server_policy.hpp"
 
 using namespace occxc;
+using occxc::serving::ScfState;
 
 namespace {
 
@@ -93,8 +95,6 @@ struct Ticket {
   double latency_ms = 0.0;
 };
 
-enum class ScfState { Waiting, Admitting, Resident };
-
 struct Scf {
   long long id = 0;
   std::unique_ptr<ScfInputs> in;
@@ -130,59 +130,17 @@ struct occxc_server {
 
 namespace {
 
-// A waiting SCF with work may be admitted if its footprint (when already
-// known from an earlier try) fits beside the resident ones.  Admission is in
-// registration order: a large SCF at the head is not overtaken forever.
-Scf *admission_head(occxc_server *s) {
-  for (auto &kv : s->scfs) {
-    Scf *f = kv.second.get();
-    if (f->state == ScfState::Waiting && !f->busy && !f->pending.empty()) return f;
-  }
-  return nullptr;
-}
-
 Scf *next_admission(occxc_server *s) {
-  if (s->admitting) return nullptr;
-  Scf *f = admission_head(s);
-  if (f && f->bytes && s->resident_bytes + f->bytes > s->budget) return nullptr;
-  return f;
+  return serving::next_admission(s->scfs, s->admitting, s->resident_bytes, s->budget);
 }
 
-// An idle resident SCF to evict so that the admission head fits.
 Scf *next_eviction(occxc_server *s) {
-  if (s->admitting) return nullptr;
-  Scf *h = admission_head(s);
-  if (!h || !h->bytes || s->resident_bytes + h->bytes <= s->budget) return nullptr;
-  Scf *victim = nullptr;
-  size_t idle = 0;
-  for (auto &kv : s->scfs) {
-    Scf *f = kv.second.get();
-    if (f->state != ScfState::Resident || f->busy || !f->pending.empty()) continue;
-    idle += f->bytes;
-    if (!victim || f->last_used < victim->last_used) victim = f;
-  }
-  if (!victim || s->resident_bytes - idle + h->bytes > s->budget) return nullptr;
-  return victim;
+  return serving::next_eviction(s->scfs, s->admitting, s->resident_bytes, s->budget);
 }
 
-// The resident SCF whose oldest pending request arrived first.
-Scf *next_build(occxc_server *s) {
-  Scf *best = nullptr;
-  for (auto &kv : s->scfs) {
-    Scf *f = kv.second.get();
-    if (f->state != ScfState::Resident || f->busy || f->pending.empty()) continue;
-    if (!best || f->pending.front().arrival < best->pending.front().arrival) best = f;
-  }
-  return best;
-}
+Scf *next_build(occxc_server *s) { return serving::next_build(s->scfs); }
 
-Scf *next_release(occxc_server *s) {
-  for (auto &kv : s->scfs) {
-    Scf *f = kv.second.get();
-    if (f->closing && !f->busy && f->pending.empty()) return f;
-  }
-  return nullptr;
-}
+Scf *next_release(occxc_server *s) { return serving::next_release(s->scfs); }
 
 void finish(occxc_server *s, const Request &r, occxc_status st, const std::string &msg) {
   Ticket &t = s->tickets[r.ticket];
@@ -191,7 +149,31 @@ void finish(occxc_server *s, const Request &r, occxc_status st, const std::strin
   t.message = msg;
   t.latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r.arrival).count();
   if (st == OCCXC_OK) s->stats.builds_completed += 1;
+  else if (st == OCCXC_ERR_CANCELLED) s->stats.builds_cancelled += 1;
   else s->stats.builds_failed += 1;
+}
+
+// Completes every queued (not started) request of an SCF as cancelled.
+void cancel_pending(occxc_server *s, Scf *f) {
+  while (!f->pending.empty()) {
+    finish(s, f->pending.front(), OCCXC_ERR_CANCELLED, "build cancelled before it started");
+    f->pending.pop_front();
+  }
+}
+
+long long scf_nbf(const ScfInputs &in) {
+  long long nbf = 0;
+  for (int i = 0; i < in.basis.nshell; ++i) {
+    const int l = in.l[i];
+    nbf += in.pure[i] ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
+  }
+  return nbf;
+}
+
+// device bytes of an SCF's staging buffers (D, Vxc, {Exc, N_elec})
+size_t staging_bytes(const ScfInputs &in) {
+  const long long nbf = scf_nbf(in);
+  return (size_t)(2 * nbf * nbf + 2) * sizeof(double);
 }
 
 void admit(occxc_server *s, Scf *f, std::unique_lock<std::mutex> &lk) {
@@ -202,12 +184,16 @@ void admit(occxc_server *s, Scf *f, std::unique_lock<std::mutex> &lk) {
   s->resident_bytes += f->bytes;  // reserve what is known so far
   const size_t reserved = f->bytes;
   lk.unlock();
+  // the context may use what the staging buffers leave of the limit (a
+  // limit of 0 would mean "no limit" to context_setup: keep it at least 1)
+  const size_t staging = staging_bytes(*f->in);
+  const size_t ctx_limit = limit > staging ? limit - staging : 1;
   auto c = new occxc_context();
   occxc_status st = OCCXC_OK;
   std::string msg;
   st = guarded([&] {
     try {
-      context_setup(c, 1, &f->in->basis, &f->in->grid, &f->in->func, &f->in->opt, limit);
+      context_setup(c, 1, &f->in->basis, &f->in->grid, &f->in->func, &f->in->opt, ctx_limit);
       DeviceGuard g(c->device);
       f->d_D.alloc((size_t)c->nbf * c->nbf);
       f->d_V.alloc((size_t)c->nbf * c->nbf);
@@ -223,7 +209,7 @@ void admit(occxc_server *s, Scf *f, std::unique_lock<std::mutex> &lk) {
     }
   });
   if (st != OCCXC_OK) msg = occxc_last_error_message();
-  const size_t need = c->device_bytes;
+  const size_t need = c->device_bytes ? c->device_bytes + staging : 0;
   if (st != OCCXC_OK) {
     delete c;
     c = nullptr;
@@ -395,8 +381,13 @@ occxc_status occxc_server_destroy(occxc_server *srv) {
     if (!srv) return;
     {
       std::unique_lock<std::mutex> lk(srv->mu);
-      for (auto &kv : srv->scfs) kv.second->closing = true;
-      // every pending build runs, then every SCF is released
+      // builds not yet started are cancelled, running ones finish, then
+      // every SCF is released
+      for (auto &kv : srv->scfs) {
+        cancel_pending(srv, kv.second.get());
+        kv.second->closing = true;
+      }
+      srv->cv_done.notify_all();
       srv->cv_work.notify_all();
       srv->cv_done.wait(lk, [&] { return srv->scfs.empty(); });
       srv->stop = true;
@@ -434,11 +425,7 @@ occxc_status occxc_server_submit(occxc_server *srv, long long scf, const double 
                   "server: unknown or closed SCF");
     Scf *f = it->second.get();
     // nbf from the basis inputs (the context may not exist yet)
-    long long nbf = 0;
-    for (int i = 0; i < f->in->basis.nshell; ++i) {
-      const int l = f->in->l[i];
-      nbf += f->in->pure[i] ? 2 * l + 1 : (l + 1) * (l + 2) / 2;
-    }
+    const long long nbf = scf_nbf(*f->in);
     Request r;
     r.ticket = srv->next_ticket++;
     r.scf = scf;
@@ -491,13 +478,39 @@ occxc_status occxc_server_poll(occxc_server *srv, long long ticket, int *done) {
   });
 }
 
+occxc_status occxc_server_cancel(occxc_server *srv, long long ticket, int *cancelled) {
+  return guarded([&] {
+    OCCXC_REQUIRE(srv && cancelled, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
+    *cancelled = 0;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    auto it = srv->tickets.find(ticket);
+    OCCXC_REQUIRE(it != srv->tickets.end(), OCCXC_ERR_INVALID_ARGUMENT, "server: unknown ticket");
+    if (it->second.done) return;
+    for (auto &kv : srv->scfs) {
+      auto &q = kv.second->pending;
+      for (auto r = q.begin(); r != q.end(); ++r) {
+        if (r->ticket != ticket) continue;
+        finish(srv, *r, OCCXC_ERR_CANCELLED, "build cancelled before it started");
+        q.erase(r);
+        *cancelled = 1;
+        srv->cv_done.notify_all();
+        srv->cv_work.notify_all();
+        return;
+      }
+    }
+    // not queued: the build is running and completes normally
+  });
+}
+
 occxc_status occxc_server_close(occxc_server *srv, long long scf) {
   return guarded([&] {
     OCCXC_REQUIRE(srv, OCCXC_ERR_INVALID_ARGUMENT, "null pointer");
     std::lock_guard<std::mutex> lk(srv->mu);
     auto it = srv->scfs.find(scf);
     OCCXC_REQUIRE(it != srv->scfs.end(), OCCXC_ERR_INVALID_ARGUMENT, "server: unknown SCF");
+    cancel_pending(srv, it->second.get());
     it->second->closing = true;
+    srv->cv_done.notify_all();
     srv->cv_work.notify_all();
   });
 }

@@ -72,6 +72,13 @@ class XCServer:
         _lib.check(_lib.lib().occxc_server_poll(self._h, ticket, ctypes.byref(d)))
         return bool(d.value)
 
+    def cancel(self, ticket):
+        """Cancels a build that has not started; True if it was cancelled
+        (its wait() then raises with status CANCELLED)."""
+        c = ctypes.c_int()
+        _lib.check(_lib.lib().occxc_server_cancel(self._h, ticket, ctypes.byref(c)))
+        return bool(c.value)
+
     def close_scf(self, scf):
         _lib.check(_lib.lib().occxc_server_close(self._h, scf))
 

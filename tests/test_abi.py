@@ -122,3 +122,22 @@ def test_unrestricted_and_polarised_reject_without_gpu():
     st = L.occxc_functional_evaluate_polarized(9999, 1, _lib.dptr(x), _lib.dptr(x), _lib.dptr(x),
                                                _lib.dptr(x), _lib.dptr(x), 0)
     assert st == 2
+
+
+def test_launch_wrappers_refuse_short_coefficient_buffers(tmp_path):
+    """The density and Vxc launch wrappers check that the per-point
+    coefficient buffers hold every plane their kernels access, before any
+    launch (tests/native/launch_guard.cpp calls them with a one-plane buffer
+    for a GGA and with no tau buffer for a meta-GGA: OCCXC_ERR_INTERNAL)."""
+    import subprocess
+    root = pathlib.Path(__file__).resolve().parent.parent
+    exe = tmp_path / "launch_guard"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    f"-I{root / 'occ_amd' / 'csrc'}", f"-I{root / 'include'}",
+                    str(root / "tests" / "native" / "launch_guard.cpp"),
+                    f"-L{root / 'occ_amd'}", "-loccxc", "-L/opt/rocm/lib", "-lamdhip64",
+                    f"-Wl,-rpath,{root / 'occ_amd'}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    print(out.stdout)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.count("status 7") == 3

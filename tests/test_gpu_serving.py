@@ -123,3 +123,39 @@ def test_server_rejects_scf_larger_than_budget():
         with pytest.raises(OccxcError) as e:
             srv.wait(t)
         assert e.value.status == 4
+
+
+def test_server_cancels_builds_that_have_not_started():
+    """occxc_server_cancel completes a queued build at once with
+    OCCXC_ERR_CANCELLED; a build already running finishes normally.  Closing
+    an SCF cancels its queued builds."""
+    from occ_amd._lib import OccxcError
+    name, basis, pts, w, Ds = _systems()[1]
+    with XCServer(slots=1) as srv:
+        scf = srv.open(basis, pts, w, FUNCS)
+        tickets = [srv.submit(scf, Ds[k % 3]) for k in range(6)]
+        cancelled = {t: srv.cancel(t) for t in tickets[3:]}
+        assert any(cancelled.values())
+        for k, t in enumerate(tickets):
+            if cancelled.get(t):
+                with pytest.raises(OccxcError) as e:
+                    srv.wait(t)
+                assert e.value.status == 6
+            else:
+                out = srv.wait(t)
+                assert _matches_oracle(out, _oracle(basis, pts, w, Ds[k % 3])), k
+        st = srv.stats()
+        assert st["builds_cancelled"] == sum(cancelled.values())
+        assert st["builds_completed"] == 6 - sum(cancelled.values())
+        more = [srv.submit(scf, Ds[0]) for _ in range(4)]
+        srv.close_scf(scf)
+        n_cancelled = 0
+        for t in more:
+            try:
+                out = srv.wait(t)
+                assert _matches_oracle(out, _oracle(basis, pts, w, Ds[0]))
+            except OccxcError as err:
+                assert err.status == 6
+                n_cancelled += 1
+        assert n_cancelled >= 1
+        assert srv.stats()["builds_cancelled"] == sum(cancelled.values()) + n_cancelled
