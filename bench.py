@@ -679,12 +679,16 @@ def run_batch(args):
         if cur:
             chunks.append(cur)
         warm = cold = 0.0
+        tuned = []
         exc = np.zeros(len(mols))
         nel = np.zeros(len(mols))
         for ch in chunks:
             synchronize()
             t1 = time.perf_counter()
             batch = XCBatch([bases[i] for i in ch], [grids[i] for i in ch], ids, precision=mode)
+            binfo = batch.info()
+            tuned.append({"molecules": len(ch), "tile_ao": binfo["tile_ao"],
+                          "persistent": binfo["persistent"], "tuning_row": binfo["tuning_row"]})
             off = batch.offsets
             tot = int(off[-1] + batch.nbf[-1] ** 2)
             hD = np.zeros(tot)
@@ -709,7 +713,7 @@ def run_batch(args):
                 a.free()
         results[mode] = {"warm_s": warm, "cold_s": cold, "chunks": len(chunks),
                          "estimate_s": t_est, "exc": exc, "nel": nel,
-                         "bytes": float(sum(est))}
+                         "bytes": float(sum(est)), "tuning": tuned}
         log(f"batch {mode}: {len(chunks)} contexts, warm {warm * 1e3:.1f} ms "
             f"({len(mols) / warm:.0f} mol/s), cold {cold:.2f} s, estimate {t_est:.1f} s, "
             f"{sum(est) / 1e9:.1f} GB")
@@ -764,7 +768,8 @@ def run_batch(args):
         "modes": {m: {"warm_molecules_per_s": len(mols) / results[m]["warm_s"],
                       "cold_molecules_per_s": len(mols) / results[m]["cold_s"],
                       "contexts": results[m]["chunks"], "device_gb": results[m]["bytes"] / 1e9,
-                      "max_dexc_sample": worst[m]} for m in results},
+                      "max_dexc_sample": worst[m], "launch_settings": results[m]["tuning"]}
+                  for m in results},
         "grid_build_s": t_grid,
     }
     return out

@@ -97,6 +97,11 @@ typedef struct {
   double screening_threshold;/* shell decay cutoff, m_xc_screening_threshold = 1e-10 (dft.h:761) */
   int screen;                /* 1: screened leaf path (dft.h:279-372); 0: dense path (OCC_XC_NO_SCREEN) */
   int device;                /* HIP device ordinal */
+  /* launch tuning of the FP64 / FP32-cache contractions (no effect on the
+   * results beyond FP64 rounding); the defaults take the library's tuning
+   * table entry for the context's largest molecule */
+  int tile_ao;               /* largest AO block per work item: 32..128, a multiple of 32; 0: table */
+  int persistent;            /* bit 0: persistent density launch, bit 1: persistent Vxc launch; -1: table */
 } occxc_options;
 
 typedef struct occxc_context occxc_context;
@@ -112,7 +117,9 @@ typedef struct {
   double ao_write_bytes;     /* algorithmic bytes of one AO evaluation */
   double device_bytes;       /* total device footprint of the context */
   int max_local_nbf;         /* largest m_b over the active leaves */
-  int pad_;
+  int tile_ao;               /* launch settings in use (occxc_options) */
+  int persistent;
+  int tuning_row;            /* the tuning table entry they came from (-1: options) */
 } occxc_context_info;
 
 /* --- the XC build ---------------------------------------------------- */

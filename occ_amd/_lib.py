@@ -42,7 +42,7 @@ class CFunctional(ctypes.Structure):
 class COptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("density_threshold", ctypes.c_double),
                 ("screening_threshold", ctypes.c_double), ("screen", ctypes.c_int),
-                ("device", ctypes.c_int)]
+                ("device", ctypes.c_int), ("tile_ao", ctypes.c_int), ("persistent", ctypes.c_int)]
 
 
 class CContextInfo(ctypes.Structure):
@@ -51,7 +51,8 @@ class CContextInfo(ctypes.Structure):
                 ("mean_local_nbf", ctypes.c_double), ("contraction_flops", ctypes.c_double),
                 ("ao_bytes", ctypes.c_double), ("ao_write_bytes", ctypes.c_double),
                 ("device_bytes", ctypes.c_double), ("max_local_nbf", ctypes.c_int),
-                ("pad_", ctypes.c_int)]
+                ("tile_ao", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("tuning_row", ctypes.c_int)]
 
 
 class CServerOptions(ctypes.Structure):

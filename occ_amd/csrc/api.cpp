@@ -375,6 +375,11 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                 "need at least one molecule (basis and grid)");
   OCCXC_REQUIRE(opt->precision >= 0 && opt->precision <= 3, OCCXC_ERR_INVALID_ARGUMENT,
                 "options: unknown precision mode");
+  OCCXC_REQUIRE(opt->tile_ao == 0 || (opt->tile_ao >= 32 && opt->tile_ao <= kTileAo &&
+                                      opt->tile_ao % 32 == 0),
+                OCCXC_ERR_INVALID_ARGUMENT, "options: tile_ao must be 0 or 32..128, a multiple of 32");
+  OCCXC_REQUIRE(opt->persistent >= -1 && opt->persistent <= 3, OCCXC_ERR_INVALID_ARGUMENT,
+                "options: persistent must be -1 or 0..3");
   OCCXC_REQUIRE(func && func->nfunc >= 1 && func->nfunc <= kMaxFunc && func->ids,
                 OCCXC_ERR_INVALID_ARGUMENT, "functional: need 1..8 functional ids");
   int deriv = 0;
@@ -670,7 +675,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
 
-  c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream);
+  c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream,
+                               tune_choice(opt, c->maxnbf));
   c->plan.two_pass = c->plan.two_pass && c->plan.use_mfma;
   if (c->plan.two_pass) {
     if (ao_map.empty() && c->d_ao_map.size()) {
@@ -1254,6 +1260,8 @@ void occxc_default_options(occxc_options *opt) {
   opt->screening_threshold = 1e-10;
   opt->screen = 1;
   opt->device = 0;
+  opt->tile_ao = 0;
+  opt->persistent = -1;
 }
 
 occxc_status occxc_batch_create(int nmol, const occxc_basis *bases, const occxc_grid *grids,
@@ -1334,7 +1342,9 @@ occxc_status occxc_context_get_info(const occxc_context *ctx, occxc_context_info
     info->device_bytes = (double)ctx->device_bytes;
     info->max_local_nbf = 0;
     for (const auto &L : ctx->leaves) info->max_local_nbf = std::max(info->max_local_nbf, L.m);
-    info->pad_ = 0;
+    info->tile_ao = ctx->plan.tiled ? ctx->plan.tune.tile_ao : 0;
+    info->persistent = ctx->plan.tiled ? ctx->plan.tune.persistent : 0;
+    info->tuning_row = ctx->plan.tiled ? ctx->plan.tune.table_row : -1;
   });
 }
 

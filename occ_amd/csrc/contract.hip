@@ -29,6 +29,7 @@
 #include <string>
 
 #include "contract.hpp"
+#include "tuning_table.hpp"
 #include "xc_functionals.hpp"
 
 namespace occxc {
@@ -1403,19 +1404,15 @@ constexpr int kTVld = kTVp + 2;  // LDS row stride (doubles) of the [row][point]
 // column-block partials of rho = sum_nu Phi T and
 // grad rho = 2 sum_nu grad Phi T (density.cpp:32-43); the leaf's only item
 // runs the functional epilogue at once, the others store their partials.
+// One density item (the body of k_den_tile): s_phi, s_d, s_map and s_red
+// are the workgroup's LDS arrays.
 template <typename S, bool GGA>
-__global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__restrict__ ao,
-                                                     const DenItem *__restrict__ items,
-                                                     double *__restrict__ part) {
-  __shared__ double s_phi[kTileAo * kTKld];  // [point][k]; after the k loop s_q and s_h
-  __shared__ double s_d[kTileAo * kTKld];    // [column][k]
-  __shared__ int s_map[kMaxLocalAo];
-  __shared__ double s_red[4];
-  static_assert(4 * kMaxLeafPts + 4 * kTileAo <= kTileAo * kTKld, "epilogue sums fit s_phi");
+__device__ __forceinline__ void den_tile_item(const BuildArgs &a, const S *__restrict__ ao,
+                                              const DenItem &it, double *__restrict__ part,
+                                              double *s_phi, double *s_d, int *s_map,
+                                              double *s_red) {
   double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(s_phi);
   double(*s_h)[kTileAo] = reinterpret_cast<double(*)[kTileAo]>(s_phi + 4 * kMaxLeafPts);
-  const DenItem it = items[blockIdx.x];
-  if (it.leaf < 0) return;
   const LeafDev L = a.leaves[it.leaf];
   const MolDev MD = a.mols[L.mol];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -1573,6 +1570,32 @@ __global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__res
       const int k = u / it.np, p = u - k * it.np;
       dst[k * kTileAo + p] = s_q[k][p] + s_h[k][p];
     }
+  }
+}
+
+// PERSIST: the workgroups loop over the items blockIdx.x, blockIdx.x +
+// gridDim.x, ... (gridDim.x a multiple of 8, so every item of a workgroup is
+// from the same XCD's run of leaves) instead of one item each.
+template <typename S, bool GGA, bool PERSIST = false>
+__global__ __launch_bounds__(256, 2) void k_den_tile(BuildArgs a, const S *__restrict__ ao,
+                                                     const DenItem *__restrict__ items,
+                                                     double *__restrict__ part, int n_items) {
+  __shared__ double s_phi[kTileAo * kTKld];  // [point][k]; after the k loop s_q and s_h
+  __shared__ double s_d[kTileAo * kTKld];    // [column][k]
+  __shared__ int s_map[kMaxLocalAo];
+  __shared__ double s_red[4];
+  static_assert(4 * kMaxLeafPts + 4 * kTileAo <= kTileAo * kTKld, "epilogue sums fit s_phi");
+  if constexpr (PERSIST) {
+    for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+      const DenItem it = items[item];
+      if (it.leaf < 0) continue;
+      den_tile_item<S, GGA>(a, ao, it, part, s_phi, s_d, s_map, s_red);
+      __syncthreads();  // this item's LDS reads are done before the next one's writes
+    }
+  } else {
+    const DenItem it = items[blockIdx.x];
+    if (it.leaf < 0) return;
+    den_tile_item<S, GGA>(a, ao, it, part, s_phi, s_d, s_map, s_red);
   }
 }
 
@@ -1914,11 +1937,12 @@ static size_t vxc_smem(size_t elem) {
   return sizeof(double) * 4 * kMaxLeafPts + elem * 64 * (128 + pad);
 }
 
-// Blocks of a leaf's padded AOs (or points): n32 units of 32 split into
-// ceil(n32 / 4) blocks of at most 4 units, the first ones one unit larger.
-static std::vector<std::pair<int, int>> tile_blocks(int n) {
+// Blocks of a leaf's padded AOs: n / 32 units of 32 split into the fewest
+// blocks of at most maxw / 32 units, the first ones one unit larger.
+static std::vector<std::pair<int, int>> tile_blocks(int n, int maxw = kTileAo) {
   std::vector<std::pair<int, int>> b;
-  const int units = n / 32, nb = (units + 3) / 4;
+  const int per = std::max(1, maxw / 32);
+  const int units = n / 32, nb = (units + per - 1) / per;
   int at = 0;
   for (int k = 0; k < nb; ++k) {
     const int u = units / nb + (k < units % nb ? 1 : 0);
@@ -1941,6 +1965,31 @@ double tile_plan_bytes(const LeafDev &L) {
 // workgroup i runs on XCD i % 8, so entry 8 k + x of a list is the k-th item
 // of run x (padding entries have leaf = -1) and each XCD's L2 sees a compact
 // run of leaves.
+TuneChoice tune_choice(const occxc_options *opt, int max_nbf) {
+  TuneChoice t;
+  const int rows = (int)(sizeof(kTuneTable) / sizeof(kTuneTable[0]));
+  for (int r = 0; r < rows; ++r)
+    if (max_nbf <= kTuneTable[r].max_nbf || r == rows - 1) {
+      t.tile_ao = kTuneTable[r].tile_ao;
+      t.persistent = kTuneTable[r].persistent;
+      t.table_row = r;
+      break;
+    }
+  if (opt && opt->tile_ao != 0) {
+    OCCXC_REQUIRE(opt->tile_ao >= 32 && opt->tile_ao <= kTileAo && opt->tile_ao % 32 == 0,
+                  OCCXC_ERR_INVALID_ARGUMENT, "options: tile_ao must be 32..128, a multiple of 32");
+    t.tile_ao = opt->tile_ao;
+    t.table_row = -1;
+  }
+  if (opt && opt->persistent >= 0) {
+    OCCXC_REQUIRE(opt->persistent <= 3, OCCXC_ERR_INVALID_ARGUMENT,
+                  "options: persistent must be -1 or 0..3");
+    t.persistent = opt->persistent;
+    t.table_row = -1;
+  }
+  return t;
+}
+
 static void make_tile_plan(ContractPlan &p, const std::vector<LeafDev> &leaves) {
   constexpr int kXcd = 8;
   const int nl = (int)leaves.size();
@@ -1955,7 +2004,7 @@ static void make_tile_plan(ContractPlan &p, const std::vector<LeafDev> &leaves) 
     const LeafDev &L = leaves[li];
     const int x = std::min(kXcd - 1, (int)(acc / std::max(total, 1.0) * kXcd));
     acc += (double)L.np * L.mp * L.mp;
-    const auto cb = tile_blocks(L.mp);
+    const auto cb = tile_blocks(L.mp, p.tune.tile_ao);
     std::vector<std::pair<int, int>> pb;
     for (int p0 = 0; p0 < L.np; p0 += kTileAo) pb.push_back({p0, std::min(kTileAo, L.np - p0)});
     const bool single = cb.size() == 1 && pb.size() == 1;
@@ -2002,10 +2051,11 @@ int vxc_variant_setting() {
 }
 
 ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
-                                int precision, hipStream_t s) {
+                                int precision, hipStream_t s, const TuneChoice &tune) {
   (void)deriv;
   (void)s;
   ContractPlan p;
+  p.tune = tune;
   p.use_mfma = std::getenv("OCCXC_REFERENCE_KERNELS") == nullptr;
   for (const auto &L : leaves) {
     p.max_np = std::max(p.max_np, L.np);
@@ -2031,11 +2081,8 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
     OCCXC_HIP(hipGetDevice(&dev));
     OCCXC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     p.persistent_wgs = std::max(8, cus / 8 * 8);
-    // the persistent form is slower than one workgroup per item on the
-    // single-molecule clusters (staging latency with one workgroup per CU);
-    // OCCXC_VXC_WS=1 selects it
-    const char *ws = std::getenv("OCCXC_VXC_WS");
-    p.vxc_ws = ws && std::atoi(ws) != 0;
+    p.vxc_ws = (p.tune.persistent & 2) != 0;
+    p.den_persistent = (p.tune.persistent & 1) != 0;
     for (const void *k : {reinterpret_cast<const void *>(&k_vxc_ws<double, true>),
                           reinterpret_cast<const void *>(&k_vxc_ws<double, false>),
                           reinterpret_cast<const void *>(&k_vxc_ws<float, true>),
@@ -2112,8 +2159,14 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
     auto launch = [&](auto s_tag) {
       using S = decltype(s_tag);
       const S *x = static_cast<const S *>(ao);
-      (a.deriv ? k_den_tile<S, true> : k_den_tile<S, false>)
-          <<<plan.n_den_items, 256, 0, s>>>(a, x, plan.den_items.get(), plan.den_part.get());
+      if (plan.den_persistent)
+        (a.deriv ? k_den_tile<S, true, true> : k_den_tile<S, false, true>)
+            <<<std::min(2 * plan.persistent_wgs, plan.n_den_items), 256, 0, s>>>(
+                a, x, plan.den_items.get(), plan.den_part.get(), plan.n_den_items);
+      else
+        (a.deriv ? k_den_tile<S, true> : k_den_tile<S, false>)
+            <<<plan.n_den_items, 256, 0, s>>>(a, x, plan.den_items.get(), plan.den_part.get(),
+                                              plan.n_den_items);
       check_launch("k_den_tile");
       if (plan.n_den_finish > 0) {
         k_den_finish<<<plan.n_den_finish, 256, 0, s>>>(a, plan.den_finish.get(),

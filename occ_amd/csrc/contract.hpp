@@ -12,6 +12,17 @@
 
 namespace occxc {
 
+// Launch settings of the tiled contractions (options or tuning table).
+struct TuneChoice {
+  int tile_ao = 128;     // largest AO block per work item (32..128, multiple of 32)
+  int persistent = 0;    // bit 0 persistent density launch, bit 1 persistent Vxc launch
+  int table_row = -1;    // the tuning table entry used (-1: set by the options)
+};
+
+// The options' settings where given, else the tuning table's entry for a
+// context whose largest molecule has max_nbf basis functions.
+TuneChoice tune_choice(const occxc_options *opt, int max_nbf);
+
 struct ContractPlan {
   bool use_mfma = false;   // false: the FP64 reference kernels of kernels.hip
   int max_np = 0;          // largest padded leaf (points)
@@ -26,7 +37,9 @@ struct ContractPlan {
   // them off): item lists interleaved over the XCDs, the multi-item leaves
   // and their partial sums
   bool tiled = false;
-  bool vxc_ws = false;     // persistent wavefront-specialised Vxc (OCCXC_VXC_WS=1) instead of one item per workgroup
+  TuneChoice tune;         // the launch settings in use
+  bool vxc_ws = false;     // persistent wavefront-specialised Vxc instead of one item per workgroup
+  bool den_persistent = false;  // persistent density launch (workgroups loop over items)
   int persistent_wgs = 256;  // workgroups of the persistent kernels (a multiple of 8)
   int n_den_items = 0, n_vxc_items = 0, n_den_finish = 0, n_den_parts = 0;
   DevBuf<DenItem> den_items;
@@ -46,7 +59,7 @@ double tile_plan_bytes(const LeafDev &L);
 int vxc_variant_setting();
 
 ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
-                                int precision, hipStream_t s);
+                                int precision, hipStream_t s, const TuneChoice &tune = {});
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,
                  hipStream_t s);
 // rho and grad rho of every leaf point for the density D2t into a.rho_out

@@ -56,7 +56,8 @@ class XCContext:
     """One occxc context: basis + grid + functional resident on one GPU."""
 
     def __init__(self, basis, points, weights, funcs, precision="fp64", screen=True,
-                 density_threshold=1e-10, screening_threshold=1e-10, device=0):
+                 density_threshold=1e-10, screening_threshold=1e-10, device=0, tile_ao=0,
+                 persistent=-1):
         L = _lib.lib()
         self.nbf = basis.nbf
         self._basis, self._keep = basis.c_struct()
@@ -73,6 +74,8 @@ class XCContext:
         o.density_threshold = density_threshold
         o.screening_threshold = screening_threshold
         o.device = device
+        o.tile_ao = tile_ao
+        o.persistent = persistent
         self.options = o
         h = ctypes.c_void_p()
         _lib.check(L.occxc_context_create(ctypes.byref(self._basis), ctypes.byref(self._grid),
@@ -256,7 +259,8 @@ def estimate_batch_bytes(bases, grids, funcs, precision="fp64", screen=True, dev
 class XCBatch:
     """Many molecules in one occxc context: one launch per stage for all."""
 
-    def __init__(self, bases, grids, funcs, precision="fp64", screen=True, device=0):
+    def __init__(self, bases, grids, funcs, precision="fp64", screen=True, device=0, tile_ao=0,
+                 persistent=-1):
         L = _lib.lib()
         n = len(bases)
         if n != len(grids) or n == 0:
@@ -279,6 +283,8 @@ class XCBatch:
         o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         o.screen = int(bool(screen))
         o.device = device
+        o.tile_ao = tile_ao
+        o.persistent = persistent
         h = ctypes.c_void_p()
         _lib.check(L.occxc_batch_create(n, cb, cg, ctypes.byref(self._func), ctypes.byref(o),
                                         ctypes.byref(h)))

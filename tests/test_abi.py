@@ -141,3 +141,25 @@ def test_launch_wrappers_refuse_short_coefficient_buffers(tmp_path):
     print(out.stdout)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.count("status 7") == 3
+
+
+@pytest.mark.parametrize("field,value", [("tile_ao", 48), ("tile_ao", 256), ("persistent", 4),
+                                         ("persistent", -2)])
+def test_context_create_rejects_bad_launch_settings(field, value):
+    from occ_amd.dft import default_options
+    o = default_options()
+    assert o.tile_ao == 0 and o.persistent == -1
+    setattr(o, field, value)
+    shells = _lib.i32([0])
+    b = _lib.CBasis(1, _lib.iptr(shells), _lib.iptr(shells), _lib.iptr(_lib.i32([1])),
+                    _lib.dptr(np.zeros(3)), _lib.dptr(np.ones(1)), _lib.dptr(np.ones(1)))
+    pts = np.zeros(3)
+    w = np.ones(1)
+    g = _lib.CGrid(1, _lib.dptr(pts), _lib.dptr(w))
+    ids = _lib.i32([1])
+    f = _lib.CFunctional(1, _lib.iptr(ids))
+    h = ctypes.c_void_p()
+    st = _lib.lib().occxc_context_create(ctypes.byref(b), ctypes.byref(g), ctypes.byref(f),
+                                         ctypes.byref(o), ctypes.byref(h))
+    assert st == 1
+    assert field.encode() in _lib.lib().occxc_last_error_message()

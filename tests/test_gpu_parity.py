@@ -8,7 +8,7 @@ import grid_ref
 import oracle
 from conftest import random_density
 from occ_amd import AOBasis, molecules
-from occ_amd.dft import DFT, DensityFunctional, evaluate_basis, evaluate_decay_cutoff
+from occ_amd.dft import DFT, DensityFunctional, XCContext, evaluate_basis, evaluate_decay_cutoff
 from occ_amd.grid import GridSettings, MolecularGrid
 
 pytestmark = pytest.mark.gpu
@@ -381,11 +381,14 @@ def test_vxc_variants_agree(monkeypatch, precision):
     out = []
     for v in ("0", "1", "2", "tiled", "ws"):
         # per-leaf kernels (OCCXC_TILED=0) in each Vxc variant, then the tiled
-        # kernels one item per workgroup and in the persistent form
+        # kernels one item per workgroup and in the persistent forms
         monkeypatch.setenv("OCCXC_TILED", "1" if v in ("tiled", "ws") else "0")
-        monkeypatch.setenv("OCCXC_VXC_WS", "1" if v == "ws" else "0")
         monkeypatch.setenv("OCCXC_VXC_VARIANT", v if v in ("0", "1", "2") else "0")
         dft = DFT("pbe", basis, pts, wts, precision=precision)
+        if v in ("tiled", "ws"):
+            dft.ctx.close()
+            dft.ctx = XCContext(basis, pts, wts, dft.ids, precision=precision, tile_ao=128,
+                                persistent=3 if v == "ws" else 0)
         out.append((dft.compute_vxc(D), dft.exchange_correlation_energy(), dft.total_density()))
         if v == "2":
             # the two-pass form sums in a fixed order: builds repeat bit for bit

@@ -1,0 +1,63 @@
+"""The launch settings the autotuner chooses between (largest AO block per
+work item, persistent density / Vxc launches) change only the launch shape:
+every setting gives the oracle's result (|dExc| <= 1e-6, max|dVxc| <= 1e-7,
+|dN| <= 1e-8) and the settings agree with each other to FP64 rounding, for
+one molecule and for a batch context; the committed tuning table's choice is
+one of them."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import random_density
+from occ_amd import AOBasis, molecules
+from occ_amd.dft import XCBatch, XCContext
+from occ_amd.grid import GridSettings, MolecularGrid
+
+pytestmark = pytest.mark.gpu
+
+IDS = [101, 130]
+SETTINGS = [(0, -1)] + [(t, p) for t in (32, 64, 96, 128) for p in (0, 1, 2, 3)]
+
+
+def _grid(atoms, nrad=31, nang=110):
+    return MolecularGrid(atoms, GridSettings(radial_points=nrad, angular_points=nang)).build(device=0)
+
+
+def test_settings_agree_on_one_molecule():
+    atoms = molecules.water_cluster(6, seed=9)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 30, 91)
+    K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(basis.flat()), pts, wts, IDS, D)
+    out = {}
+    for tile, pers in SETTINGS:
+        ctx = XCContext(basis, pts, wts, IDS, tile_ao=tile, persistent=pers)
+        V, e, n = ctx.build(D)
+        ctx.close()
+        out[tile, pers] = (V, e, n)
+        assert abs(e - exc) <= 1e-6 and abs(n - nel) <= 1e-8 and np.abs(V - K).max() <= 1e-7
+    V0, e0, n0 = out[0, -1]
+    for key, (V, e, n) in out.items():
+        assert np.abs(V - V0).max() <= 1e-12 and abs(e - e0) <= 1e-12 and abs(n - n0) <= 1e-12, key
+
+
+def test_settings_agree_on_a_batch():
+    mols = [molecules.water(), molecules.alkane(5), molecules.water_cluster(3, seed=4)]
+    bases = [AOBasis.load(m, "def2-svp", pure=True) for m in mols]
+    grids = [_grid(m, 21, 50) for m in mols]
+    Ds = [random_density(b.nbf, 8, 120 + i) for i, b in enumerate(bases)]
+    refs = [oracle.xc_build(oracle.OracleBasis.from_flat(b.flat()), p, w, IDS, D)
+            for b, (p, w), D in zip(bases, grids, Ds)]
+    first = None
+    for tile, pers in SETTINGS:
+        batch = XCBatch(bases, grids, IDS, tile_ao=tile, persistent=pers)
+        Vs, excs, nels = batch.build(Ds)
+        batch.close()
+        for i, (K, exc, nel) in enumerate(refs):
+            assert abs(excs[i] - exc) <= 1e-6 and abs(nels[i] - nel) <= 1e-8
+            assert np.abs(Vs[i] - K).max() <= 1e-7
+        if first is None:
+            first = (Vs, excs, nels)
+        for i in range(len(mols)):
+            assert np.abs(Vs[i] - first[0][i]).max() <= 1e-12, (tile, pers, i)
+            assert abs(excs[i] - first[1][i]) <= 1e-12
