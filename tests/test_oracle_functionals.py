@@ -12,8 +12,10 @@ What pins what:
   spin polarised (E_x[n, 0] = E_x^unpol[2n] / 2): LSDA -0.2680 (closed form),
   B88 -0.3098 (Becke, PRA 38, 3098 (1988), Table I), PBE -0.3059 (Perdew,
   Burke, Ernzerhof, PRL 77, 3865 (1996));
-- VWN5, PW92 and LYP energies are otherwise pinned only by their papers'
-  closed forms restated in the oracle ("parity unpinned" against libxc).
+- the correlation functionals (VWN5, PW92, PBE-c, LYP) by their papers'
+  limits and identities in tests/test_oracle_functional_pins.py, and, on a
+  molecule with OCC's default grid, by the reference's whole-path known
+  answers (tests/isolated_xc_tests.cpp) in tests/test_oracle_known_answers.py.
 """
 import ctypes
 import pathlib

@@ -13,8 +13,10 @@ src/dft/xc_potential_matrix.cpp:61-109).
 - complex-step derivatives against central differences;
 - dExc/dD_s(Delta) = 2 tr(V_s Delta) for each spin on an open-shell density,
   and screened == dense.
-libxc's polarised forms themselves are absent here ("parity unpinned" at the
-libxc boundary, as for the unpolarised functionals)."""
+libxc itself is absent here: between the closed-shell and fully polarised
+limits the spin interpolations (VWN5 / PW92 f(zeta), PBE's phi(zeta), the
+general LYP form) are pinned by the limits above, the PW92 spin stiffness and
+the derivative checks, not by values computed with libxc."""
 import numpy as np
 import pytest
 
