@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -290,6 +291,35 @@ __device__ double block_max_256(double v, double *scratch) {
   return r;
 }
 
+// FP64 -> FP32 with unbiased rounding: v goes to one of its two neighbouring
+// floats, chosen with the probabilities that make the expected stored value
+// v itself (stochastic rounding).  On an atom-centred grid an s-type AO has
+// the same value at all angular points of a radial shell, so rounding to
+// nearest repeats one error there and those errors add up coherently in Exc;
+// unbiased rounding makes them independent, and they average out.  The
+// random bits are a hash of the point's coordinates, the global AO index and
+// the component, so the stored value depends only on what it represents:
+// evaluations are reproducible and a molecule's AO values are the same in a
+// single-molecule context and in a batch.
+__device__ __forceinline__ float round_unbiased(double v, unsigned long long key) {
+  const float f = (float)v;
+  const double r = v - (double)f;  // exact, |r| <= half an ulp of f
+  if (r == 0.0 || !isfinite(f)) return f;
+  const float g = nextafterf(f, r > 0.0 ? INFINITY : -INFINITY);
+  const double prob = r / ((double)g - (double)f);  // in (0, 1/2]
+  unsigned long long h = key + 0x9E3779B97F4A7C15ull;  // splitmix64 finaliser
+  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  const double u = (double)(h >> 11) * 0x1.0p-53;
+  return u < prob ? g : f;
+}
+
+template <typename T> __device__ __forceinline__ T store_ao(double v, unsigned long long key) {
+  if constexpr (std::is_same<T, float>::value) return round_unbiased(v, key);
+  else return static_cast<T>(v);
+}
+
 // ------------------------------------------------------------------ AO --
 // One workgroup per active leaf, one thread per (padded) point; for every
 // significant shell each thread writes its components.  The AO block of a
@@ -307,12 +337,21 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
   const bool deriv = a.deriv != 0;
   T *base = ao + L.ao_off;
   const long long cstride = (long long)L.mp * L.np;
+  // the point's part of the unbiased-rounding key (FP32 storage)
+  const unsigned long long pkey = (unsigned long long)__double_as_longlong(x) ^
+                                  ((unsigned long long)__double_as_longlong(y) << 21 |
+                                   (unsigned long long)__double_as_longlong(y) >> 43) ^
+                                  ((unsigned long long)__double_as_longlong(z) << 42 |
+                                   (unsigned long long)__double_as_longlong(z) >> 22);
   int mu = 0;
   for (int k = 0; k < L.nsig; ++k) {
     const ShellDev sh = a.shells[a.leaf_shells[L.sh_off + k]];
     T *row = base + (long long)mu * L.np + p;
+    const unsigned long long key0 = pkey + 0x2545F4914F6CDD1Dull * (unsigned long long)sh.bf0;
     auto sink = [&](int q, int m, double v) {
-      row[q * cstride + (long long)m * L.np] = static_cast<T>(v);
+      const long long off = q * cstride + (long long)m * L.np;
+      row[off] = store_ao<T>(v, key0 + 0x2545F4914F6CDD1Dull * (unsigned long long)m +
+                                    0x9E3779B97F4A7C15ull * (unsigned long long)q);
     };
     const double dx = x - sh.x, dy = y - sh.y, dz = z - sh.z;
     if (sh.l == 0) shell_point<0>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
