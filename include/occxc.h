@@ -65,7 +65,8 @@ enum {
 
 typedef enum {
   OCCXC_PREC_FP64 = 0,       /* FP64 AO cache and contractions */
-  OCCXC_PREC_FP32 = 1,       /* FP32 AO cache, FP32 contractions, FP32 accumulation */
+  OCCXC_PREC_FP32 = 1,       /* FP32 AO cache; density contraction in FP64; Vxc contraction in
+                                FP32 with FP32 accumulation over 16 points, FP64 beyond */
   OCCXC_PREC_BF16X3 = 2,     /* FP32 AO cache, 3-term split-BF16 MFMA, FP32 accumulation */
   OCCXC_PREC_FP32_CACHE = 3  /* FP32 AO cache, FP64 density matrix, FP64 contractions */
 } occxc_precision;

@@ -1391,6 +1391,15 @@ __global__ __launch_bounds__(256) void k_vxc_reduce_final(BuildArgs a, double *_
 // wavefront works on a 2D block of up to 4 x 4 16x16 MFMA tiles (FP64
 // arithmetic; S is the AO storage type).
 
+// Vxc scatter screening: a block entry below kScatterCut in magnitude is not
+// added into K.  An entry of Vxc = K + K^T gathers at most two terms per
+// leaf, so the change of any Vxc entry is below 2 kScatterCut times the
+// number of leaves (under 2e-9 for 1e5 leaves, against the 1e-7 tolerance);
+// Exc and N_elec are not affected.  About half the block entries are that small (AO pairs
+// whose product is negligible over the leaf), and the FP64 atomics they save
+// are the largest part of the Vxc stage.
+constexpr double kScatterCut = 1e-14;
+
 constexpr int kTKc = 16;         // AOs per density k-chunk
 constexpr int kTKld = kTKc + 4;  // LDS row stride (doubles) of the [row][k] density operands
 constexpr int kTVp = 16;         // points per Vxc chunk
@@ -1725,7 +1734,129 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
-        if (mu < L.m) atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+        if (mu < L.m && fabs(acc[i][j][r]) >= kScatterCut)
+                atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+      }
+    }
+  }
+}
+
+// FP32 form of the Vxc item (FP32 mode): the same block pairs with Phi and
+// A rounded to FP32 in LDS and K_loc accumulated over the item's points on
+// v_mfma_f32_16x16x4f32 (accumulator register r of lane l holds row
+// 4 (l/16) + r), then added into the FP64 K with FP64 atomics, so the sum
+// across leaves stays FP64.  Half the LDS and accumulator registers of the
+// FP64 form: three workgroups per CU.
+constexpr int kTVldf = kTVp + 4;  // floats per LDS row (80 bytes)
+
+template <bool GGA>
+__global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const float *__restrict__ ao,
+                                                         const VxcItem *__restrict__ items) {
+  __shared__ float s_pr[kTileAo * kTVldf], s_ar[kTileAo * kTVldf];
+  __shared__ float s_pc[kTileAo * kTVldf], s_ac[kTileAo * kTVldf];
+  __shared__ double s_c[2][4][kTVp];
+  const VxcItem it = items[blockIdx.x];
+  if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
+  const LeafDev L = a.leaves[it.leaf];
+  const MolDev MD = a.mols[L.mol];
+  double *Km = a.K + MD.moff;
+  const int *map = a.ao_map + L.map_off;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wr = wave & 1, wcw = wave >> 1;
+  const bool diag = it.r0 == it.c0;
+  const int hr = it.wr >> 1, hc = it.wc >> 1;
+  const int RT = hr >> 4, CT = hc >> 4;
+  const float *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const int nrow = diag ? it.wr : it.wr + it.wc;
+  const double *cf = a.coef + L.pt_off;
+  constexpr int NQ = GGA ? 4 : 1;
+  // FP32 MFMA accumulation over one 16-point chunk at a time, added into
+  // FP64 accumulators after each chunk: the FP32 sums stay 16 terms long
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  if (t < NQ * kTVp) s_c[0][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + t % kTVp];
+  int buf = 0;
+  for (int p0 = 0; p0 < L.np; p0 += kTVp, buf ^= 1) {
+    __syncthreads();
+    if (p0 + kTVp < L.np && t < NQ * kTVp)
+      s_c[buf ^ 1][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + p0 + kTVp + t % kTVp];
+    for (int gi = t; gi < nrow * (kTVp / 4); gi += 256) {
+      const int row = gi >> 2, p = 4 * (gi & 3);
+      const bool rb = row < it.wr;
+      const int mu = rb ? it.r0 + row : it.c0 + row - it.wr;
+      const long long o = (long long)mu * L.np + p0 + p;
+      const f4 f = *reinterpret_cast<const f4 *>(phi + o);
+      f4 fx, fy, fz;
+      if constexpr (GGA) {
+        fx = *reinterpret_cast<const f4 *>(phi + cs + o);
+        fy = *reinterpret_cast<const f4 *>(phi + 2 * cs + o);
+        fz = *reinterpret_cast<const f4 *>(phi + 3 * cs + o);
+      }
+      f4 A;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double v = s_c[buf][0][p + e] * (double)f[e];
+        if constexpr (GGA)
+          v += s_c[buf][1][p + e] * (double)fx[e] + s_c[buf][2][p + e] * (double)fy[e] +
+               s_c[buf][3][p + e] * (double)fz[e];
+        A[e] = (float)v;
+      }
+      const int lr = rb ? row : row - it.wr;
+      *reinterpret_cast<f4 *>((rb ? s_pr : s_pc) + lr * kTVldf + p) = f;
+      *reinterpret_cast<f4 *>((rb ? s_ar : s_ac) + lr * kTVldf + p) = A;
+    }
+    __syncthreads();
+    f4 acf[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acf[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    auto product = [&](const float *xa, const float *yb) {
+      f4 xv[4], yv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (i < RT) xv[i] = *reinterpret_cast<const f4 *>(xa + (wr * hr + 16 * i + l16) * kTVldf + 4 * g);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j < CT) yv[j] = *reinterpret_cast<const f4 *>(yb + (wcw * hc + 16 * j + l16) * kTVldf + 4 * g);
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (i < RT && j < CT)
+              acf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[i][s4], yv[j][s4], acf[i][j], 0, 0, 0);
+    };
+    product(s_pr, diag ? s_ar : s_ac);
+    if (!diag) product(s_ar, s_pc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)acf[i][j][r];
+  }
+  // acc[i][j][r] = (mu = r0 + wr*hr + 16 i + 4 g + r, nu = c0 + wcw*hc + 16 j + l16)
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j >= CT) break;
+    const int nu = it.c0 + wcw * hc + 16 * j + l16;
+    if (nu >= L.m) continue;
+    double *kc = Km + map[nu];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= RT) break;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mu = it.r0 + wr * hr + 16 * i + 4 * g + r;
+        if (mu < L.m && fabs(acc[i][j][r]) >= kScatterCut)
+          atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
       }
     }
   }
@@ -1904,7 +2035,8 @@ __global__ __launch_bounds__(512, 1) void k_vxc_ws(BuildArgs a, const S *__restr
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
-              if (mu < L.m) atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+              if (mu < L.m && fabs(acc[i][j][r]) >= kScatterCut)
+                atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
             }
           }
         }
@@ -2073,7 +2205,7 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   p.vxc_variant = vxc_variant_setting();
   p.two_pass = p.vxc_variant == 2 && precision != 2;  // not with the split-BF16 Vxc
   const char *tv = std::getenv("OCCXC_TILED");
-  p.tiled = p.use_mfma && (precision == 0 || precision == 3) && !p.two_pass &&
+  p.tiled = p.use_mfma && (precision == 0 || precision == 1 || precision == 3) && !p.two_pass &&
             !(tv && std::atoi(tv) == 0);
   if (p.tiled) {
     make_tile_plan(p, leaves);
@@ -2090,7 +2222,7 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
       OCCXC_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)vxc_ws_smem()));
   }
-  if (precision == 1) {
+  if (precision == 1 && !p.tiled) {
     p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
     p.d2l.alloc((size_t)std::max<long long>(mtot, 1));
     p.bytes += p.d2f.bytes() + p.d2l.bytes();
@@ -2280,6 +2412,13 @@ void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int p
   check_coef_planes(a.deriv ? 4 : 1, a.coef_planes, "Vxc stage");
   if (a.nleaves == 0) return;
   if (plan.tiled) {
+    if (precision == 1) {
+      // FP32 mode: FP32 Vxc contraction (the persistent form is FP64 only)
+      (a.deriv ? k_vxc_tile_f32<true> : k_vxc_tile_f32<false>)
+          <<<plan.n_vxc_items, 256, 0, s>>>(a, static_cast<const float *>(ao), plan.vxc_items.get());
+      check_launch("k_vxc_tile_f32");
+      return;
+    }
     auto launch = [&](auto s_tag) {
       using S = decltype(s_tag);
       if (plan.vxc_ws) {
