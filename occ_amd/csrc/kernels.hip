@@ -304,15 +304,21 @@ __device__ double block_max_256(double v, double *scratch) {
 __device__ __forceinline__ float round_unbiased(double v, unsigned long long key) {
   const float f = (float)v;
   const double r = v - (double)f;  // exact, |r| <= half an ulp of f
-  if (r == 0.0 || !isfinite(f)) return f;
-  const float g = nextafterf(f, r > 0.0 ? INFINITY : -INFINITY);
-  const double prob = r / ((double)g - (double)f);  // in (0, 1/2]
-  unsigned long long h = key + 0x9E3779B97F4A7C15ull;  // splitmix64 finaliser
-  h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
-  h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
-  h ^= h >> 31;
-  const double u = (double)(h >> 11) * 0x1.0p-53;
-  return u < prob ? g : f;
+  if (r == 0.0) return f;
+  // g: the float next to f on v's side (integer step of the bit pattern)
+  const unsigned int b = __float_as_uint(f);
+  const bool away = (r > 0.0) == (f > 0.0f);
+  const unsigned int gb = f == 0.0f ? (r > 0.0 ? 1u : 0x80000001u) : (away ? b + 1u : b - 1u);
+  const float g = __uint_as_float(gb);
+  const float ulp = fabsf(g - f);  // exact
+  unsigned int h = (unsigned int)(key ^ (key >> 32)) * 0x9E3779B9u;  // murmur3 finaliser
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  const float u = (float)(h >> 8) * 0x1.0p-24f;  // uniform on [0, 1)
+  return u * ulp < (float)fabs(r) ? g : f;
 }
 
 template <typename T> __device__ __forceinline__ T store_ao(double v, unsigned long long key) {
@@ -331,8 +337,14 @@ template <typename T> __device__ __forceinline__ T store_ao(double v, unsigned l
 template <typename T, int LMAX>
 __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao) {
   const LeafDev L = a.leaves[blockIdx.x];
-  const int p = threadIdx.x;
-  if (p >= L.np) return;
+  // lanes over (point, shell split): a leaf of np points takes 256 / np
+  // splits of its shells (np = 64: 4, 128: 2, 192 and 256: 1), split s taking
+  // shells s, s + nsplit, ...; every wavefront lies in one split, so the
+  // split test is uniform and leaves of fewer than 256 points keep their lanes
+  const int nsplit = L.np <= 64 ? 4 : (L.np <= 128 ? 2 : 1);
+  const int split = threadIdx.x / L.np;
+  if (split >= nsplit) return;
+  const int p = threadIdx.x - split * L.np;
   const double x = a.px[L.pt_off + p], y = a.py[L.pt_off + p], z = a.pz[L.pt_off + p];
   const bool deriv = a.deriv != 0;
   T *base = ao + L.ao_off;
@@ -346,6 +358,10 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
   int mu = 0;
   for (int k = 0; k < L.nsig; ++k) {
     const ShellDev sh = a.shells[a.leaf_shells[L.sh_off + k]];
+    if (k % nsplit != split) {
+      mu += sh.size;
+      continue;
+    }
     T *row = base + (long long)mu * L.np + p;
     const unsigned long long key0 = pkey + 0x2545F4914F6CDD1Dull * (unsigned long long)sh.bf0;
     auto sink = [&](int q, int m, double v) {
@@ -362,7 +378,7 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
     else shell_point<LMAX>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
     mu += sh.size;
   }
-  for (int r = L.m; r < L.mp; ++r)
+  for (int r = L.m + split; r < L.mp; r += nsplit)
     for (int c = 0; c < a.ncomp; ++c)
       base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
 }
