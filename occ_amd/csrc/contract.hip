@@ -1899,6 +1899,18 @@ __device__ __forceinline__ void vxc_next(const BuildArgs &a, const VxcItem *item
 }
 
 constexpr int kWsBuf = 4 * kTileAo * kTVld;  // doubles per staging buffer
+
+// Workgroup barrier for LDS hand-offs only: waits for this wavefront's LDS
+// accesses (lgkmcnt) but not for its outstanding global memory operations,
+// so the MFMA wavefronts' scatter atomics drain while the next steps run
+// (__syncthreads would wait for them at every step).  The empty asm
+// statements keep the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt and expcnt: no wait; lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 static size_t vxc_ws_smem() { return sizeof(double) * 2 * kWsBuf; }
 
 template <typename S, bool GGA>
@@ -1960,20 +1972,20 @@ __global__ __launch_bounds__(512, 1) void k_vxc_ws(BuildArgs a, const S *__restr
   // barrier per step, the producers one step ahead
   if (producer) {
     stage(cc, smem);
-    __syncthreads();
+    lds_barrier();
     int b = 0;
     while (true) {
       VxcCursor nx = cc;
       vxc_next(a, items, n, nx);
       if (nx.ii < n) stage(nx, smem + (b ^ 1) * kWsBuf);
-      __syncthreads();
+      lds_barrier();
       if (nx.ii >= n) break;
       cc = nx;
       b ^= 1;
     }
     return;
   }
-  __syncthreads();
+  lds_barrier();
   const int wr = wave & 1, wcw = (wave >> 1) & 1;
   d4 acc[4][4];
   int b = 0;
@@ -2042,7 +2054,7 @@ __global__ __launch_bounds__(512, 1) void k_vxc_ws(BuildArgs a, const S *__restr
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (nx.ii >= n) break;
     cc = nx;
     b ^= 1;
