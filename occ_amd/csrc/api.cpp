@@ -328,6 +328,8 @@ static void setup_leaves_device(occxc_context *c, int nmol, const occxc_grid *gr
       L.mp = (int)round_up(L.m, kAoPad);
       L.ao_off = ao_off;
       ao_off += (long long)c->ncomp * L.mp * L.np;
+      L.res_off = (long long)c->res_elems;
+      c->res_elems += (size_t)L.mp * L.np;
       L.kl_off = c->kloc_elems;
       c->kloc_elems += (long long)L.mp * L.mp;
       sh_total += L.nsig;
@@ -336,7 +338,8 @@ static void setup_leaves_device(occxc_context *c, int nmol, const occxc_grid *gr
       msum += L.m;
       c->flops += 4.0 * L.n * (double)L.m * L.m;
       c->ao_write_bytes +=
-          (double)L.n * L.m * c->ncomp * (c->opt.precision == 0 ? 8.0 : 4.0) + 32.0 * L.n;
+          (double)L.n * L.m * (c->opt.precision == 0 ? 8.0 * c->ncomp : 4.0 * c->ncomp + 4.0) +
+          32.0 * L.n;
       c->leaves.push_back(L);
       fill.push_back(FillLeaf{g, shell_base[m], hbs[m].nsh, 0, pbase[m] + 128LL * li, pbase[m]});
     }
@@ -560,6 +563,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
       for (int q = m; q < L.mp; ++q) ao_map.push_back(0);
       L.ao_off = ao_off;
       ao_off += (long long)c->ncomp * L.mp * L.np;
+      L.res_off = (long long)c->res_elems;
+      c->res_elems += (size_t)L.mp * L.np;
       L.kl_off = c->kloc_elems;
       c->kloc_elems += (long long)L.mp * L.mp;
       const long long o = h.off[li];
@@ -575,7 +580,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
       msum += m;
       c->flops += 4.0 * L.n * (double)m * m;
       c->ao_write_bytes +=
-          (double)L.n * m * c->ncomp * (opt->precision == 0 ? 8.0 : 4.0) + 32.0 * L.n;
+          (double)L.n * m * (opt->precision == 0 ? 8.0 * c->ncomp : 4.0 * c->ncomp + 4.0) +
+          32.0 * L.n;
       c->leaves.push_back(L);
     }
     mols[mol].leaf_end = (int)c->leaves.size();
@@ -606,12 +612,17 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   // device layout
   c->d_mols.alloc(mols.size());
   c->d_mols.upload(mols.data(), mols.size(), c->stream);
+  // FP32 storage: the value plane's rounding residuals exist during set-up
+  // only; the context keeps dS (nbf^2 per molecule) for the N_elec correction
+  if (opt->precision == 0) c->res_elems = 0;
   const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
+  const size_t ds_bytes = c->res_elems ? (size_t)c->mtot * 8 : 0;
   // device memory the context will hold: AO values, the nbf^2 matrices
   // (D, D2, K, V and the FP32 copies), per-point data and the leaf tables
   const bool two_pass = vxc_variant_setting() == 2;
   c->mol_bytes.assign(nmol, 0.0);
-  for (int m = 0; m < nmol; ++m) c->mol_bytes[m] = 6.0 * 8.0 * c->mol_nbf[m] * c->mol_nbf[m];
+  for (int m = 0; m < nmol; ++m)
+    c->mol_bytes[m] = (c->res_elems ? 7.0 : 6.0) * 8.0 * c->mol_nbf[m] * c->mol_nbf[m];
   for (const LeafDev &L : c->leaves)
     c->mol_bytes[L.mol] += (double)c->ncomp * L.mp * L.np * c->ao_elem_size + 64.0 * L.np +
                            4.0 * (L.mp + L.nsig) + sizeof(LeafDev) +
@@ -619,7 +630,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   double tile_bytes = 0.0;
   if (!two_pass)
     for (const LeafDev &L : c->leaves) tile_bytes += tile_plan_bytes(L);
-  const size_t planned = ao_bytes + (size_t)tile_bytes + (size_t)c->mtot * 8 * 6 + (size_t)c->ptot * 8 * 8 +
+  const size_t planned = ao_bytes + ds_bytes + (size_t)tile_bytes + (size_t)c->mtot * 8 * 6 +
+                         (size_t)c->ptot * 8 * 8 +
                          (c->mgga ? (size_t)c->ptot * 8 * 8 : 0) +
                          c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                          (two_pass ? (size_t)c->kloc_elems * 8 : 0);
@@ -629,10 +641,12 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                     std::to_string(byte_limit >> 20) + " MiB");
   size_t free_b = 0, total_b = 0;
   OCCXC_HIP(hipMemGetInfo(&free_b, &total_b));
-  OCCXC_REQUIRE(ao_bytes + (size_t)6 * c->mtot * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
-                "resident AO values need " + std::to_string(ao_bytes >> 20) + " MiB, " +
+  const size_t setup_bytes = ao_bytes + ds_bytes + c->res_elems * sizeof(float);
+  OCCXC_REQUIRE(setup_bytes + (size_t)6 * c->mtot * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
+                "resident AO values need " + std::to_string(setup_bytes >> 20) + " MiB, " +
                     std::to_string(free_b >> 20) + " MiB free");
   c->d_ao.alloc(ao_bytes);
+  if (c->res_elems) c->d_ao_res.alloc(c->res_elems);
   const size_t nn = (size_t)c->mtot;
   c->d_D.alloc(nn);
   c->d_D2.alloc(nn);
@@ -674,6 +688,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.coef_tau_planes = 0;
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
+  a.ao_res = c->res_elems ? c->d_ao_res.get() : nullptr;  // set-up only
 
   c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream,
                                tune_choice(opt, c->maxnbf));
@@ -740,7 +755,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     a.ntchunks = (int)tchunks.size();
     a.tpartial = c->d_tpartial.get();
   }
-  c->device_bytes = ao_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
+  c->device_bytes = ao_bytes + ds_bytes + c->d_D.bytes() * 4 + c->d_coef.bytes() + c->d_px.bytes() * 4 +
                     c->d_porig.bytes() +
                     c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
                     c->plan.bytes + c->d_kloc.bytes() + c->d_tpairs.bytes() +
@@ -770,6 +785,18 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   }
 
   launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
+  if (c->res_elems) {
+    // dS from the residuals (the Vxc item list drives the contraction),
+    // then the residuals are released
+    c->d_dS.alloc((size_t)c->mtot);
+    c->d_ncorr.alloc((size_t)kNcorrBlocks * nmol);
+    run_storage_overlap(c->plan, a, c->d_ao.get(), c->d_K.get(), c->d_dS.get(), c->d_active.get(),
+                        c->d_mols.get(), c->nmol, c->maxnbf, c->stream);
+    c->device_bytes += c->d_ncorr.bytes();
+    OCCXC_HIP(hipStreamSynchronize(c->stream));
+    c->d_ao_res.release();
+    a.ao_res = nullptr;
+  }
   OCCXC_HIP(hipStreamSynchronize(c->stream));
   c->ao_ready = true;
 }
@@ -842,6 +869,14 @@ static void mgga_density(occxc_context *c, hipStream_t s) {
   launch_functional_mgga_leaves(a, c->d_mrho.get(), c->d_mtau.get(), c->d_coef_tau.get(), s);
 }
 
+// FP32 AO storage: N_elec += sum D2 (.) dS (run_storage_overlap), the
+// first-order rounding change of sum_p w_p rho_p; D2 is the build's
+static void storage_correction(occxc_context *c, double *d_result, hipStream_t s) {
+  if (c->d_dS.get() == nullptr) return;
+  launch_storage_correction(c->d_D2.get(), c->d_dS.get(), c->d_mols.get(), c->nmol, c->maxnbf,
+                            c->d_ncorr.get(), d_result, s);
+}
+
 static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, double *d_result,
                            hipStream_t s) {
   const BuildArgs &a = c->args;
@@ -861,6 +896,7 @@ static void run_build_mgga(occxc_context *c, const double *d_D, double *d_V, dou
   t.run(OCCXC_STAGE_FINALIZE, [&] {
     launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
                     d_result, s);
+    storage_correction(c, d_result, s);
   });
 }
 
@@ -892,6 +928,7 @@ void run_build(occxc_context *c, const double *d_D, double *d_V, double *d_resul
       launch_finalize(c->d_K.get(), d_V, c->d_mols.get(), c->nmol, c->maxnbf, c->d_part.get(),
                       d_result, s);
     }
+    storage_correction(c, d_result, s);
   });
 }
 

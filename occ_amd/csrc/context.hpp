@@ -27,7 +27,7 @@ struct occxc_context {
   long long npts = 0, ptot = 0;
   int nleaves = 0;
   double mean_m = 0.0, flops = 0.0, ao_write_bytes = 0.0;
-  size_t ao_elems = 0, ao_elem_size = 8;
+  size_t ao_elems = 0, ao_elem_size = 8, res_elems = 0;
   std::vector<occxc::LeafDev> leaves;
   occxc::DevBuf<occxc::ShellDev> d_shells;
   occxc::DevBuf<double> d_exps, d_coefs;
@@ -37,6 +37,11 @@ struct occxc_context {
   occxc::DevBuf<double> d_px, d_py, d_pz, d_pw;
   occxc::DevBuf<int> d_porig;  // input index (within its molecule) of each padded point, -1 for padding
   occxc::DevBuf<char> d_ao;
+  // FP32 AO storage: the value plane's rounding residuals (set-up only) and
+  // dS = sum_p w_p (r Phi^T + Phi r^T), the first-order change of the grid
+  // overlap from the stored values to the exact ones
+  occxc::DevBuf<float> d_ao_res;
+  occxc::DevBuf<double> d_dS, d_ncorr;
   occxc::DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
   occxc::ContractPlan plan;
   // two-pass Vxc: per-leaf K_loc blocks and the tile-pair leaf lists

@@ -1405,6 +1405,61 @@ constexpr int kTKld = kTKc + 4;  // LDS row stride (doubles) of the [row][k] den
 constexpr int kTVp = 16;         // points per Vxc chunk
 constexpr int kTVld = kTVp + 2;  // LDS row stride (doubles) of the [row][point] Vxc operands
 
+// Tile shapes of the MFMA k-steps.  A wavefront owns RT x CT 16x16 tiles
+// (RT, CT <= 4).  RT and CT are compile-time constants, so a step's LDS
+// operand reads are issued together ahead of its MFMAs (with run-time tile
+// counts every MFMA sits in its own branch behind a wait for its operand).
+// The shapes items mostly have (blocks of 96 or 128 rows: 3 or 4 tiles each
+// way) get their own code; the others run the 4 x 4 form, whose extra tiles
+// read LDS rows that are in the arrays (128 rows) but not staged for the
+// item: those tiles' accumulators are never used, and a tile (i, j) only
+// ever combines row tile i with column tile j.
+template <int N> using IntC = std::integral_constant<int, N>;
+
+template <typename F>
+__device__ __forceinline__ void tile_shape(int rt, int ct, F &&f) {
+  if (rt == 4 && ct == 3) f(IntC<4>{}, IntC<3>{});
+  else if (rt == 3 && ct == 4) f(IntC<3>{}, IntC<4>{});
+  else if (rt == 3 && ct == 3) f(IntC<3>{}, IntC<3>{});
+  else f(IntC<4>{}, IntC<4>{});
+}
+
+// Two FP64 k-steps of acc[i][j] += X_i^T Y_j.  x and y point at this lane's
+// pair of operands in row tile 0 and column tile 0 ([row][k] layout, LD
+// doubles per row: tile i starts 16 i rows further).
+template <int R, int C, int LD>
+__device__ __forceinline__ void mfma_f64_k2(d4 (&acc)[4][4], const double *x, const double *y) {
+  d2u xv[R], yv[C];
+#pragma unroll
+  for (int i = 0; i < R; ++i) xv[i] = *reinterpret_cast<const d2u *>(x + 16 * i * LD);
+#pragma unroll
+  for (int j = 0; j < C; ++j) yv[j] = *reinterpret_cast<const d2u *>(y + 16 * j * LD);
+#pragma unroll
+  for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i][s2], yv[j][s2], acc[i][j], 0, 0, 0);
+}
+
+// Four FP32 k-steps of the same product (v_mfma_f32_16x16x4f32).
+template <int R, int C, int LD>
+__device__ __forceinline__ void mfma_f32_k4(f4 (&acc)[4][4], const float *x, const float *y) {
+  f4 xv[R], yv[C];
+#pragma unroll
+  for (int i = 0; i < R; ++i) xv[i] = *reinterpret_cast<const f4 *>(x + 16 * i * LD);
+#pragma unroll
+  for (int j = 0; j < C; ++j) yv[j] = *reinterpret_cast<const f4 *>(y + 16 * j * LD);
+#pragma unroll
+  for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < C; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[i][s4], yv[j][s4], acc[i][j], 0, 0, 0);
+}
+
 // Density item: T(p, nu) = sum_mu Phi(p, mu) D2(mu, nu) for the item's points p
 // and columns nu, over all of the leaf's AOs mu in k-chunks staged through LDS
 // (Phi transposed to [point][mu], D2 gathered through the AO map to
@@ -1483,30 +1538,22 @@ __device__ __forceinline__ void den_tile_item(const BuildArgs &a, const S *__res
   };
   __syncthreads();  // s_map
   fetch(0);
-  for (int k0 = 0; k0 < L.mp; k0 += kTKc) {
-    __syncthreads();  // the previous chunk's operands are consumed
-    store();
-    __syncthreads();
-    if (k0 + kTKc < L.mp) fetch(k0 + kTKc);
+  const double *xa = s_phi + (wp * hp + pa) * kTKld + 4 * g;
+  const double *yb = s_d + (wc * hc + l16) * kTKld + 4 * g;
+  // one k loop per tile shape (a shape test inside the loop puts every
+  // shape's accumulators through the merge points and spills)
+  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+    for (int k0 = 0; k0 < L.mp; k0 += kTKc) {
+      __syncthreads();  // the previous chunk's operands are consumed
+      store();
+      __syncthreads();
+      if (k0 + kTKc < L.mp) fetch(k0 + kTKc);
 #pragma unroll
-    for (int sp = 0; sp < kTKc / 4; sp += 2) {
-      d2u av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (i < RT) av[i] = *reinterpret_cast<const d2u *>(s_phi + (wp * hp + 16 * i + pa) * kTKld + 4 * g + sp);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j < CT) bv[j] = *reinterpret_cast<const d2u *>(s_d + (wc * hc + 16 * j + l16) * kTKld + 4 * g + sp);
-#pragma unroll
-      for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (i < RT && j < CT)
-              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i][s2], bv[j][s2], acc[i][j], 0, 0, 0);
+      for (int sp = 0; sp < kTKc / 4; sp += 2)
+        mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTKld>(
+            acc, xa + sp, yb + sp);
     }
-  }
+  });
   __syncthreads();  // s_phi is reused for the sums below
   // acc[i][j][r] = T(point wp*hp + 16 i + 4 g + r, column wc*hc + 16 j + l16)
 #pragma unroll
@@ -1639,7 +1686,10 @@ __global__ __launch_bounds__(256) void k_den_finish(BuildArgs a, const DenFinish
 //   r <  c:  V_rc = Phi_r^T A_c + A_r^T Phi_c
 // in registers.  Entry (mu, nu) is added to K(map[nu], map[mu]) with FP64
 // atomics; Vxc = K + K^T afterwards counts every entry of V_loc once.
-template <typename S, bool GGA>
+// RES (FP32 storage, context set-up): A = w r with r the value plane's
+// rounding residuals, so K + K^T = dS = sum_p w_p (r Phi^T + Phi r^T); no
+// entry is screened out.
+template <typename S, bool GGA, bool RES = false>
 __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__restrict__ ao,
                                                      const VxcItem *__restrict__ items) {
   __shared__ double s_pr[kTileAo * kTVld], s_ar[kTileAo * kTVld];
@@ -1660,7 +1710,8 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
   const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
   const int nrow = diag ? it.wr : it.wr + it.wc;
-  const double *cf = a.coef + L.pt_off;
+  static_assert(!RES || (!GGA && std::is_same<S, float>::value), "dS: FP32 storage, LDA form");
+  const double *cf = RES ? a.pw + L.pt_off : a.coef + L.pt_off;
   constexpr int NQ = GGA ? 4 : 1;
   d4 acc[4][4];
 #pragma unroll
@@ -1668,6 +1719,9 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   if (t < NQ * kTVp) s_c[0][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + t % kTVp];
+  const int xo = (wr * hr + l16) * kTVld + 4 * g, yo = (wcw * hc + l16) * kTVld + 4 * g;
+  // one point loop per tile shape
+  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
   int buf = 0;
   for (int p0 = 0; p0 < L.np; p0 += kTVp, buf ^= 1) {
     __syncthreads();
@@ -1681,8 +1735,14 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
       const long long o = (long long)mu * L.np + p0 + p;
       const d4 f = load4_as<double>(phi + o);
       d4 A;
+      if constexpr (RES) {
+        const d4 r = load4_as<double>(a.ao_res + L.res_off + o);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) A[e] = s_c[buf][0][p + e] * f[e];
+        for (int e = 0; e < 4; ++e) A[e] = s_c[buf][0][p + e] * r[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) A[e] = s_c[buf][0][p + e] * f[e];
+      }
       if constexpr (GGA) {
         const d4 fx = load4_as<double>(phi + cs + o);
         const d4 fy = load4_as<double>(phi + 2 * cs + o);
@@ -1698,29 +1758,16 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
     __syncthreads();
     // two k-steps (points) per operand load keeps the operands at 32
     // registers next to the 128 of the accumulators
-    auto product = [&](const double *xa, const double *yb) {
+    auto product = [&](const double *xa, const double *yb) __attribute__((always_inline)) {
 #pragma unroll
-      for (int sp = 0; sp < 4; sp += 2) {
-        d2u xv[4], yv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (i < RT) xv[i] = *reinterpret_cast<const d2u *>(xa + (wr * hr + 16 * i + l16) * kTVld + 4 * g + sp);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (j < CT) yv[j] = *reinterpret_cast<const d2u *>(yb + (wcw * hc + 16 * j + l16) * kTVld + 4 * g + sp);
-#pragma unroll
-        for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-              if (i < RT && j < CT)
-                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i][s2], yv[j][s2], acc[i][j], 0, 0, 0);
-      }
+      for (int sp = 0; sp < 4; sp += 2)
+        mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTVld>(
+            acc, xa + xo + sp, yb + yo + sp);
     };
     product(s_pr, diag ? s_ar : s_ac);
     if (!diag) product(s_ar, s_pc);
   }
+  });
   // acc[i][j][r] = (mu = r0 + wr*hr + 16 i + g + 4 r, nu = c0 + wcw*hc + 16 j + l16)
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -1734,8 +1781,8 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
-        if (mu < L.m && fabs(acc[i][j][r]) >= kScatterCut)
-                atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+        if (mu < L.m && (RES || fabs(acc[i][j][r]) >= kScatterCut))
+          atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
       }
     }
   }
@@ -1745,16 +1792,17 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
 // A rounded to FP32 in LDS and K_loc accumulated over the item's points on
 // v_mfma_f32_16x16x4f32 (accumulator register r of lane l holds row
 // 4 (l/16) + r), then added into the FP64 K with FP64 atomics, so the sum
-// across leaves stays FP64.  Half the LDS and accumulator registers of the
-// FP64 form: three workgroups per CU.
-constexpr int kTVldf = kTVp + 4;  // floats per LDS row (80 bytes)
+// across leaves stays FP64.  Chunks of 32 points (half the FP64 form's LDS
+// per point): half as many barriers and FP64 accumulations per point.
+constexpr int kTVpf = 32;          // points per FP32 Vxc chunk
+constexpr int kTVldf = kTVpf + 4;  // floats per LDS row (144 bytes)
 
 template <bool GGA>
 __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const float *__restrict__ ao,
                                                          const VxcItem *__restrict__ items) {
   __shared__ float s_pr[kTileAo * kTVldf], s_ar[kTileAo * kTVldf];
   __shared__ float s_pc[kTileAo * kTVldf], s_ac[kTileAo * kTVldf];
-  __shared__ double s_c[2][4][kTVp];
+  __shared__ double s_c[2][4][kTVpf];
   const VxcItem it = items[blockIdx.x];
   if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
   const LeafDev L = a.leaves[it.leaf];
@@ -1772,21 +1820,24 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
   const int nrow = diag ? it.wr : it.wr + it.wc;
   const double *cf = a.coef + L.pt_off;
   constexpr int NQ = GGA ? 4 : 1;
-  // FP32 MFMA accumulation over one 16-point chunk at a time, added into
-  // FP64 accumulators after each chunk: the FP32 sums stay 16 terms long
+  // FP32 MFMA accumulation over one 32-point chunk at a time, added into
+  // FP64 accumulators after each chunk: the FP32 sums stay 32 terms long
   d4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  if (t < NQ * kTVp) s_c[0][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + t % kTVp];
+  if (t < NQ * kTVpf) s_c[0][t / kTVpf][t % kTVpf] = cf[(t / kTVpf) * a.ptot + t % kTVpf];
+  const int xo = (wr * hr + l16) * kTVldf + 4 * g, yo = (wcw * hc + l16) * kTVldf + 4 * g;
+  // one point loop per tile shape
+  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
   int buf = 0;
-  for (int p0 = 0; p0 < L.np; p0 += kTVp, buf ^= 1) {
+  for (int p0 = 0; p0 < L.np; p0 += kTVpf, buf ^= 1) {
     __syncthreads();
-    if (p0 + kTVp < L.np && t < NQ * kTVp)
-      s_c[buf ^ 1][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + p0 + kTVp + t % kTVp];
-    for (int gi = t; gi < nrow * (kTVp / 4); gi += 256) {
-      const int row = gi >> 2, p = 4 * (gi & 3);
+    if (p0 + kTVpf < L.np && t < NQ * kTVpf)
+      s_c[buf ^ 1][t / kTVpf][t % kTVpf] = cf[(t / kTVpf) * a.ptot + p0 + kTVpf + t % kTVpf];
+    for (int gi = t; gi < nrow * (kTVpf / 4); gi += 256) {
+      const int row = gi >> 3, p = 4 * (gi & 7);
       const bool rb = row < it.wr;
       const int mu = rb ? it.r0 + row : it.c0 + row - it.wr;
       const long long o = (long long)mu * L.np + p0 + p;
@@ -1816,22 +1867,11 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acf[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    auto product = [&](const float *xa, const float *yb) {
-      f4 xv[4], yv[4];
+    auto product = [&](const float *xa, const float *yb) __attribute__((always_inline)) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        if (i < RT) xv[i] = *reinterpret_cast<const f4 *>(xa + (wr * hr + 16 * i + l16) * kTVldf + 4 * g);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j < CT) yv[j] = *reinterpret_cast<const f4 *>(yb + (wcw * hc + 16 * j + l16) * kTVldf + 4 * g);
-#pragma unroll
-      for (int s4 = 0; s4 < 4; ++s4)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (i < RT && j < CT)
-              acf[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[i][s4], yv[j][s4], acf[i][j], 0, 0, 0);
+      for (int ks = 0; ks < kTVpf; ks += 16)
+        mfma_f32_k4<decltype(R)::value, decltype(C)::value, kTVldf>(
+            acf, xa + xo + ks, yb + yo + ks);
     };
     product(s_pr, diag ? s_ar : s_ac);
     if (!diag) product(s_ar, s_pc);
@@ -1842,6 +1882,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)acf[i][j][r];
   }
+  });
   // acc[i][j][r] = (mu = r0 + wr*hr + 16 i + 4 g + r, nu = c0 + wcw*hc + 16 j + l16)
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -2006,28 +2047,17 @@ __global__ __launch_bounds__(512, 1) void k_vxc_ws(BuildArgs a, const S *__restr
       const double *B = smem + b * kWsBuf;
       const double *s_pr = B, *s_ar = B + kTileAo * kTVld;
       const double *s_pc = B + 2 * kTileAo * kTVld, *s_ac = B + 3 * kTileAo * kTVld;
-      auto product = [&](const double *xa, const double *yb) {
+      const int xo = (wr * hr + l16) * kTVld + 4 * g, yo = (wcw * hc + l16) * kTVld + 4 * g;
+      tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+        auto product = [&](const double *xa, const double *yb) __attribute__((always_inline)) {
 #pragma unroll
-        for (int sp = 0; sp < 4; sp += 2) {
-          d2u xv[4], yv[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (i < RT) xv[i] = *reinterpret_cast<const d2u *>(xa + (wr * hr + 16 * i + l16) * kTVld + 4 * g + sp);
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (j < CT) yv[j] = *reinterpret_cast<const d2u *>(yb + (wcw * hc + 16 * j + l16) * kTVld + 4 * g + sp);
-#pragma unroll
-          for (int s2 = 0; s2 < 2; ++s2)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-              for (int j = 0; j < 4; ++j)
-                if (i < RT && j < CT)
-                  acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i][s2], yv[j][s2], acc[i][j], 0, 0, 0);
-        }
-      };
-      product(s_pr, diag ? s_ar : s_ac);
-      if (!diag) product(s_ar, s_pc);
+          for (int sp = 0; sp < 4; sp += 2)
+            mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTVld>(
+                acc, xa + xo + sp, yb + yo + sp);
+        };
+        product(s_pr, diag ? s_ar : s_ac);
+        if (!diag) product(s_ar, s_pc);
+      });
       if (nx.ii != cc.ii) {
         // the item's last chunk: scatter (mu = r0 + wr*hr + 16 i + g + 4 r,
         // nu = c0 + wcw*hc + 16 j + l16) into K(map[nu], map[mu])
@@ -2134,7 +2164,10 @@ TuneChoice tune_choice(const occxc_options *opt, int max_nbf) {
   return t;
 }
 
-static void make_tile_plan(ContractPlan &p, const std::vector<LeafDev> &leaves) {
+// vxc_only: the Vxc item list alone (FP32 storage without the tiled build
+// kernels needs it once, for dS).
+static void make_tile_plan(ContractPlan &p, const std::vector<LeafDev> &leaves,
+                           bool vxc_only = false) {
   constexpr int kXcd = 8;
   const int nl = (int)leaves.size();
   double total = 0.0;
@@ -2174,17 +2207,19 @@ static void make_tile_plan(ContractPlan &p, const std::vector<LeafDev> &leaves) 
   };
   const auto d = interleave(den, DenItem{-1, -1, 0, 0, 0, 0});
   const auto v = interleave(vxc, VxcItem{-1, 0, 0, 0, 0, 0});
-  p.n_den_items = (int)d.size();
   p.n_vxc_items = (int)v.size();
-  p.n_den_finish = (int)fin.size();
-  p.n_den_parts = nparts;
-  p.den_items.alloc(std::max<size_t>(d.size(), 1));
-  p.den_items.upload(d.data(), d.size());
   p.vxc_items.alloc(std::max<size_t>(v.size(), 1));
   p.vxc_items.upload(v.data(), v.size());
-  p.den_finish.alloc(std::max<size_t>(fin.size(), 1));
-  p.den_finish.upload(fin.data(), fin.size());
-  p.den_part.alloc((size_t)std::max(nparts, 1) * 4 * kTileAo);
+  if (!vxc_only) {
+    p.n_den_items = (int)d.size();
+    p.n_den_finish = (int)fin.size();
+    p.n_den_parts = nparts;
+    p.den_items.alloc(std::max<size_t>(d.size(), 1));
+    p.den_items.upload(d.data(), d.size());
+    p.den_finish.alloc(std::max<size_t>(fin.size(), 1));
+    p.den_finish.upload(fin.data(), fin.size());
+    p.den_part.alloc((size_t)std::max(nparts, 1) * 4 * kTileAo);
+  }
   OCCXC_HIP(hipStreamSynchronize(nullptr));  // the host lists are freed on return
   p.bytes += p.den_items.bytes() + p.vxc_items.bytes() + p.den_finish.bytes() + p.den_part.bytes();
 }
@@ -2233,6 +2268,8 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
                           reinterpret_cast<const void *>(&k_vxc_ws<float, false>)})
       OCCXC_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)vxc_ws_smem()));
+  } else if (precision != 0) {
+    make_tile_plan(p, leaves, true);
   }
   if (precision == 1 && !p.tiled) {
     p.d2f.alloc((size_t)std::max<long long>(mtot, 1));
@@ -2407,6 +2444,26 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
                                                              plan.d2f.get(), 0);
   }
   check_launch("k_density_mfma");
+}
+
+// FP32 AO storage, context set-up: dS = sum_p w_p (r Phi^T + Phi r^T) over
+// every leaf with the tiled Vxc items (A = w r), K as the accumulator.
+void run_storage_overlap(const ContractPlan &plan, const BuildArgs &a, const void *ao, double *K,
+                         double *dS, int *active, const MolDev *mols, int nmol, int maxnbf,
+                         hipStream_t s) {
+  if (plan.vxc_items.get() == nullptr || a.ao_res == nullptr)
+    throw Error(OCCXC_ERR_INTERNAL, "storage overlap: needs the Vxc item list and the residuals");
+  launch_fill_int(active, a.nleaves, 1, s);
+  launch_zero(K, a.mtot, s);
+  if (plan.n_vxc_items > 0) {
+    BuildArgs x = a;
+    x.leaf_active = active;
+    x.K = K;
+    k_vxc_tile<float, false, true><<<plan.n_vxc_items, 256, 0, s>>>(x, static_cast<const float *>(ao),
+                                                                    plan.vxc_items.get());
+    check_launch("k_vxc_tile (dS)");
+  }
+  launch_symmetrize(K, dS, mols, nmol, maxnbf, s);
 }
 
 void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s) {

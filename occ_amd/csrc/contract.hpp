@@ -78,5 +78,10 @@ void run_vxc_mgga(const ContractPlan &plan, const BuildArgs &a, const LeafDev *l
 // two-pass Vxc: V = sum over leaves of the stored K_loc blocks (+ transposes),
 // one workgroup per 32 x 32 tile pair, fixed summation order
 void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s);
+// FP32 AO storage, at context set-up: dS = sum_p w_p (r Phi^T + Phi r^T) from
+// the value plane's rounding residuals a.ao_res (K is overwritten)
+void run_storage_overlap(const ContractPlan &plan, const BuildArgs &a, const void *ao, double *K,
+                         double *dS, int *active, const MolDev *mols, int nmol, int maxnbf,
+                         hipStream_t s);
 
 } // namespace occxc

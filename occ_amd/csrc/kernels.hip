@@ -363,11 +363,15 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
       continue;
     }
     T *row = base + (long long)mu * L.np + p;
+    float *res = a.ao_res ? a.ao_res + L.res_off + (long long)mu * L.np + p : nullptr;
     const unsigned long long key0 = pkey + 0x2545F4914F6CDD1Dull * (unsigned long long)sh.bf0;
     auto sink = [&](int q, int m, double v) {
       const long long off = q * cstride + (long long)m * L.np;
-      row[off] = store_ao<T>(v, key0 + 0x2545F4914F6CDD1Dull * (unsigned long long)m +
-                                    0x9E3779B97F4A7C15ull * (unsigned long long)q);
+      const T s = store_ao<T>(v, key0 + 0x2545F4914F6CDD1Dull * (unsigned long long)m +
+                                     0x9E3779B97F4A7C15ull * (unsigned long long)q);
+      row[off] = s;
+      if constexpr (std::is_same<T, float>::value)
+        if (q == 0 && res) res[(long long)m * L.np] = (float)(v - (double)s);
     };
     const double dx = x - sh.x, dy = y - sh.y, dz = z - sh.z;
     if (sh.l == 0) shell_point<0>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
@@ -378,9 +382,11 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
     else shell_point<LMAX>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
     mu += sh.size;
   }
-  for (int r = L.m + split; r < L.mp; r += nsplit)
+  for (int r = L.m + split; r < L.mp; r += nsplit) {
     for (int c = 0; c < a.ncomp; ++c)
       base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
+    if (a.ao_res) a.ao_res[L.res_off + (long long)r * L.np + p] = 0.0f;
+  }
 }
 
 // Derivative components the resident AO cache does not hold, for the XC
@@ -826,6 +832,40 @@ __global__ void k_functional_mgga_pol_points(int id, long long n, const double *
   }
 }
 
+__global__ void k_fill_int(int *__restrict__ p, int n, int v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// FP32 AO storage: partial[m * kNcorrBlocks + b] = sum over block b's
+// contiguous share of molecule m's nbf^2 entries of D2 dS (a fixed order:
+// every build gives the same sum), then k_ncorr_final adds the blocks in
+// order to N_elec.
+__global__ __launch_bounds__(256) void k_ncorr_partial(const double *__restrict__ D2,
+                                                       const double *__restrict__ dS,
+                                                       const MolDev *__restrict__ mols,
+                                                       double *__restrict__ partial) {
+  __shared__ double s_red[4];
+  const MolDev M = mols[blockIdx.y];
+  const long long n = (long long)M.nbf * M.nbf;
+  const long long per = (n + kNcorrBlocks - 1) / kNcorrBlocks;
+  const long long k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+  double v = 0.0;
+  for (long long k = k0 + threadIdx.x; k < k1; k += 256) v += D2[M.moff + k] * dS[M.moff + k];
+  v = block_sum_256(v, s_red);
+  if (threadIdx.x == 0) partial[(long long)blockIdx.y * kNcorrBlocks + blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(256) void k_ncorr_final(const double *__restrict__ partial,
+                                                     double *__restrict__ result) {
+  __shared__ double s_red[4];
+  const double v = threadIdx.x < kNcorrBlocks
+                       ? partial[(long long)blockIdx.x * kNcorrBlocks + threadIdx.x]
+                       : 0.0;
+  const double t = block_sum_256(v, s_red);
+  if (threadIdx.x == 0) result[2 * blockIdx.x + 1] += t;
+}
+
 __global__ void k_finalize_matrix(const double *__restrict__ K, double *__restrict__ V,
                                   const MolDev *__restrict__ mols) {
   const MolDev M = mols[blockIdx.y];
@@ -1038,6 +1078,12 @@ void launch_zero(double *p, long long n, hipStream_t s) {
   check_launch();
 }
 
+void launch_fill_int(int *p, int n, int v, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill_int<<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+  check_launch();
+}
+
 static dim3 mol_grid(int maxnbf, int nmol) {
   long long gx = ((long long)maxnbf * maxnbf + 255) / 256;
   gx = gx < 1 ? 1 : (gx > 1024 ? 1024 : gx);
@@ -1169,6 +1215,24 @@ void launch_functional_mgga_pol(int id, long long n, const double *rho, const do
   if (n == 0) return;
   k_functional_mgga_pol_points<<<grid_for(n, 256), 256, 0, s>>>(id, n, rho, sigma, tau, exc,
                                                                  vrho, vsigma, vtau);
+  check_launch();
+}
+
+void launch_symmetrize(const double *K, double *out, const MolDev *mols, int nmol, int maxnbf,
+                       hipStream_t s) {
+  if (nmol == 0) return;
+  k_finalize_matrix<<<mol_grid(maxnbf, nmol), 256, 0, s>>>(K, out, mols);
+  check_launch();
+}
+
+void launch_storage_correction(const double *D2, const double *dS, const MolDev *mols, int nmol,
+                               int maxnbf, double *partial, double *result, hipStream_t s) {
+  (void)maxnbf;
+  if (nmol == 0) return;
+  static_assert(kNcorrBlocks <= 256, "one thread per partial in k_ncorr_final");
+  k_ncorr_partial<<<dim3(kNcorrBlocks, nmol), 256, 0, s>>>(D2, dS, mols, partial);
+  check_launch();
+  k_ncorr_final<<<nmol, 256, 0, s>>>(partial, result);
   check_launch();
 }
 

@@ -36,6 +36,7 @@ struct LeafDev {
   int mol;      // molecule of this leaf (index into BuildArgs::mols)
   long long ao_off;  // element offset of this leaf's AO block [ncomp][mp][np]
   long long kl_off;  // element offset of this leaf's K_loc block (two-pass Vxc)
+  long long res_off; // element offset of this leaf's value-plane residuals [mp][np] (FP32 storage)
 };
 
 // Two-pass Vxc: one global 32 x 32 AO tile pair (TI <= TJ) of a molecule and
@@ -151,6 +152,10 @@ struct BuildArgs {
                         // planes; the Vxc stage reads 4 for a GGA, 1 for an LDA)
   double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
   int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
+  // FP32 AO storage, during context set-up only: the value plane's rounding
+  // residuals r = exact value - stored float, [mp][np] per leaf at res_off
+  // (null otherwise)
+  float *ao_res;
   double *kloc;         // two-pass Vxc: per-leaf K_loc blocks (null: atomics into K)
   const TilePairDev *tpairs;
   int ntpairs;
@@ -188,6 +193,16 @@ void launch_fill_leaves(const LeafDev *leaves, const FillLeaf *fill, int nactive
 // host-callable launchers (kernels.hip)
 void upload_sph_table();
 void launch_zero(double *p, long long n, hipStream_t s);
+void launch_fill_int(int *p, int n, int v, hipStream_t s);
+// out = K + K^T per molecule
+void launch_symmetrize(const double *K, double *out, const MolDev *mols, int nmol, int maxnbf,
+                       hipStream_t s);
+// FP32 AO storage: result[2 m + 1] (N_elec of molecule m) += sum D2 (.) dS,
+// the first-order change of sum_p w_p rho_p from the stored AO values to the
+// exact ones; partial holds kNcorrBlocks x nmol doubles
+constexpr int kNcorrBlocks = 64;
+void launch_storage_correction(const double *D2, const double *dS, const MolDev *mols, int nmol,
+                               int maxnbf, double *partial, double *result, hipStream_t s);
 void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int nmol,
                             int maxnbf, hipStream_t s);
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s);

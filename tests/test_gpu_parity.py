@@ -183,6 +183,21 @@ def test_xc_mixed_precision_reported(precision):
     assert de <= TOL_E
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp32cache"])
+def test_fp32_storage_n_elec_corrected(precision):
+    """FP32 AO storage: N_elec carries the first-order correction sum D2 dS
+    (dS = sum_p w_p (r Phi^T + Phi r^T) from the rounding residuals r), so it
+    matches the FP64 oracle to well inside 1e-8, while Exc and Vxc keep the
+    FP32 storage error (within their stated tolerances).  (The split-BF16
+    mode's N_elec error comes from its density contraction, not the storage.)"""
+    de, dv, dn, exc, nel = _xc_case(molecules.water_cluster(3, seed=2), "def2-svp", "blyp", 17,
+                                    pure=True, precision=precision)
+    print(f"{precision}: |dExc|={de:.2e} max|dVxc|={dv:.2e} |dN|={dn:.2e}")
+    assert dn <= 1e-10
+    assert de <= TOL_E
+    assert dv <= TOL_V
+
+
 def test_cpp_driver_drop_in(tmp_path):
     """A C++ caller through include/occxc_dft.hpp gives the Python result and
     repeats it on the second build of a resident context."""
