@@ -1399,9 +1399,15 @@ __global__ __launch_bounds__(256) void k_vxc_reduce_final(BuildArgs a, double *_
 // whose product is negligible over the leaf), and the FP64 atomics they save
 // are the largest part of the Vxc stage.
 constexpr double kScatterCut = 1e-14;
+// dS entries (run_storage_overlap) below this are not added: the N_elec
+// correction sum D2 dS changes by at most 1e-18 sum |D2| over the dropped
+// entries (under 2e-11 for 3072 AOs and |D2| <= 2).
+constexpr double kStorageCut = 1e-18;
 
 constexpr int kTKc = 16;         // AOs per density k-chunk
-constexpr int kTKld = kTKc + 4;  // LDS row stride (doubles) of the [row][k] density operands
+// LDS row stride (doubles) of the [row][k] density operands: 36 dwords, so
+// the 16 rows of an MFMA operand read start in 16 different 4-bank groups
+constexpr int kTKld = kTKc + 2;
 constexpr int kTVp = 16;         // points per Vxc chunk
 constexpr int kTVld = kTVp + 2;  // LDS row stride (doubles) of the [row][point] Vxc operands
 
@@ -1532,7 +1538,7 @@ __device__ __forceinline__ void den_tile_item(const BuildArgs &a, const S *__res
       }
       if (gi < nd) {
         const int col = gi >> 2, kk = 4 * (gi & 3);
-        *reinterpret_cast<d4 *>(s_d + col * kTKld + kk) = rd[u];
+        *reinterpret_cast<d4u *>(s_d + col * kTKld + kk) = rd[u];
       }
     }
   };
@@ -1687,8 +1693,8 @@ __global__ __launch_bounds__(256) void k_den_finish(BuildArgs a, const DenFinish
 // in registers.  Entry (mu, nu) is added to K(map[nu], map[mu]) with FP64
 // atomics; Vxc = K + K^T afterwards counts every entry of V_loc once.
 // RES (FP32 storage, context set-up): A = w r with r the value plane's
-// rounding residuals, so K + K^T = dS = sum_p w_p (r Phi^T + Phi r^T); no
-// entry is screened out.
+// rounding residuals, so K + K^T = dS = sum_p w_p (r Phi^T + Phi r^T);
+// entries below kStorageCut are not added.
 template <typename S, bool GGA, bool RES = false>
 __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__restrict__ ao,
                                                      const VxcItem *__restrict__ items) {
@@ -1781,7 +1787,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
-        if (mu < L.m && (RES || fabs(acc[i][j][r]) >= kScatterCut))
+        if (mu < L.m && fabs(acc[i][j][r]) >= (RES ? kStorageCut : kScatterCut))
           atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
       }
     }
@@ -2047,17 +2053,15 @@ __global__ __launch_bounds__(512, 1) void k_vxc_ws(BuildArgs a, const S *__restr
       const double *B = smem + b * kWsBuf;
       const double *s_pr = B, *s_ar = B + kTileAo * kTVld;
       const double *s_pc = B + 2 * kTileAo * kTVld, *s_ac = B + 3 * kTileAo * kTVld;
+      // the 4 x 4 tile form for every item (a shape dispatch inside this
+      // loop spills): the extra tiles of smaller blocks are never scattered
       const int xo = (wr * hr + l16) * kTVld + 4 * g, yo = (wcw * hc + l16) * kTVld + 4 * g;
-      tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
-        auto product = [&](const double *xa, const double *yb) __attribute__((always_inline)) {
+      auto product = [&](const double *xa, const double *yb) __attribute__((always_inline)) {
 #pragma unroll
-          for (int sp = 0; sp < 4; sp += 2)
-            mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTVld>(
-                acc, xa + xo + sp, yb + yo + sp);
-        };
-        product(s_pr, diag ? s_ar : s_ac);
-        if (!diag) product(s_ar, s_pc);
-      });
+        for (int sp = 0; sp < 4; sp += 2) mfma_f64_k2<4, 4, kTVld>(acc, xa + xo + sp, yb + yo + sp);
+      };
+      product(s_pr, diag ? s_ar : s_ac);
+      if (!diag) product(s_ar, s_pc);
       if (nx.ii != cc.ii) {
         // the item's last chunk: scatter (mu = r0 + wr*hr + 16 i + g + 4 r,
         // nu = c0 + wcw*hc + 16 j + l16) into K(map[nu], map[mu])

@@ -20,6 +20,7 @@
 // kernels (contract_mfma.hip) replace k_density_functional / k_vxc for speed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <mutex>
@@ -334,7 +335,9 @@ template <typename T> __device__ __forceinline__ T store_ao(double v, unsigned l
 // LMAX is the largest angular momentum the kernel handles: the register
 // allocation is that of the largest shell, so bases with l <= 2 run the
 // LMAX = 2 instance.
-template <typename T, int LMAX>
+// RES (FP32 storage, context set-up): also the value plane's rounding
+// residuals v - s into a.ao_res.
+template <typename T, int LMAX, bool RES = false>
 __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao) {
   const LeafDev L = a.leaves[blockIdx.x];
   // lanes over (point, shell split): a leaf of np points takes 256 / np
@@ -363,15 +366,15 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
       continue;
     }
     T *row = base + (long long)mu * L.np + p;
-    float *res = a.ao_res ? a.ao_res + L.res_off + (long long)mu * L.np + p : nullptr;
+    float *res = RES ? a.ao_res + L.res_off + (long long)mu * L.np + p : nullptr;
     const unsigned long long key0 = pkey + 0x2545F4914F6CDD1Dull * (unsigned long long)sh.bf0;
     auto sink = [&](int q, int m, double v) {
       const long long off = q * cstride + (long long)m * L.np;
       const T s = store_ao<T>(v, key0 + 0x2545F4914F6CDD1Dull * (unsigned long long)m +
                                      0x9E3779B97F4A7C15ull * (unsigned long long)q);
       row[off] = s;
-      if constexpr (std::is_same<T, float>::value)
-        if (q == 0 && res) res[(long long)m * L.np] = (float)(v - (double)s);
+      if constexpr (RES)
+        if (q == 0) res[(long long)m * L.np] = (float)(v - (double)s);
     };
     const double dx = x - sh.x, dy = y - sh.y, dz = z - sh.z;
     if (sh.l == 0) shell_point<0>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
@@ -385,7 +388,7 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
   for (int r = L.m + split; r < L.mp; r += nsplit) {
     for (int c = 0; c < a.ncomp; ++c)
       base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
-    if (a.ao_res) a.ao_res[L.res_off + (long long)r * L.np + p] = 0.0f;
+    if constexpr (RES) a.ao_res[L.res_off + (long long)r * L.np + p] = 0.0f;
   }
 }
 
@@ -838,9 +841,9 @@ __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
 }
 
 // FP32 AO storage: partial[m * kNcorrBlocks + b] = sum over block b's
-// contiguous share of molecule m's nbf^2 entries of D2 dS (a fixed order:
-// every build gives the same sum), then k_ncorr_final adds the blocks in
-// order to N_elec.
+// contiguous share (of gridDim.x) of molecule m's nbf^2 entries of D2 dS (a
+// fixed order: every build gives the same sum), then k_ncorr_final adds the
+// blocks in order to N_elec.
 __global__ __launch_bounds__(256) void k_ncorr_partial(const double *__restrict__ D2,
                                                        const double *__restrict__ dS,
                                                        const MolDev *__restrict__ mols,
@@ -848,7 +851,7 @@ __global__ __launch_bounds__(256) void k_ncorr_partial(const double *__restrict_
   __shared__ double s_red[4];
   const MolDev M = mols[blockIdx.y];
   const long long n = (long long)M.nbf * M.nbf;
-  const long long per = (n + kNcorrBlocks - 1) / kNcorrBlocks;
+  const long long per = (n + gridDim.x - 1) / gridDim.x;
   const long long k0 = blockIdx.x * per, k1 = min(n, k0 + per);
   double v = 0.0;
   for (long long k = k0 + threadIdx.x; k < k1; k += 256) v += D2[M.moff + k] * dS[M.moff + k];
@@ -856,12 +859,11 @@ __global__ __launch_bounds__(256) void k_ncorr_partial(const double *__restrict_
   if (threadIdx.x == 0) partial[(long long)blockIdx.y * kNcorrBlocks + blockIdx.x] = v;
 }
 
-__global__ __launch_bounds__(256) void k_ncorr_final(const double *__restrict__ partial,
+__global__ __launch_bounds__(256) void k_ncorr_final(const double *__restrict__ partial, int nb,
                                                      double *__restrict__ result) {
   __shared__ double s_red[4];
-  const double v = threadIdx.x < kNcorrBlocks
-                       ? partial[(long long)blockIdx.x * kNcorrBlocks + threadIdx.x]
-                       : 0.0;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) v += partial[(long long)blockIdx.x * kNcorrBlocks + b];
   const double t = block_sum_256(v, s_red);
   if (threadIdx.x == 0) result[2 * blockIdx.x + 1] += t;
 }
@@ -1100,7 +1102,10 @@ void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s) {
   if (a.nleaves == 0) return;
   const bool low = a.lmax <= 2;
-  if (fp32) {
+  if (fp32 && a.ao_res) {
+    if (low) k_ao_eval<float, 2, true><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
+    else k_ao_eval<float, kMaxL, true><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
+  } else if (fp32) {
     if (low) k_ao_eval<float, 2><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
     else k_ao_eval<float, kMaxL><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
   } else {
@@ -1227,12 +1232,13 @@ void launch_symmetrize(const double *K, double *out, const MolDev *mols, int nmo
 
 void launch_storage_correction(const double *D2, const double *dS, const MolDev *mols, int nmol,
                                int maxnbf, double *partial, double *result, hipStream_t s) {
-  (void)maxnbf;
   if (nmol == 0) return;
-  static_assert(kNcorrBlocks <= 256, "one thread per partial in k_ncorr_final");
-  k_ncorr_partial<<<dim3(kNcorrBlocks, nmol), 256, 0, s>>>(D2, dS, mols, partial);
+  // about 8192 entries per block for the largest molecule
+  const long long nn = (long long)maxnbf * maxnbf;
+  const int nb = (int)std::max(1LL, std::min((long long)kNcorrBlocks, (nn + 8191) / 8192));
+  k_ncorr_partial<<<dim3(nb, nmol), 256, 0, s>>>(D2, dS, mols, partial);
   check_launch();
-  k_ncorr_final<<<nmol, 256, 0, s>>>(partial, result);
+  k_ncorr_final<<<nmol, 256, 0, s>>>(partial, nb, result);
   check_launch();
 }
 

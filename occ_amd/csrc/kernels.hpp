@@ -200,7 +200,7 @@ void launch_symmetrize(const double *K, double *out, const MolDev *mols, int nmo
 // FP32 AO storage: result[2 m + 1] (N_elec of molecule m) += sum D2 (.) dS,
 // the first-order change of sum_p w_p rho_p from the stored AO values to the
 // exact ones; partial holds kNcorrBlocks x nmol doubles
-constexpr int kNcorrBlocks = 64;
+constexpr int kNcorrBlocks = 1024;
 void launch_storage_correction(const double *D2, const double *dS, const MolDev *mols, int nmol,
                                int maxnbf, double *partial, double *result, hipStream_t s);
 void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int nmol,
