@@ -641,12 +641,19 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
                     std::to_string(byte_limit >> 20) + " MiB");
   size_t free_b = 0, total_b = 0;
   OCCXC_HIP(hipMemGetInfo(&free_b, &total_b));
-  const size_t setup_bytes = ao_bytes + ds_bytes + c->res_elems * sizeof(float);
+  // the residuals are formed a bounded range of leaves at a time
+  size_t res_cap = 0;
+  if (c->res_elems) {
+    size_t max_leaf = 0;
+    for (const LeafDev &L : c->leaves) max_leaf = std::max(max_leaf, (size_t)L.mp * L.np);
+    res_cap = std::min(c->res_elems, std::max(kResidualChunkElems, max_leaf));
+  }
+  const size_t setup_bytes = ao_bytes + ds_bytes + res_cap * sizeof(float);
   OCCXC_REQUIRE(setup_bytes + (size_t)6 * c->mtot * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
                 "resident AO values need " + std::to_string(setup_bytes >> 20) + " MiB, " +
                     std::to_string(free_b >> 20) + " MiB free");
   c->d_ao.alloc(ao_bytes);
-  if (c->res_elems) c->d_ao_res.alloc(c->res_elems);
+  if (res_cap) c->d_ao_res.alloc(res_cap);
   const size_t nn = (size_t)c->mtot;
   c->d_D.alloc(nn);
   c->d_D2.alloc(nn);
@@ -688,7 +695,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.coef_tau_planes = 0;
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
-  a.ao_res = c->res_elems ? c->d_ao_res.get() : nullptr;  // set-up only
+  a.ao_res = nullptr;  // set only while dS is formed
 
   c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream,
                                tune_choice(opt, c->maxnbf));
@@ -786,16 +793,34 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
 
   launch_ao_eval(a, c->d_ao.get(), opt->precision != 0, c->stream);
   if (c->res_elems) {
-    // dS from the residuals (the Vxc item list drives the contraction),
-    // then the residuals are released
+    // dS = K + K^T, K = sum_p w_p r Phi^T accumulated over ranges of leaves
+    // whose residuals fit the buffer (the Vxc item list drives the
+    // contraction); then the residuals are released
     c->d_dS.alloc((size_t)c->mtot);
     c->d_ncorr.alloc((size_t)kNcorrBlocks * nmol);
-    run_storage_overlap(c->plan, a, c->d_ao.get(), c->d_K.get(), c->d_dS.get(), c->d_active.get(),
-                        c->d_mols.get(), c->nmol, c->maxnbf, c->stream);
+    launch_fill_int(c->d_active.get(), c->nleaves, 1, c->stream);
+    launch_zero(c->d_K.get(), c->mtot, c->stream);
+    const size_t cap = c->d_ao_res.size();
+    for (int l0 = 0; l0 < c->nleaves;) {
+      size_t used = 0;
+      int l1 = l0;
+      while (l1 < c->nleaves && used + (size_t)c->leaves[l1].mp * c->leaves[l1].np <= cap) {
+        used += (size_t)c->leaves[l1].mp * c->leaves[l1].np;
+        ++l1;
+      }
+      BuildArgs x = a;
+      x.ao_res = c->d_ao_res.get();
+      x.res_leaf0 = l0;
+      x.res_leaf1 = l1;
+      x.res_base = c->leaves[l0].res_off;
+      launch_ao_residuals(x, c->d_ao.get(), c->stream);
+      run_storage_overlap(c->plan, x, c->d_ao.get(), c->stream);
+      l0 = l1;
+    }
+    launch_symmetrize(c->d_K.get(), c->d_dS.get(), c->d_mols.get(), c->nmol, c->maxnbf, c->stream);
     c->device_bytes += c->d_ncorr.bytes();
     OCCXC_HIP(hipStreamSynchronize(c->stream));
     c->d_ao_res.release();
-    a.ao_res = nullptr;
   }
   OCCXC_HIP(hipStreamSynchronize(c->stream));
   c->ao_ready = true;

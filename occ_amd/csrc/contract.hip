@@ -295,6 +295,10 @@ __global__ __launch_bounds__(256) void k_density_mfma(BuildArgs a, const S *__re
 // one are issued.
 constexpr int kMapCap = 2048;
 constexpr int kDefaultDensityVariant = 2;
+// tiled density items staged by direct-to-LDS loads (k_den_tile_dma; measured
+// faster than the register-staged k_den_tile in both storage precisions);
+// OCCXC_DENSITY_DMA=0/1 overrides
+constexpr bool kDefaultDensityDma = true;
 
 // STORE: write rho and grad rho of every (padded) point to a.rho_out
 // instead of running the functional epilogue (the unrestricted build
@@ -1466,100 +1470,25 @@ __device__ __forceinline__ void mfma_f32_k4(f4 (&acc)[4][4], const float *x, con
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xv[i][s4], yv[j][s4], acc[i][j], 0, 0, 0);
 }
 
-// Density item: T(p, nu) = sum_mu Phi(p, mu) D2(mu, nu) for the item's points p
-// and columns nu, over all of the leaf's AOs mu in k-chunks staged through LDS
-// (Phi transposed to [point][mu], D2 gathered through the AO map to
-// [nu][mu]); wavefront (wp, wc) owns half the points and half the columns.
-// The loads of chunk k+1 are issued before the MFMAs of chunk k.  Then the
-// column-block partials of rho = sum_nu Phi T and
-// grad rho = 2 sum_nu grad Phi T (density.cpp:32-43); the leaf's only item
-// runs the functional epilogue at once, the others store their partials.
-// One density item (the body of k_den_tile): s_phi, s_d, s_map and s_red
-// are the workgroup's LDS arrays.
+// The epilogue of a density item: from acc[i][j][r] = T(point wp*hp + 16 i +
+// 4 g + r, column wc*hc + 16 j + l16), the item's partials of rho and
+// grad rho (Phi and grad Phi of its columns read from HBM), then the
+// functional (the leaf's only item) or the stored partials.  s_phi (at least
+// 4 kMaxLeafPts + 4 kTileAo doubles of LDS) holds the sums.
 template <typename S, bool GGA>
-__device__ __forceinline__ void den_tile_item(const BuildArgs &a, const S *__restrict__ ao,
-                                              const DenItem &it, double *__restrict__ part,
-                                              double *s_phi, double *s_d, int *s_map,
-                                              double *s_red) {
+__device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *__restrict__ ao,
+                                                  const DenItem &it, const LeafDev &L,
+                                                  d4 (&acc)[4][4], double *__restrict__ part,
+                                                  double *s_phi, double *s_red) {
   double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(s_phi);
   double(*s_h)[kTileAo] = reinterpret_cast<double(*)[kTileAo]>(s_phi + 4 * kMaxLeafPts);
-  const LeafDev L = a.leaves[it.leaf];
-  const MolDev MD = a.mols[L.mol];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int l16 = lane & 15, g = lane >> 4;
   const int wp = wave & 1, wc = wave >> 1;
-  const int hp = it.np >> 1, hc = it.wc >> 1;  // points / columns per wavefront
-  const int RT = hp >> 4, CT = hc >> 4;        // 16x16 tiles per wavefront
-  // f64 MFMA: feeding point 4(i%4) + i/4 as row i gives lane group g the
-  // accumulator rows of points 4g .. 4g+3 (one vector load in the epilogue)
-  const int pa = 4 * (l16 & 3) + (l16 >> 2);
+  const int hp = it.np >> 1, hc = it.wc >> 1;
+  const int RT = hp >> 4, CT = hc >> 4;
   const S *phi = ao + L.ao_off;
   const long long cs = (long long)L.mp * L.np;
-  const double *Dm = a.D2 + MD.moff;
-  {
-    const int *map = a.ao_map + L.map_off;
-    for (int i = t; i < L.mp; i += 256) s_map[i] = map[i];
-  }
-  d4 acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  // per chunk: kTKc x np Phi values in groups of 4 points, wc x kTKc D values
-  // in groups of 4 AOs; at most 512 groups of each, two per thread
-  const int nphi = kTKc * (it.np >> 2), nd = it.wc * (kTKc / 4);
-  d4 rp[2], rd[2];
-  auto fetch = [&](int k0) {
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int gi = t + 256 * u;
-      if (gi < nphi) {
-        const int mu = gi & (kTKc - 1), p = 4 * (gi / kTKc);
-        rp[u] = load4_as<double>(phi + (long long)(k0 + mu) * L.np + it.p0 + p);
-      }
-      if (gi < nd) {
-        const int col = gi >> 2, kk = 4 * (gi & 3);
-        const double *dc = Dm + (long long)s_map[it.c0 + col] * MD.nbf;
-        rd[u] = d4{dc[s_map[k0 + kk]], dc[s_map[k0 + kk + 1]], dc[s_map[k0 + kk + 2]],
-                   dc[s_map[k0 + kk + 3]]};
-      }
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int gi = t + 256 * u;
-      if (gi < nphi) {
-        // AO fastest over the lanes: the transposed writes of 16 lanes fall in
-        // consecutive LDS words
-        const int mu = gi & (kTKc - 1), p = 4 * (gi / kTKc);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) s_phi[(p + e) * kTKld + mu] = rp[u][e];
-      }
-      if (gi < nd) {
-        const int col = gi >> 2, kk = 4 * (gi & 3);
-        *reinterpret_cast<d4u *>(s_d + col * kTKld + kk) = rd[u];
-      }
-    }
-  };
-  __syncthreads();  // s_map
-  fetch(0);
-  const double *xa = s_phi + (wp * hp + pa) * kTKld + 4 * g;
-  const double *yb = s_d + (wc * hc + l16) * kTKld + 4 * g;
-  // one k loop per tile shape (a shape test inside the loop puts every
-  // shape's accumulators through the merge points and spills)
-  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
-    for (int k0 = 0; k0 < L.mp; k0 += kTKc) {
-      __syncthreads();  // the previous chunk's operands are consumed
-      store();
-      __syncthreads();
-      if (k0 + kTKc < L.mp) fetch(k0 + kTKc);
-#pragma unroll
-      for (int sp = 0; sp < kTKc / 4; sp += 2)
-        mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTKld>(
-            acc, xa + sp, yb + sp);
-    }
-  });
   __syncthreads();  // s_phi is reused for the sums below
   // acc[i][j][r] = T(point wp*hp + 16 i + 4 g + r, column wc*hc + 16 j + l16)
 #pragma unroll
@@ -1635,6 +1564,252 @@ __device__ __forceinline__ void den_tile_item(const BuildArgs &a, const S *__res
   }
 }
 
+// Density item: T(p, nu) = sum_mu Phi(p, mu) D2(mu, nu) for the item's points p
+// and columns nu, over all of the leaf's AOs mu in k-chunks staged through LDS
+// (Phi transposed to [point][mu], D2 gathered through the AO map to
+// [nu][mu]); wavefront (wp, wc) owns half the points and half the columns.
+// The loads of chunk k+1 are issued before the MFMAs of chunk k.  Then the
+// column-block partials of rho = sum_nu Phi T and
+// grad rho = 2 sum_nu grad Phi T (density.cpp:32-43); the leaf's only item
+// runs the functional epilogue at once, the others store their partials.
+// One density item (the body of k_den_tile): s_phi, s_d, s_map and s_red
+// are the workgroup's LDS arrays.
+template <typename S, bool GGA>
+__device__ __forceinline__ void den_tile_item(const BuildArgs &a, const S *__restrict__ ao,
+                                              const DenItem &it, double *__restrict__ part,
+                                              double *s_phi, double *s_d, int *s_map,
+                                              double *s_red) {
+  const LeafDev L = a.leaves[it.leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wp = wave & 1, wc = wave >> 1;
+  const int hp = it.np >> 1, hc = it.wc >> 1;  // points / columns per wavefront
+  const int RT = hp >> 4, CT = hc >> 4;        // 16x16 tiles per wavefront
+  // f64 MFMA: feeding point 4(i%4) + i/4 as row i gives lane group g the
+  // accumulator rows of points 4g .. 4g+3 (one vector load in the epilogue)
+  const int pa = 4 * (l16 & 3) + (l16 >> 2);
+  const S *phi = ao + L.ao_off;
+  const double *Dm = a.D2 + MD.moff;
+  {
+    const int *map = a.ao_map + L.map_off;
+    for (int i = t; i < L.mp; i += 256) s_map[i] = map[i];
+  }
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  // per chunk: kTKc x np Phi values in groups of 4 points, wc x kTKc D values
+  // in groups of 4 AOs; at most 512 groups of each, two per thread
+  const int nphi = kTKc * (it.np >> 2), nd = it.wc * (kTKc / 4);
+  d4 rp[2], rd[2];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int gi = t + 256 * u;
+      if (gi < nphi) {
+        const int mu = gi & (kTKc - 1), p = 4 * (gi / kTKc);
+        rp[u] = load4_as<double>(phi + (long long)(k0 + mu) * L.np + it.p0 + p);
+      }
+      if (gi < nd) {
+        const int col = gi >> 2, kk = 4 * (gi & 3);
+        const double *dc = Dm + (long long)s_map[it.c0 + col] * MD.nbf;
+        rd[u] = d4{dc[s_map[k0 + kk]], dc[s_map[k0 + kk + 1]], dc[s_map[k0 + kk + 2]],
+                   dc[s_map[k0 + kk + 3]]};
+      }
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int gi = t + 256 * u;
+      if (gi < nphi) {
+        // AO fastest over the lanes: the transposed writes of 16 lanes fall in
+        // consecutive LDS words
+        const int mu = gi & (kTKc - 1), p = 4 * (gi / kTKc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) s_phi[(p + e) * kTKld + mu] = rp[u][e];
+      }
+      if (gi < nd) {
+        const int col = gi >> 2, kk = 4 * (gi & 3);
+        *reinterpret_cast<d4u *>(s_d + col * kTKld + kk) = rd[u];
+      }
+    }
+  };
+  __syncthreads();  // s_map
+  fetch(0);
+  const double *xa = s_phi + (wp * hp + pa) * kTKld + 4 * g;
+  const double *yb = s_d + (wc * hc + l16) * kTKld + 4 * g;
+  // one k loop per tile shape (a shape test inside the loop puts every
+  // shape's accumulators through the merge points and spills)
+  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+    for (int k0 = 0; k0 < L.mp; k0 += kTKc) {
+      __syncthreads();  // the previous chunk's operands are consumed
+      store();
+      __syncthreads();
+      if (k0 + kTKc < L.mp) fetch(k0 + kTKc);
+#pragma unroll
+      for (int sp = 0; sp < kTKc / 4; sp += 2)
+        mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTKld>(
+            acc, xa + sp, yb + sp);
+    }
+  });
+  den_tile_epilogue<S, GGA>(a, ao, it, L, acc, part, s_phi, s_red);
+}
+
+// Density item with the operands staged by direct-to-LDS loads
+// (global_load_lds: each lane's bytes land at the wavefront's LDS base +
+// lane x size, with no registers involved) into NS LDS chunk buffers, NS - 1
+// chunks ahead of the MFMAs: FP32 storage keeps three buffers (two chunks in
+// flight), FP64 storage two.  A buffer holds the chunk's Phi as stored
+// ([16 AOs][128 points], S), then D2 gathered through the AO map ([16 AOs]
+// [128 columns], FP64, each double as two 4-byte pieces from two lanes).
+// Every wavefront issues the same loads for every chunk (lanes past the
+// item's points or columns load a valid duplicate into LDS slots that only
+// the unused tiles read), so a chunk's loads are waited for by count.  The
+// MFMA k-step t reads AO 4 g + t of both, so the accumulators are those of
+// den_tile_item and the epilogue is shared.
+template <typename S> struct DenDma {
+  static constexpr int NS = sizeof(S) == 8 ? 2 : 3;               // LDS chunk buffers
+  static constexpr int PHI = kTKc * kTileAo * (int)sizeof(S);     // bytes of Phi per buffer
+  static constexpr int STAGE = PHI + kTKc * kTileAo * 8;          // bytes per buffer
+  static constexpr int LOADS = (sizeof(S) == 8 ? 4 : 2) + 16;     // loads per chunk per wavefront
+};
+static_assert(4 * kMaxLeafPts + 4 * kTileAo <= DenDma<float>::STAGE / 8, "epilogue sums fit a stage");
+
+// Barrier after this wavefront's vector memory loads but the newest N are
+// done (and its LDS accesses): the s_waitcnt immediate holds vmcnt in bits
+// 3:0 and 15:14, expcnt (no wait) in 6:4 and lgkmcnt (0) in 11:8.
+template <int N>
+__device__ __forceinline__ void vmcnt_barrier() {
+  static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | (0x7 << 4) | ((N >> 4) << 14));
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <typename S, bool GGA>
+__device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *__restrict__ ao,
+                                                  const int *__restrict__ ao_map,
+                                                  const DenItem &it, double *__restrict__ part,
+                                                  char *s_stage, double *s_red) {
+  using P = DenDma<S>;
+  const LeafDev L = a.leaves[it.leaf];
+  const MolDev MD = a.mols[L.mol];
+  const int t = threadIdx.x, lane = t & 63;
+  // the wavefront index as a scalar: the AO rows of a load and their map
+  // entries are then scalar values (scalar loads, counted apart from the
+  // vector memory loads)
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wp = wave & 1, wc = wave >> 1;
+  const int hp = it.np >> 1, hc = it.wc >> 1;
+  const int RT = hp >> 4, CT = hc >> 4;
+  const int pa = 4 * (l16 & 3) + (l16 >> 2);
+  const S *phi = ao + L.ao_off;
+  const double *Dm = a.D2 + MD.moff;
+  // the AO map through a restrict kernel argument: uniform reads of it are
+  // scalar loads (no wait on the vector memory counter, which counts the
+  // chunks' loads)
+  const int *map = ao_map + L.map_off;
+  // D2 gather: load q (0..15) of wave w fetches AO row 4 w + q / 4, columns
+  // 32 (q % 4) .. + 31; lane l takes column 32 (q % 4) + l / 2 (clamped to
+  // the item's last column), half l % 2.  D2 is symmetric (k_prepare_density
+  // mirrors one triangle), so the row's entries are read along D2's
+  // contiguous index: consecutive lanes, consecutive addresses wherever the
+  // map is contiguous
+  long long colbase[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) colbase[cb] = map[it.c0 + min(32 * cb + (lane >> 1), it.wc - 1)];
+  // Phi rows: 2 (FP64) or 4 (FP32) points per lane, clamped to the item's last ones
+  constexpr int PPL = 16 / (int)sizeof(S);
+  const int prow = sizeof(S) == 8 ? PPL * lane : PPL * (lane & 31);
+  const S *phi_pt = phi + it.p0 + min(prow, it.np - PPL);
+  auto dma = [&](int k0, char *st) __attribute__((always_inline)) {
+    long long kmap[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) kmap[r] = (long long)map[k0 + 4 * wave + r] * MD.nbf;
+    if constexpr (sizeof(S) == 8) {
+      // a 128-point row (1 KB) per load; wave w takes rows 4 w .. 4 w + 3
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int kk = 4 * wave + r;
+        __builtin_amdgcn_global_load_lds(phi_pt + (long long)(k0 + kk) * L.np,
+                                         st + kk * kTileAo * 8, 16, 0, 0);
+      }
+    } else {
+      // two 128-point rows (512 B each) per load: lanes 32-63 the second
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int kk = 4 * wave + 2 * r + (lane >> 5);
+        __builtin_amdgcn_global_load_lds(phi_pt + (long long)(k0 + kk) * L.np,
+                                         st + (4 * wave + 2 * r) * kTileAo * 4, 16, 0, 0);
+      }
+    }
+    char *sd = st + P::PHI;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int kk = 4 * wave + (q >> 2), cb = q & 3;
+      __builtin_amdgcn_global_load_lds(
+          reinterpret_cast<const char *>(Dm + colbase[cb] + kmap[q >> 2]) + 4 * (lane & 1),
+          sd + (kk * kTileAo + 32 * cb) * 8, 4, 0, 0);
+    }
+  };
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int nch = L.mp / kTKc;
+  for (int c = 0; c < P::NS - 1 && c < nch; ++c) dma(c * kTKc, s_stage + c * P::STAGE);
+  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+    constexpr int NR = decltype(R)::value, NC = decltype(C)::value;
+    int cur = 0;  // buffer of chunk c
+    for (int c = 0; c < nch; ++c) {
+      // chunk c's loads are complete in every wavefront (the newer chunks'
+      // may still run) and every wavefront is done with chunk c - 1, whose
+      // buffer the loads of chunk c + NS - 1 overwrite
+      if (P::NS == 3 && c + 1 < nch) vmcnt_barrier<P::LOADS>();
+      else vmcnt_barrier<0>();
+      const int nxt = cur == 0 ? P::NS - 1 : cur - 1;  // (c + NS - 1) % NS
+      if (c + P::NS - 1 < nch) dma((c + P::NS - 1) * kTKc, s_stage + nxt * P::STAGE);
+      const char *buf = s_stage + cur * P::STAGE;
+      const S *sp = reinterpret_cast<const S *>(buf);
+      const double *sd = reinterpret_cast<const double *>(buf + P::PHI);
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int kk = 4 * g + ks;
+        double xv[NR], yv[NC];
+#pragma unroll
+        for (int i = 0; i < NR; ++i) xv[i] = (double)sp[kk * kTileAo + wp * hp + 16 * i + pa];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) yv[j] = sd[kk * kTileAo + wc * hc + 16 * j + l16];
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+#pragma unroll
+          for (int j = 0; j < NC; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(xv[i], yv[j], acc[i][j], 0, 0, 0);
+      }
+      cur = cur + 1 == P::NS ? 0 : cur + 1;
+    }
+  });
+  den_tile_epilogue<S, GGA>(a, ao, it, L, acc, part, reinterpret_cast<double *>(s_stage), s_red);
+}
+
+template <typename S, bool GGA>
+__global__ __launch_bounds__(256, 2) void k_den_tile_dma(BuildArgs a, const S *__restrict__ ao,
+                                                         const int *__restrict__ ao_map,
+                                                         const DenItem *__restrict__ items,
+                                                         double *__restrict__ part) {
+  __shared__ __align__(16) char s_stage[DenDma<S>::NS * DenDma<S>::STAGE];
+  __shared__ double s_red[4];
+  const DenItem it = items[blockIdx.x];
+  if (it.leaf < 0) return;
+  den_tile_item_dma<S, GGA>(a, ao, ao_map, it, part, s_stage, s_red);
+}
+
 // PERSIST: the workgroups loop over the items blockIdx.x, blockIdx.x +
 // gridDim.x, ... (gridDim.x a multiple of 8, so every item of a workgroup is
 // from the same XCD's run of leaves) instead of one item each.
@@ -1693,8 +1868,9 @@ __global__ __launch_bounds__(256) void k_den_finish(BuildArgs a, const DenFinish
 // in registers.  Entry (mu, nu) is added to K(map[nu], map[mu]) with FP64
 // atomics; Vxc = K + K^T afterwards counts every entry of V_loc once.
 // RES (FP32 storage, context set-up): A = w r with r the value plane's
-// rounding residuals, so K + K^T = dS = sum_p w_p (r Phi^T + Phi r^T);
-// entries below kStorageCut are not added.
+// rounding residuals, so K + K^T = dS = sum_p w_p (r Phi^T + Phi r^T); only
+// the items of the leaves [res_leaf0, res_leaf1) whose residuals a.ao_res
+// holds run, and entries below kStorageCut are not added.
 template <typename S, bool GGA, bool RES = false>
 __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__restrict__ ao,
                                                      const VxcItem *__restrict__ items) {
@@ -1703,6 +1879,8 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
   __shared__ double s_c[2][4][kTVp];
   const VxcItem it = items[blockIdx.x];
   if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
+  if constexpr (RES)
+    if (it.leaf < a.res_leaf0 || it.leaf >= a.res_leaf1) return;
   const LeafDev L = a.leaves[it.leaf];
   const MolDev MD = a.mols[L.mol];
   double *Km = a.K + MD.moff;
@@ -1742,7 +1920,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
       const d4 f = load4_as<double>(phi + o);
       d4 A;
       if constexpr (RES) {
-        const d4 r = load4_as<double>(a.ao_res + L.res_off + o);
+        const d4 r = load4_as<double>(a.ao_res + (L.res_off - a.res_base) + o);
 #pragma unroll
         for (int e = 0; e < 4; ++e) A[e] = s_c[buf][0][p + e] * r[e];
       } else {
@@ -2266,6 +2444,8 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
     p.persistent_wgs = std::max(8, cus / 8 * 8);
     p.vxc_ws = (p.tune.persistent & 2) != 0;
     p.den_persistent = (p.tune.persistent & 1) != 0;
+    const char *dd = std::getenv("OCCXC_DENSITY_DMA");
+    p.den_dma = dd ? std::atoi(dd) != 0 : kDefaultDensityDma;
     for (const void *k : {reinterpret_cast<const void *>(&k_vxc_ws<double, true>),
                           reinterpret_cast<const void *>(&k_vxc_ws<double, false>),
                           reinterpret_cast<const void *>(&k_vxc_ws<float, true>),
@@ -2348,6 +2528,10 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
         (a.deriv ? k_den_tile<S, true, true> : k_den_tile<S, false, true>)
             <<<std::min(2 * plan.persistent_wgs, plan.n_den_items), 256, 0, s>>>(
                 a, x, plan.den_items.get(), plan.den_part.get(), plan.n_den_items);
+      else if (plan.den_dma)
+        (a.deriv ? k_den_tile_dma<S, true> : k_den_tile_dma<S, false>)
+            <<<plan.n_den_items, 256, 0, s>>>(a, x, a.ao_map, plan.den_items.get(),
+                                              plan.den_part.get());
       else
         (a.deriv ? k_den_tile<S, true> : k_den_tile<S, false>)
             <<<plan.n_den_items, 256, 0, s>>>(a, x, plan.den_items.get(), plan.den_part.get(),
@@ -2450,24 +2634,17 @@ void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, i
   check_launch("k_density_mfma");
 }
 
-// FP32 AO storage, context set-up: dS = sum_p w_p (r Phi^T + Phi r^T) over
-// every leaf with the tiled Vxc items (A = w r), K as the accumulator.
-void run_storage_overlap(const ContractPlan &plan, const BuildArgs &a, const void *ao, double *K,
-                         double *dS, int *active, const MolDev *mols, int nmol, int maxnbf,
+// FP32 AO storage, context set-up: adds sum_p w_p r Phi^T of the leaves
+// [a.res_leaf0, a.res_leaf1) (whose residuals a.ao_res holds) to a.K with
+// the tiled Vxc items (A = w r); every leaf must be marked active.
+void run_storage_overlap(const ContractPlan &plan, const BuildArgs &a, const void *ao,
                          hipStream_t s) {
   if (plan.vxc_items.get() == nullptr || a.ao_res == nullptr)
     throw Error(OCCXC_ERR_INTERNAL, "storage overlap: needs the Vxc item list and the residuals");
-  launch_fill_int(active, a.nleaves, 1, s);
-  launch_zero(K, a.mtot, s);
-  if (plan.n_vxc_items > 0) {
-    BuildArgs x = a;
-    x.leaf_active = active;
-    x.K = K;
-    k_vxc_tile<float, false, true><<<plan.n_vxc_items, 256, 0, s>>>(x, static_cast<const float *>(ao),
-                                                                    plan.vxc_items.get());
-    check_launch("k_vxc_tile (dS)");
-  }
-  launch_symmetrize(K, dS, mols, nmol, maxnbf, s);
+  if (plan.n_vxc_items == 0) return;
+  k_vxc_tile<float, false, true><<<plan.n_vxc_items, 256, 0, s>>>(a, static_cast<const float *>(ao),
+                                                                  plan.vxc_items.get());
+  check_launch("k_vxc_tile (dS)");
 }
 
 void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s) {

@@ -40,6 +40,7 @@ struct ContractPlan {
   TuneChoice tune;         // the launch settings in use
   bool vxc_ws = false;     // persistent wavefront-specialised Vxc instead of one item per workgroup
   bool den_persistent = false;  // persistent density launch (workgroups loop over items)
+  bool den_dma = false;   // density items staged by direct-to-LDS loads (k_den_tile_dma)
   int persistent_wgs = 256;  // workgroups of the persistent kernels (a multiple of 8)
   int n_den_items = 0, n_vxc_items = 0, n_den_finish = 0, n_den_parts = 0;
   DevBuf<DenItem> den_items;
@@ -78,10 +79,11 @@ void run_vxc_mgga(const ContractPlan &plan, const BuildArgs &a, const LeafDev *l
 // two-pass Vxc: V = sum over leaves of the stored K_loc blocks (+ transposes),
 // one workgroup per 32 x 32 tile pair, fixed summation order
 void run_vxc_reduce(const BuildArgs &a, double *V, hipStream_t s);
-// FP32 AO storage, at context set-up: dS = sum_p w_p (r Phi^T + Phi r^T) from
-// the value plane's rounding residuals a.ao_res (K is overwritten)
-void run_storage_overlap(const ContractPlan &plan, const BuildArgs &a, const void *ao, double *K,
-                         double *dS, int *active, const MolDev *mols, int nmol, int maxnbf,
+// FP32 AO storage, at context set-up: adds sum_p w_p r Phi^T of the leaves
+// [a.res_leaf0, a.res_leaf1) to a.K, from their rounding residuals a.ao_res
+// (all leaves marked active in a.leaf_active); dS = K + K^T once all leaves
+// are in
+void run_storage_overlap(const ContractPlan &plan, const BuildArgs &a, const void *ao,
                          hipStream_t s);
 
 } // namespace occxc

@@ -335,11 +335,12 @@ template <typename T> __device__ __forceinline__ T store_ao(double v, unsigned l
 // LMAX is the largest angular momentum the kernel handles: the register
 // allocation is that of the largest shell, so bases with l <= 2 run the
 // LMAX = 2 instance.
-// RES (FP32 storage, context set-up): also the value plane's rounding
-// residuals v - s into a.ao_res.
+// RES (FP32 storage, context set-up): instead of the stored values, the
+// value plane's rounding residuals v - s of the leaves res_leaf0 + blockIdx.x
+// into a.ao_res.
 template <typename T, int LMAX, bool RES = false>
 __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao) {
-  const LeafDev L = a.leaves[blockIdx.x];
+  const LeafDev L = a.leaves[RES ? a.res_leaf0 + blockIdx.x : blockIdx.x];
   // lanes over (point, shell split): a leaf of np points takes 256 / np
   // splits of its shells (np = 64: 4, 128: 2, 192 and 256: 1), split s taking
   // shells s, s + nsplit, ...; every wavefront lies in one split, so the
@@ -366,15 +367,17 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
       continue;
     }
     T *row = base + (long long)mu * L.np + p;
-    float *res = RES ? a.ao_res + L.res_off + (long long)mu * L.np + p : nullptr;
+    float *res = RES ? a.ao_res + (L.res_off - a.res_base) + (long long)mu * L.np + p : nullptr;
     const unsigned long long key0 = pkey + 0x2545F4914F6CDD1Dull * (unsigned long long)sh.bf0;
     auto sink = [&](int q, int m, double v) {
       const long long off = q * cstride + (long long)m * L.np;
       const T s = store_ao<T>(v, key0 + 0x2545F4914F6CDD1Dull * (unsigned long long)m +
                                      0x9E3779B97F4A7C15ull * (unsigned long long)q);
-      row[off] = s;
-      if constexpr (RES)
+      if constexpr (RES) {
         if (q == 0) res[(long long)m * L.np] = (float)(v - (double)s);
+      } else {
+        row[off] = s;
+      }
     };
     const double dx = x - sh.x, dy = y - sh.y, dz = z - sh.z;
     if (sh.l == 0) shell_point<0>(sh, a.exps, a.coefs, dx, dy, dz, deriv, sink);
@@ -386,9 +389,12 @@ __global__ __launch_bounds__(256) void k_ao_eval(BuildArgs a, T *__restrict__ ao
     mu += sh.size;
   }
   for (int r = L.m + split; r < L.mp; r += nsplit) {
-    for (int c = 0; c < a.ncomp; ++c)
-      base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
-    if constexpr (RES) a.ao_res[L.res_off + (long long)r * L.np + p] = 0.0f;
+    if constexpr (RES) {
+      a.ao_res[(L.res_off - a.res_base) + (long long)r * L.np + p] = 0.0f;
+    } else {
+      for (int c = 0; c < a.ncomp; ++c)
+        base[c * cstride + (long long)r * L.np + p] = static_cast<T>(0);
+    }
   }
 }
 
@@ -1102,16 +1108,23 @@ void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s) {
   if (a.nleaves == 0) return;
   const bool low = a.lmax <= 2;
-  if (fp32 && a.ao_res) {
-    if (low) k_ao_eval<float, 2, true><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
-    else k_ao_eval<float, kMaxL, true><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
-  } else if (fp32) {
+  if (fp32) {
     if (low) k_ao_eval<float, 2><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
     else k_ao_eval<float, kMaxL><<<a.nleaves, 256, 0, s>>>(a, static_cast<float *>(ao));
   } else {
     if (low) k_ao_eval<double, 2><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
     else k_ao_eval<double, kMaxL><<<a.nleaves, 256, 0, s>>>(a, static_cast<double *>(ao));
   }
+  check_launch();
+}
+
+void launch_ao_residuals(const BuildArgs &a, const void *ao, hipStream_t s) {
+  const int n = a.res_leaf1 - a.res_leaf0;
+  if (n <= 0) return;
+  if (a.ao_res == nullptr) throw std::runtime_error("launch_ao_residuals: no residual buffer");
+  float *x = const_cast<float *>(static_cast<const float *>(ao));  // not written with RES
+  if (a.lmax <= 2) k_ao_eval<float, 2, true><<<n, 256, 0, s>>>(a, x);
+  else k_ao_eval<float, kMaxL, true><<<n, 256, 0, s>>>(a, x);
   check_launch();
 }
 

@@ -153,9 +153,11 @@ struct BuildArgs {
   double *leaf_part;    // [2*nleaves] {Exc, N_elec} per leaf
   int *leaf_active;     // [nleaves] 0 when the leaf was skipped (max rho < dthr)
   // FP32 AO storage, during context set-up only: the value plane's rounding
-  // residuals r = exact value - stored float, [mp][np] per leaf at res_off
-  // (null otherwise)
+  // residuals r = exact value - stored float of the leaves [res_leaf0,
+  // res_leaf1), [mp][np] per leaf at res_off - res_base (null otherwise)
   float *ao_res;
+  int res_leaf0, res_leaf1;
+  long long res_base;
   double *kloc;         // two-pass Vxc: per-leaf K_loc blocks (null: atomics into K)
   const TilePairDev *tpairs;
   int ntpairs;
@@ -201,11 +203,18 @@ void launch_symmetrize(const double *K, double *out, const MolDev *mols, int nmo
 // the first-order change of sum_p w_p rho_p from the stored AO values to the
 // exact ones; partial holds kNcorrBlocks x nmol doubles
 constexpr int kNcorrBlocks = 1024;
+// residual floats formed at a time at context set-up (1 GiB; a larger leaf
+// block is taken whole)
+constexpr size_t kResidualChunkElems = (size_t)1 << 28;
 void launch_storage_correction(const double *D2, const double *dS, const MolDev *mols, int nmol,
                                int maxnbf, double *partial, double *result, hipStream_t s);
 void launch_prepare_density(const double *D, double *D2, const MolDev *mols, int nmol,
                             int maxnbf, hipStream_t s);
 void launch_ao_eval(const BuildArgs &a, void *ao, bool fp32, hipStream_t s);
+// FP32 storage, context set-up: the value plane's rounding residuals of the
+// leaves [a.res_leaf0, a.res_leaf1) into a.ao_res (the stored values are not
+// rewritten)
+void launch_ao_residuals(const BuildArgs &a, const void *ao, hipStream_t s);
 // XC gradient: the AO derivative components the cache lacks (FP64): the
 // Hessian (6 components) for a GGA context, the gradient (3) for an LDA one.
 void launch_ao_extra(const BuildArgs &a, double *out, bool hessian, hipStream_t s);

@@ -61,3 +61,28 @@ def test_settings_agree_on_a_batch():
         for i in range(len(mols)):
             assert np.abs(Vs[i] - first[0][i]).max() <= 1e-12, (tile, pers, i)
             assert abs(excs[i] - first[1][i]) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32cache", "fp32"])
+def test_density_staging_forms_agree(monkeypatch, precision):
+    """The density items staged by direct-to-LDS loads (the default) and by
+    registers (OCCXC_DENSITY_DMA=0) run the same MFMA k order, so they agree
+    to FP64 rounding (the Vxc atomics' order); both meet the oracle's
+    tolerances."""
+    atoms = molecules.water_cluster(6, seed=9)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 30, 95)
+    K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(basis.flat()), pts, wts, IDS, D)
+    out = []
+    for dma in ("1", "0"):
+        monkeypatch.setenv("OCCXC_DENSITY_DMA", dma)
+        for tile in (64, 128):
+            ctx = XCContext(basis, pts, wts, IDS, precision=precision, tile_ao=tile)
+            V, e, n = ctx.build(D)
+            ctx.close()
+            assert abs(e - exc) <= 1e-6 and abs(n - nel) <= 1e-8 and np.abs(V - K).max() <= 1e-7
+            out.append((dma, tile, V, e, n))
+    for dma, tile, V, e, n in out[2:]:
+        _, _, V0, e0, n0 = out[0 if tile == 64 else 1]
+        assert np.abs(V - V0).max() <= 1e-12 and abs(e - e0) <= 1e-12 and abs(n - n0) <= 1e-12, tile
