@@ -1500,34 +1500,37 @@ __device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *_
     for (int r = 0; r < 4; ++r)
 #pragma unroll
       for (int k = 0; k < 4; ++k) q[r][k] = 0.0;
+    // NJ column tiles' loads in flight before any is used, kept as stored
+    // (64 registers of loads: all four tiles with FP32 storage, two with FP64)
+    using V = typename Vec4Of<S>::type;
+    constexpr int NJ = sizeof(S) == 4 ? 4 : 2;
 #pragma unroll
-    for (int jj = 0; jj < 4; jj += 2) {
-      // both column tiles' loads in flight before either is used
-      d4 f[2], fx[2], fy[2], fz[2];
+    for (int jj = 0; jj < 4; jj += NJ) {
+      V f[NJ], fx[NJ], fy[NJ], fz[NJ];
 #pragma unroll
-      for (int jq = 0; jq < 2; ++jq) {
+      for (int jq = 0; jq < NJ; ++jq) {
         if (jj + jq >= CT) break;
         const long long off = (long long)(it.c0 + wc * hc + 16 * (jj + jq) + l16) * L.np + it.p0 + pl;
-        f[jq] = load4_as<double>(phi + off);
+        f[jq] = *reinterpret_cast<const V *>(phi + off);
         if constexpr (GGA) {
-          fx[jq] = load4_as<double>(phi + cs + off);
-          fy[jq] = load4_as<double>(phi + 2 * cs + off);
-          fz[jq] = load4_as<double>(phi + 3 * cs + off);
+          fx[jq] = *reinterpret_cast<const V *>(phi + cs + off);
+          fy[jq] = *reinterpret_cast<const V *>(phi + 2 * cs + off);
+          fz[jq] = *reinterpret_cast<const V *>(phi + 3 * cs + off);
         }
       }
 #pragma unroll
-      for (int jq = 0; jq < 2; ++jq) {
+      for (int jq = 0; jq < NJ; ++jq) {
         const int j = jj + jq;
         if (j >= CT) break;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) q[r][0] += f[jq][r] * acc[i][j][r];
+        for (int r = 0; r < 4; ++r) q[r][0] += (double)f[jq][r] * acc[i][j][r];
         if constexpr (GGA) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const double tt = 2.0 * acc[i][j][r];
-            q[r][1] += fx[jq][r] * tt;
-            q[r][2] += fy[jq][r] * tt;
-            q[r][3] += fz[jq][r] * tt;
+            q[r][1] += (double)fx[jq][r] * tt;
+            q[r][2] += (double)fy[jq][r] * tt;
+            q[r][3] += (double)fz[jq][r] * tt;
           }
         }
       }
