@@ -617,6 +617,9 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   if (opt->precision == 0) c->res_elems = 0;
   const size_t ao_bytes = c->ao_elems * c->ao_elem_size;
   const size_t ds_bytes = c->res_elems ? (size_t)c->mtot * 8 : 0;
+  // FP32 mode: the Vxc stage's A plane (as many floats as one AO plane)
+  const bool apass = vxc_apass_setting(opt->precision, c->mgga);
+  const size_t a_bytes = apass ? c->res_elems * sizeof(float) : 0;
   // device memory the context will hold: AO values, the nbf^2 matrices
   // (D, D2, K, V and the FP32 copies), per-point data and the leaf tables
   const bool two_pass = vxc_variant_setting() == 2;
@@ -624,13 +627,14 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   for (int m = 0; m < nmol; ++m)
     c->mol_bytes[m] = (c->res_elems ? 7.0 : 6.0) * 8.0 * c->mol_nbf[m] * c->mol_nbf[m];
   for (const LeafDev &L : c->leaves)
-    c->mol_bytes[L.mol] += (double)c->ncomp * L.mp * L.np * c->ao_elem_size + 64.0 * L.np +
+    c->mol_bytes[L.mol] += (double)c->ncomp * L.mp * L.np * c->ao_elem_size +
+                           (apass ? 4.0 * L.mp * L.np : 0.0) + 64.0 * L.np +
                            4.0 * (L.mp + L.nsig) + sizeof(LeafDev) +
                            (two_pass ? 8.0 * L.mp * L.mp : tile_plan_bytes(L));
   double tile_bytes = 0.0;
   if (!two_pass)
     for (const LeafDev &L : c->leaves) tile_bytes += tile_plan_bytes(L);
-  const size_t planned = ao_bytes + ds_bytes + (size_t)tile_bytes + (size_t)c->mtot * 8 * 6 +
+  const size_t planned = ao_bytes + ds_bytes + a_bytes + (size_t)tile_bytes + (size_t)c->mtot * 8 * 6 +
                          (size_t)c->ptot * 8 * 8 +
                          (c->mgga ? (size_t)c->ptot * 8 * 8 : 0) +
                          c->d_ao_map.bytes() + c->d_leaves.bytes() + c->d_leaf_shells.bytes() +
@@ -648,7 +652,8 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     for (const LeafDev &L : c->leaves) max_leaf = std::max(max_leaf, (size_t)L.mp * L.np);
     res_cap = std::min(c->res_elems, std::max(kResidualChunkElems, max_leaf));
   }
-  const size_t setup_bytes = ao_bytes + ds_bytes + res_cap * sizeof(float);
+  // the residuals go before the A plane comes
+  const size_t setup_bytes = ao_bytes + ds_bytes + std::max(res_cap * sizeof(float), a_bytes);
   OCCXC_REQUIRE(setup_bytes + (size_t)6 * c->mtot * 8 < free_b, OCCXC_ERR_OUT_OF_MEMORY,
                 "resident AO values need " + std::to_string(setup_bytes >> 20) + " MiB, " +
                     std::to_string(free_b >> 20) + " MiB free");
@@ -696,6 +701,7 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
   a.leaf_part = c->d_part.get();
   a.leaf_active = c->d_active.get();
   a.ao_res = nullptr;  // set only while dS is formed
+  a.ao_a = nullptr;    // set once the A plane is allocated
 
   c->plan = make_contract_plan(c->leaves, c->mtot, c->deriv, opt->precision, c->stream,
                                tune_choice(opt, c->maxnbf));
@@ -821,6 +827,11 @@ void context_setup(occxc_context *c, int nmol, const occxc_basis *bases,
     c->device_bytes += c->d_ncorr.bytes();
     OCCXC_HIP(hipStreamSynchronize(c->stream));
     c->d_ao_res.release();
+  }
+  if (apass && c->plan.tiled) {
+    c->d_ao_a.alloc(c->res_elems);
+    a.ao_a = c->d_ao_a.get();
+    c->device_bytes += c->d_ao_a.bytes();
   }
   OCCXC_HIP(hipStreamSynchronize(c->stream));
   c->ao_ready = true;

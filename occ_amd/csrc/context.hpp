@@ -41,6 +41,7 @@ struct occxc_context {
   // dS = sum_p w_p (r Phi^T + Phi r^T), the first-order change of the grid
   // overlap from the stored values to the exact ones
   occxc::DevBuf<float> d_ao_res;
+  occxc::DevBuf<float> d_ao_a;  // FP32 mode: the Vxc stage's A plane (A pass)
   occxc::DevBuf<double> d_dS, d_ncorr;
   occxc::DevBuf<double> d_D, d_D2, d_K, d_V, d_coef, d_part, d_result;
   occxc::ContractPlan plan;

@@ -1862,6 +1862,130 @@ __global__ __launch_bounds__(256) void k_den_finish(BuildArgs a, const DenFinish
   density_epilogue(a, L, F.leaf, s_q, s_red);
 }
 
+// Scatter of a Vxc item's block into K with FP64 atomics (entries below cut
+// dropped).  acc[i][j][r] of wavefront w = (wr, wcw) = (w % 2, w / 2) holds
+// the block entry (row, col) = (wr hr + lr, wcw hc + 16 j + l16), lr = 16 i +
+// 4 g + r (F32: the v_mfma_f32 accumulator rows) or 16 i + g + 4 r (FP64).
+// Without FOLD, or for an off-diagonal item, every entry is added.  With FOLD
+// a diagonal item (r0 == c0) adds only the entries row <= col: B(row, col) +
+// B(col, row) above the diagonal and B(row, row) on it; Vxc = K + K^T is
+// unchanged (the off-diagonal items also add above K's diagonal only: the AO
+// map is ascending) and the item's atomics halve.  The transposes pass
+// through s_t (2 x 64 x 64 doubles, the chunk buffers, free after the first
+// barrier): wavefront 1 (the lower-left quadrant) hands its entries to
+// wavefront 2 (upper-right), and the two diagonal quadrants (wavefronts 0 and
+// 3) each transpose their own.  The fold pays where the atomics are exposed
+// (k_vxc_gemm_f32: 2.5 ms of 23.8 on (H2O)128 / def2-SVP); in the
+// register-staged kernels they drain behind other workgroups' staging and
+// the fold's barriers cost more than it saves.
+template <bool F32, bool FOLD>
+__device__ __forceinline__ void vxc_scatter(const d4 (&acc)[4][4], const VxcItem &it,
+                                            const LeafDev &L, const MolDev &MD,
+                                            const int *__restrict__ map, double *Km, double cut,
+                                            double *s_t) {
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wr = wave & 1, wcw = wave >> 1;
+  const int hr = it.wr >> 1, hc = it.wc >> 1;
+  const int RT = hr >> 4, CT = hc >> 4;
+  auto lrow = [&](int i, int r) { return 16 * i + (F32 ? 4 * g + r : g + 4 * r); };
+  if (!FOLD || it.r0 != it.c0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j >= CT) break;
+      const int nu = it.c0 + wcw * hc + 16 * j + l16;
+      if (nu >= L.m) continue;
+      double *kc = Km + map[nu];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (i >= RT) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int mu = it.r0 + wr * hr + lrow(i, r);
+          if (mu < L.m && fabs(acc[i][j][r]) >= cut)
+            atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+        }
+      }
+    }
+    return;
+  }
+  // K offsets of the entries' rows and columns, loaded once (-1: past the
+  // leaf's AOs)
+  long long ro[4][4];
+  int co[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int mu = it.r0 + wr * hr + lrow(i, r);
+      ro[i][r] = i < RT && mu < L.m ? (long long)map[mu] * MD.nbf : -1;
+    }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int nu = it.c0 + wcw * hc + 16 * j + l16;
+    co[j] = j < CT && nu < L.m ? map[nu] : -1;
+  }
+  auto emit = [&](int i, int j, int r, double v) __attribute__((always_inline)) {
+    if (ro[i][r] >= 0 && co[j] >= 0 && fabs(v) >= cut) atomicAdd(Km + ro[i][r] + co[j], v);
+  };
+  // [row][col ^ (row % 16)] keeps a wavefront's column reads off one bank
+  auto tix = [](int row, int col) { return row * 64 + (col ^ (row & 15)); };
+  auto put = [&](double *T) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= RT) break;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j >= CT) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) T[tix(lrow(i, r), 16 * j + l16)] = acc[i][j][r];
+      }
+    }
+  };
+  // upper triangle of a diagonal quadrant from its own transpose T
+  auto fold_diag = [&](const double *T) __attribute__((always_inline)) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= RT) break;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j >= CT) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int lr = lrow(i, r), lc = 16 * j + l16;
+          if (lr < lc) emit(i, j, r, acc[i][j][r] + T[tix(lc, lr)]);
+          else if (lr == lc) emit(i, j, r, acc[i][j][r]);
+        }
+      }
+    }
+  };
+  double *R0 = s_t, *R1 = s_t + 64 * 64;
+  __syncthreads();  // every wavefront is done with the chunk buffers
+  if (wave == 1) put(R0);
+  if (wave == 0) {
+    put(R1);
+    fold_diag(R1);
+  }
+  __syncthreads();
+  if (wave == 2) {
+    // upper-right quadrant entry (lr, lc) + lower-left entry (lc, lr)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= RT) break;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j >= CT) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) emit(i, j, r, acc[i][j][r] + R0[tix(16 * j + l16, lrow(i, r))]);
+      }
+    }
+  }
+  if (wave == 3) {
+    put(R1);
+    fold_diag(R1);
+  }
+}
+
 // Vxc item, block pair (r, c) of a leaf: the leaf's points are streamed kTVp
 // at a time; each chunk stages Phi and A = c0 Phi + c . grad Phi
 // (xc_potential_matrix.cpp:19-41) of the block rows in LDS ([row][point]),
@@ -1877,8 +2001,11 @@ __global__ __launch_bounds__(256) void k_den_finish(BuildArgs a, const DenFinish
 template <typename S, bool GGA, bool RES = false>
 __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__restrict__ ao,
                                                      const VxcItem *__restrict__ items) {
-  __shared__ double s_pr[kTileAo * kTVld], s_ar[kTileAo * kTVld];
-  __shared__ double s_pc[kTileAo * kTVld], s_ac[kTileAo * kTVld];
+  // the four operand buffers in one array: the scatter's transposes reuse it
+  __shared__ __align__(16) double s_stage[4 * kTileAo * kTVld];
+  static_assert(4 * kTileAo * kTVld >= 2 * 64 * 64, "scatter transposes fit");
+  // buffer offsets: Phi rows 0, A rows kB, Phi columns 2 kB, A columns 3 kB
+  constexpr int kB = kTileAo * kTVld;
   __shared__ double s_c[2][4][kTVp];
   const VxcItem it = items[blockIdx.x];
   if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
@@ -1939,40 +2066,23 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
           A[e] += s_c[buf][1][p + e] * fx[e] + s_c[buf][2][p + e] * fy[e] + s_c[buf][3][p + e] * fz[e];
       }
       const int lr = rb ? row : row - it.wr;
-      *reinterpret_cast<d4u *>((rb ? s_pr : s_pc) + lr * kTVld + p) = f;
-      *reinterpret_cast<d4u *>((rb ? s_ar : s_ac) + lr * kTVld + p) = A;
+      *reinterpret_cast<d4u *>(s_stage + (rb ? 0 : 2 * kB) + lr * kTVld + p) = f;
+      *reinterpret_cast<d4u *>(s_stage + (rb ? kB : 3 * kB) + lr * kTVld + p) = A;
     }
     __syncthreads();
     // two k-steps (points) per operand load keeps the operands at 32
     // registers next to the 128 of the accumulators
-    auto product = [&](const double *xa, const double *yb) __attribute__((always_inline)) {
+    auto product = [&](int xa, int yb) __attribute__((always_inline)) {
 #pragma unroll
       for (int sp = 0; sp < 4; sp += 2)
         mfma_f64_k2<decltype(R)::value, decltype(C)::value, kTVld>(
-            acc, xa + xo + sp, yb + yo + sp);
+            acc, s_stage + xa + xo + sp, s_stage + yb + yo + sp);
     };
-    product(s_pr, diag ? s_ar : s_ac);
-    if (!diag) product(s_ar, s_pc);
+    product(0, diag ? kB : 3 * kB);
+    if (!diag) product(kB, 2 * kB);
   }
   });
-  // acc[i][j][r] = (mu = r0 + wr*hr + 16 i + g + 4 r, nu = c0 + wcw*hc + 16 j + l16)
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (j >= CT) break;
-    const int nu = it.c0 + wcw * hc + 16 * j + l16;
-    if (nu >= L.m) continue;
-    double *kc = Km + map[nu];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (i >= RT) break;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int mu = it.r0 + wr * hr + 16 * i + g + 4 * r;
-        if (mu < L.m && fabs(acc[i][j][r]) >= (RES ? kStorageCut : kScatterCut))
-          atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
-      }
-    }
-  }
+  vxc_scatter<false, false>(acc, it, L, MD, map, Km, RES ? kStorageCut : kScatterCut, s_stage);
 }
 
 // FP32 form of the Vxc item (FP32 mode): the same block pairs with Phi and
@@ -1987,8 +2097,11 @@ constexpr int kTVldf = kTVpf + 4;  // floats per LDS row (144 bytes)
 template <bool GGA>
 __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const float *__restrict__ ao,
                                                          const VxcItem *__restrict__ items) {
-  __shared__ float s_pr[kTileAo * kTVldf], s_ar[kTileAo * kTVldf];
-  __shared__ float s_pc[kTileAo * kTVldf], s_ac[kTileAo * kTVldf];
+  // the four operand buffers in one array: the scatter's transposes reuse it
+  __shared__ __align__(16) float s_stage[4 * kTileAo * kTVldf];
+  static_assert(4 * kTileAo * kTVldf * 4 >= 2 * 64 * 64 * 8, "scatter transposes fit");
+  // buffer offsets: Phi rows 0, A rows kB, Phi columns 2 kB, A columns 3 kB
+  constexpr int kB = kTileAo * kTVldf;
   __shared__ double s_c[2][4][kTVpf];
   const VxcItem it = items[blockIdx.x];
   if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
@@ -2045,8 +2158,8 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
         A[e] = (float)v;
       }
       const int lr = rb ? row : row - it.wr;
-      *reinterpret_cast<f4 *>((rb ? s_pr : s_pc) + lr * kTVldf + p) = f;
-      *reinterpret_cast<f4 *>((rb ? s_ar : s_ac) + lr * kTVldf + p) = A;
+      *reinterpret_cast<f4 *>(s_stage + (rb ? 0 : 2 * kB) + lr * kTVldf + p) = f;
+      *reinterpret_cast<f4 *>(s_stage + (rb ? kB : 3 * kB) + lr * kTVldf + p) = A;
     }
     __syncthreads();
     f4 acf[4][4];
@@ -2054,14 +2167,14 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acf[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    auto product = [&](const float *xa, const float *yb) __attribute__((always_inline)) {
+    auto product = [&](int xa, int yb) __attribute__((always_inline)) {
 #pragma unroll
       for (int ks = 0; ks < kTVpf; ks += 16)
         mfma_f32_k4<decltype(R)::value, decltype(C)::value, kTVldf>(
-            acf, xa + xo + ks, yb + yo + ks);
+            acf, s_stage + xa + xo + ks, s_stage + yb + yo + ks);
     };
-    product(s_pr, diag ? s_ar : s_ac);
-    if (!diag) product(s_ar, s_pc);
+    product(0, diag ? kB : 3 * kB);
+    if (!diag) product(kB, 2 * kB);
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -2070,24 +2183,160 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
         for (int r = 0; r < 4; ++r) acc[i][j][r] += (double)acf[i][j][r];
   }
   });
-  // acc[i][j][r] = (mu = r0 + wr*hr + 16 i + 4 g + r, nu = c0 + wcw*hc + 16 j + l16)
+  vxc_scatter<true, false>(acc, it, L, MD, map, Km, kScatterCut, reinterpret_cast<double *>(s_stage));
+}
+
+// FP32 mode, first step of the Vxc stage: A = sum_q c_q Phi_q of every active
+// leaf (the arithmetic of k_vxc_tile_f32's staging: FP64 sums of the FP32
+// values, rounded to FP32) into a.ao_a at the leaf's res_off, for all mp rows
+// and np points.  One workgroup per leaf; np / 4 lanes per AO row (4 points
+// each), so a lane's coefficients are fixed.  HBM-bound: 16 bytes read and 4
+// written per element (GGA), both streamed past the caches (non-temporal:
+// 0.4 ms of 6 on (H2O)128 / def2-SVP).
+template <bool GGA>
+__global__ __launch_bounds__(256) void k_vxc_apass(BuildArgs a, const float *__restrict__ ao) {
+  const int li = blockIdx.x;
+  if (!a.leaf_active[li]) return;
+  const LeafDev L = a.leaves[li];
+  const int tpr = L.np >> 2, rpp = 256 / tpr;  // lanes per row, rows per pass
+  const int t = threadIdx.x, row0 = t / tpr;
+  if (row0 >= rpp) return;
+  const int p = 4 * (t - row0 * tpr);
+  constexpr int NQ = GGA ? 4 : 1;
+  double c[NQ][4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (j >= CT) break;
-    const int nu = it.c0 + wcw * hc + 16 * j + l16;
-    if (nu >= L.m) continue;
-    double *kc = Km + map[nu];
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (i >= RT) break;
+    for (int e = 0; e < 4; ++e) c[q][e] = a.coef[q * a.ptot + L.pt_off + p + e];
+  const float *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  float *out = a.ao_a + L.res_off;
+#pragma unroll 2
+  for (int mu = row0; mu < L.mp; mu += rpp) {
+    const long long o = (long long)mu * L.np + p;
+    const f4 f = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(phi + o));
+    f4 fx, fy, fz;
+    if constexpr (GGA) {
+      fx = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(phi + cs + o));
+      fy = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(phi + 2 * cs + o));
+      fz = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(phi + 3 * cs + o));
+    }
+    f4 A;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int mu = it.r0 + wr * hr + 16 * i + 4 * g + r;
-        if (mu < L.m && fabs(acc[i][j][r]) >= kScatterCut)
-          atomicAdd(kc + (long long)map[mu] * MD.nbf, acc[i][j][r]);
+    for (int e = 0; e < 4; ++e) {
+      double v = c[0][e] * (double)f[e];
+      if constexpr (GGA)
+        v += c[1][e] * (double)fx[e] + c[2][e] * (double)fy[e] + c[3][e] * (double)fz[e];
+      A[e] = (float)v;
+    }
+    __builtin_nontemporal_store(A, reinterpret_cast<f4 *>(out + o));
+  }
+}
+
+// FP32 mode, second step: the Vxc item's block pair from the Phi and A planes
+// (k_vxc_apass), staged into LDS by direct loads (global_load_lds, 16 bytes
+// a lane) in chunks of kVgP points, two chunk buffers, so the next chunk
+// loads while the MFMAs run on this one and no registers hold staged values.  A
+// buffer holds four operand blocks [Phi rows | A rows | Phi columns | A
+// columns] of kTileAo AO rows x kVgP points, 64 bytes a row with no padding:
+// the 16-byte column slot of 4-point group q in row R is q ^ ((R / 2) % 4).
+// With that swizzle a wavefront's ds_read_b128 of an MFMA operand (lane l:
+// row l % 16, group l / 16) runs at the LDS's full rate on gfx950 (measured
+// about 4.8 cycles a wavefront; padded rows of 144 or 80 bytes, or other
+// swizzles, take about 8.5 and read as bank conflicts).
+// The products and the FP64 folding (every 32 points) are those of
+// k_vxc_tile_f32, so are the accumulator layout and the scatter.
+constexpr int kVgP = 16;                       // points per chunk
+constexpr int kVgBlock = kTileAo * kVgP * 4;   // bytes of one operand block
+constexpr int kVgStage = 4 * kVgBlock;         // bytes of one chunk buffer
+
+__device__ __forceinline__ int vxc_swz(int r) { return (r >> 1) & 3; }
+
+__global__ __launch_bounds__(256, 2) void k_vxc_gemm_f32(BuildArgs a, const float *__restrict__ ao,
+                                                         const int *__restrict__ ao_map,
+                                                         const VxcItem *__restrict__ items) {
+  constexpr int NS = 2;  // chunk buffers (three, one workgroup per CU, measured slower)
+  __shared__ __align__(16) char s_stage[NS * kVgStage];
+  const VxcItem it = items[blockIdx.x];
+  if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
+  const LeafDev L = a.leaves[it.leaf];
+  const MolDev MD = a.mols[L.mol];
+  double *Km = a.K + MD.moff;
+  const int *map = ao_map + L.map_off;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wr = wave & 1, wcw = wave >> 1;
+  const bool diag = it.r0 == it.c0;
+  const int hr = it.wr >> 1, hc = it.wc >> 1;
+  const int RT = hr >> 4, CT = hc >> 4;
+  const float *phi = ao + L.ao_off;
+  const float *am = a.ao_a + L.res_off;
+  // loads: wavefront w fills 16-row groups w and w + 4 of each block, lane l
+  // row 16 group + l / 4 (clamped to the block's last row: rows past it are
+  // read only by tiles whose results are dropped) and slot l % 4; the Phi and
+  // A planes share the [mp][np] layout, so one offset serves both
+  const int lrow = lane >> 2, lcol = 4 * ((lane & 3) ^ vxc_swz(lrow));
+  int off[2][2];  // [row block, column block][group]
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int rb = h ? it.c0 : it.r0, wb = h ? it.wc : it.wr;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) off[h][k] = (rb + min(16 * (wave + 4 * k) + lrow, wb - 1)) * L.np + lcol;
+  }
+  auto dma = [&](int p0, char *st) __attribute__((always_inline)) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && diag) break;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        char *dst = st + 2 * h * kVgBlock + (wave + 4 * k) * 1024;
+        __builtin_amdgcn_global_load_lds(phi + off[h][k] + p0, dst, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(am + off[h][k] + p0, dst + kVgBlock, 16, 0, 0);
       }
     }
-  }
+  };
+  const int nch = L.np / kVgP;  // np is a multiple of 64: an even count
+  if (nch > 0) dma(0, s_stage);
+  d4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  const int sw = 4 * (g ^ vxc_swz(l16));
+  const int xo = (wr * hr + l16) * kVgP + sw, yo = (wcw * hc + l16) * kVgP + sw;
+  tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+    constexpr int NR = decltype(R)::value, NC = decltype(C)::value;
+    f4 acf[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acf[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    int cur = 0;
+    for (int c = 0; c < nch; ++c) {
+      // chunk c's loads are complete in every wavefront and every wavefront
+      // is done with chunk c - 1, whose buffer chunk c + 1 fills
+      vmcnt_barrier<0>();
+      if (c + 1 < nch) dma((c + 1) * kVgP, s_stage + (cur ^ 1) * kVgStage);
+      const float *pr = reinterpret_cast<const float *>(s_stage + cur * kVgStage);
+      const float *ar = pr + kTileAo * kVgP, *pc = ar + kTileAo * kVgP, *ac = pc + kTileAo * kVgP;
+      mfma_f32_k4<NR, NC, kVgP>(acf, pr + xo, (diag ? ar : ac) + yo);
+      if (!diag) mfma_f32_k4<NR, NC, kVgP>(acf, ar + xo, pc + yo);
+      if (c & 1) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+#pragma unroll
+          for (int j = 0; j < NC; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              acc[i][j][r] += (double)acf[i][j][r];
+              acf[i][j][r] = 0.f;
+            }
+      }
+      cur ^= 1;
+    }
+  });
+  vxc_scatter<true, true>(acc, it, L, MD, map, Km, kScatterCut, reinterpret_cast<double *>(s_stage));
 }
 
 // Persistent, wavefront-specialised form of the Vxc items: one 512-thread
@@ -2414,6 +2663,19 @@ int vxc_variant_setting() {
   return vv ? std::atoi(vv) : 0;
 }
 
+bool tiled_setting(int precision) {
+  const bool use_mfma = std::getenv("OCCXC_REFERENCE_KERNELS") == nullptr;
+  const bool two_pass = vxc_variant_setting() == 2 && precision != 2;
+  const char *tv = std::getenv("OCCXC_TILED");
+  return use_mfma && (precision == 0 || precision == 1 || precision == 3) && !two_pass &&
+         !(tv && std::atoi(tv) == 0);
+}
+
+bool vxc_apass_setting(int precision, bool mgga) {
+  const char *av = std::getenv("OCCXC_VXC_APASS");
+  return precision == 1 && !mgga && tiled_setting(precision) && !(av && std::atoi(av) == 0);
+}
+
 ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
                                 int precision, hipStream_t s, const TuneChoice &tune) {
   (void)deriv;
@@ -2436,9 +2698,7 @@ ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mt
   // or (2) as 0 with the K_loc blocks stored and reduced in a second pass
   p.vxc_variant = vxc_variant_setting();
   p.two_pass = p.vxc_variant == 2 && precision != 2;  // not with the split-BF16 Vxc
-  const char *tv = std::getenv("OCCXC_TILED");
-  p.tiled = p.use_mfma && (precision == 0 || precision == 1 || precision == 3) && !p.two_pass &&
-            !(tv && std::atoi(tv) == 0);
+  p.tiled = tiled_setting(precision);
   if (p.tiled) {
     make_tile_plan(p, leaves);
     int dev = 0, cus = 0;
@@ -2665,6 +2925,16 @@ void run_vxc(const ContractPlan &plan, const BuildArgs &a, const void *ao, int p
   check_coef_planes(a.deriv ? 4 : 1, a.coef_planes, "Vxc stage");
   if (a.nleaves == 0) return;
   if (plan.tiled) {
+    if (precision == 1 && a.ao_a != nullptr) {
+      // FP32 mode with the A plane: A pass, then the directly loaded contraction
+      (a.deriv ? k_vxc_apass<true> : k_vxc_apass<false>)
+          <<<a.nleaves, 256, 0, s>>>(a, static_cast<const float *>(ao));
+      check_launch("k_vxc_apass");
+      k_vxc_gemm_f32<<<plan.n_vxc_items, 256, 0, s>>>(a, static_cast<const float *>(ao), a.ao_map,
+                                                      plan.vxc_items.get());
+      check_launch("k_vxc_gemm_f32");
+      return;
+    }
     if (precision == 1) {
       // FP32 mode: FP32 Vxc contraction (the persistent form is FP64 only)
       (a.deriv ? k_vxc_tile_f32<true> : k_vxc_tile_f32<false>)

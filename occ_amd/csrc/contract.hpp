@@ -59,6 +59,17 @@ double tile_plan_bytes(const LeafDev &L);
 // Vxc kernel selection (OCCXC_VXC_VARIANT, default 0)
 int vxc_variant_setting();
 
+// Whether a context of this precision takes the tiled contractions (the
+// plan's tiled flag, from the same settings).
+bool tiled_setting(int precision);
+
+// FP32 mode (precision 1), tiled, not meta-GGA: the Vxc stage first forms
+// A = sum_q c_q Phi_q (FP32) of every active leaf in a plane of its own
+// (4 mp np bytes per leaf, held by the context), then contracts Phi with A
+// from LDS filled by direct loads.  OCCXC_VXC_APASS=0 keeps the one-kernel
+// form, which forms A per block pair.
+bool vxc_apass_setting(int precision, bool mgga);
+
 ContractPlan make_contract_plan(const std::vector<LeafDev> &leaves, long long mtot, int deriv,
                                 int precision, hipStream_t s, const TuneChoice &tune = {});
 void run_density(const ContractPlan &plan, const BuildArgs &a, const void *ao, int precision,

@@ -158,6 +158,9 @@ struct BuildArgs {
   float *ao_res;
   int res_leaf0, res_leaf1;
   long long res_base;
+  // FP32 mode with the A pass: A = sum_q c_q Phi_q of every active leaf,
+  // [mp][np] per leaf at res_off (null otherwise)
+  float *ao_a;
   double *kloc;         // two-pass Vxc: per-leaf K_loc blocks (null: atomics into K)
   const TilePairDev *tpairs;
   int ntpairs;

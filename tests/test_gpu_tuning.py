@@ -10,7 +10,7 @@ import pytest
 import oracle
 from conftest import random_density
 from occ_amd import AOBasis, molecules
-from occ_amd.dft import XCBatch, XCContext
+from occ_amd.dft import XCBatch, XCContext, functional_ids
 from occ_amd.grid import GridSettings, MolecularGrid
 
 pytestmark = pytest.mark.gpu
@@ -86,3 +86,35 @@ def test_density_staging_forms_agree(monkeypatch, precision):
     for dma, tile, V, e, n in out[2:]:
         _, _, V0, e0, n0 = out[0 if tile == 64 else 1]
         assert np.abs(V - V0).max() <= 1e-12 and abs(e - e0) <= 1e-12 and abs(n - n0) <= 1e-12, tile
+
+
+@pytest.mark.parametrize("deriv_ids", ["gga", "lda"])
+def test_fp32_vxc_forms_agree(monkeypatch, deriv_ids):
+    """FP32 mode's Vxc stage in its two forms: A = sum_q c_q Phi_q formed once
+    per leaf (k_vxc_apass) and contracted from directly loaded LDS with the
+    diagonal blocks folded before the scatter (the default), and the
+    one-kernel form that forms A per block pair (OCCXC_VXC_APASS=0).  Both
+    round A to FP32 the same way and sum 32 points at a time in FP32, in a
+    different order, so they agree far inside the mode's Vxc error against
+    FP64 (about 4e-8 on (H2O)128); both meet the oracle's tolerances, and Exc
+    and N_elec, which the Vxc stage does not touch, agree to rounding."""
+    ids = IDS if deriv_ids == "gga" else functional_ids("svwn5")
+    atoms = molecules.water_cluster(6, seed=4)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    D = random_density(basis.nbf, 30, 41)
+    K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(basis.flat()), pts, wts, ids, D)
+    out = []
+    for apass in ("1", "0"):
+        monkeypatch.setenv("OCCXC_VXC_APASS", apass)
+        for tile in (64, 128):
+            ctx = XCContext(basis, pts, wts, ids, precision="fp32", tile_ao=tile)
+            V, e, n = ctx.build(D)
+            ctx.close()
+            assert abs(e - exc) <= 1e-6 and abs(n - nel) <= 1e-8 and np.abs(V - K).max() <= 1e-7
+            assert np.abs(V - V.T).max() == 0.0
+            out.append((apass, tile, V, e, n))
+    for apass, tile, V, e, n in out[2:]:
+        _, _, V0, e0, n0 = out[0 if tile == 64 else 1]
+        assert np.abs(V - V0).max() <= 1e-8, tile
+        assert abs(e - e0) <= 1e-12 and abs(n - n0) <= 1e-12, tile
