@@ -1806,8 +1806,12 @@ __global__ __launch_bounds__(256, 2) void k_den_tile_dma(BuildArgs a, const S *_
                                                          const int *__restrict__ ao_map,
                                                          const DenItem *__restrict__ items,
                                                          double *__restrict__ part) {
-  __shared__ __align__(16) char s_stage[DenDma<S>::NS * DenDma<S>::STAGE];
-  __shared__ double s_red[4];
+  // one LDS array (the reduction slots after the chunk buffers): with a
+  // second __shared__ variable the LDS accesses carry alias scopes, and the
+  // compiler then waits for every direct load in flight before each chunk's
+  // LDS reads, which serialises the load pipeline
+  __shared__ __align__(16) char s_stage[DenDma<S>::NS * DenDma<S>::STAGE + 4 * sizeof(double)];
+  double *s_red = reinterpret_cast<double *>(s_stage + DenDma<S>::NS * DenDma<S>::STAGE);
   const DenItem it = items[blockIdx.x];
   if (it.leaf < 0) return;
   den_tile_item_dma<S, GGA>(a, ao, ao_map, it, part, s_stage, s_red);
