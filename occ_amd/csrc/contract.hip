@@ -104,6 +104,48 @@ __device__ __forceinline__ double sum16(double v) {
   return v;
 }
 
+// DPP moves within each 16-lane row: mirror (l -> 15 - l), half mirror
+// (l -> l ^ 7) and the quad permutations l -> l ^ 2, l -> l ^ 1.  Each pairs
+// lanes, and together they flip bits 3, 2, 1 and 0 of the lane in turn.
+constexpr int kDppRowMirror = 0x140, kDppRowHalfMirror = 0x141, kDppXor2 = 0x4E, kDppXor1 = 0xB1;
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// One stage of a transpose-reduce over the first N of v: the lane keeps one
+// half (the upper if hi) and adds its DPP partner's copy of that half, which
+// the partner (hi flipped) sends.
+template <int CTRL, int N, int M>
+__device__ __forceinline__ void tr_stage(double (&v)[M], bool hi) {
+#pragma unroll
+  for (int i = 0; i < N / 2; ++i) {
+    const double keep = hi ? v[i + N / 2] : v[i], send = hi ? v[i] : v[i + N / 2];
+    v[i] = keep + dpp_f64<CTRL>(send);
+  }
+}
+// Sums over the 16 lanes of each row of 16 values per lane in 15 DPP
+// exchanges (16 sum16 calls take 64 shuffles): lane l16 returns the total of
+// value l16.
+__device__ __forceinline__ double row_reduce16(double (&v)[16], int l16) {
+  tr_stage<kDppRowMirror, 16>(v, (l16 >> 3) & 1);
+  tr_stage<kDppRowHalfMirror, 8>(v, (l16 >> 2) & 1);
+  tr_stage<kDppXor2, 4>(v, (l16 >> 1) & 1);
+  tr_stage<kDppXor1, 2>(v, l16 & 1);
+  return v[0];
+}
+// The same for 4 values per lane: lane l16 returns the total of value l16 / 4.
+__device__ __forceinline__ double row_reduce4(double (&v)[4], int l16) {
+  tr_stage<kDppRowMirror, 4>(v, (l16 >> 3) & 1);
+  tr_stage<kDppRowHalfMirror, 2>(v, (l16 >> 2) & 1);
+  double x = v[0];
+  x += dpp_f64<kDppXor2>(x);
+  x += dpp_f64<kDppXor1>(x);
+  return x;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
@@ -1535,16 +1577,25 @@ __device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *_
         }
       }
     }
+    // the column sums: lane l16 of a row ends with point pl + l16 / 4's
+    // quantity l16 % 4 (GGA) or point pl + l16 / 4's rho (LDA)
+    if constexpr (GGA) {
+      double v16[16];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int k = 0; k < (GGA ? 4 : 1); ++k) {
-        const double v = sum16(q[r][k]);
-        if (l16 == 0) {
-          if (wc == 0) s_q[k][pl + r] = v;
-          else s_h[k][pl + r] = v;
-        }
+        for (int k = 0; k < 4; ++k) v16[4 * r + k] = q[r][k];
+      const double v = row_reduce16(v16, l16);
+      if (wc == 0) s_q[l16 & 3][pl + (l16 >> 2)] = v;
+      else s_h[l16 & 3][pl + (l16 >> 2)] = v;
+    } else {
+      double v4[4] = {q[0][0], q[1][0], q[2][0], q[3][0]};
+      const double v = row_reduce4(v4, l16);
+      if ((l16 & 3) == 0) {
+        if (wc == 0) s_q[0][pl + (l16 >> 2)] = v;
+        else s_h[0][pl + (l16 >> 2)] = v;
       }
+    }
   }
   __syncthreads();
   const int nq = GGA ? 4 : 1;
