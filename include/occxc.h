@@ -66,9 +66,11 @@ enum {
 typedef enum {
   OCCXC_PREC_FP64 = 0,       /* FP64 AO cache and contractions */
   OCCXC_PREC_FP32 = 1,       /* FP32 AO cache; density contraction in FP64; Vxc contraction in
-                                FP32 with FP32 accumulation over 32 points, FP64 beyond.  The
-                                FP32-cache modes add a first-order storage-rounding correction
-                                to N_elec (formed at context creation) */
+                                FP32 with FP32 accumulation over 32 points, FP64 beyond, from
+                                an A plane formed each build (one more FP32 plane of device
+                                memory).  The FP32-cache modes add a first-order
+                                storage-rounding correction to N_elec (formed at context
+                                creation) */
   OCCXC_PREC_BF16X3 = 2,     /* FP32 AO cache, 3-term split-BF16 MFMA, FP32 accumulation */
   OCCXC_PREC_FP32_CACHE = 3  /* FP32 AO cache, FP64 density matrix, FP64 contractions */
 } occxc_precision;
