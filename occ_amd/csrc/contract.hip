@@ -1532,11 +1532,46 @@ __device__ __forceinline__ void mfma_f32_k4(f4 (&acc)[4][4], const float *x, con
 // grad rho (Phi and grad Phi of its columns read from HBM), then the
 // functional (the leaf's only item) or the stored partials.  s_phi (at least
 // 4 kMaxLeafPts + 4 kTileAo doubles of LDS) holds the sums.
+// The first row tile's loads (FP32 storage: all four column tiles) can be
+// issued before the epilogue (den_epi_first_loads, during the last k-chunk's
+// MFMAs) and passed in; PRE says they were.
+template <typename S> struct DenEpiFirst {
+  using V = typename Vec4Of<S>::type;
+  V f[4], fx[4], fy[4], fz[4];
+};
+
 template <typename S, bool GGA>
+__device__ __forceinline__ void den_epi_first_loads(const S *__restrict__ ao, const DenItem &it,
+                                                    const LeafDev &L, DenEpiFirst<S> &pre) {
+  using V = typename Vec4Of<S>::type;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, g = lane >> 4;
+  const int wp = wave & 1, wc = wave >> 1;
+  const int hp = it.np >> 1, hc = it.wc >> 1;
+  const int CT = hc >> 4;
+  const S *phi = ao + L.ao_off;
+  const long long cs = (long long)L.mp * L.np;
+  const int pl = wp * hp + 4 * g;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (j >= CT) break;
+    const long long off = (long long)(it.c0 + wc * hc + 16 * j + l16) * L.np + it.p0 + pl;
+    pre.f[j] = *reinterpret_cast<const V *>(phi + off);
+    if constexpr (GGA) {
+      pre.fx[j] = *reinterpret_cast<const V *>(phi + cs + off);
+      pre.fy[j] = *reinterpret_cast<const V *>(phi + 2 * cs + off);
+      pre.fz[j] = *reinterpret_cast<const V *>(phi + 3 * cs + off);
+    }
+  }
+}
+
+template <typename S, bool GGA, bool PRE = false>
 __device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *__restrict__ ao,
                                                   const DenItem &it, const LeafDev &L,
                                                   d4 (&acc)[4][4], double *__restrict__ part,
-                                                  double *s_phi, double *s_red) {
+                                                  double *s_phi, double *s_red,
+                                                  const DenEpiFirst<S> *pre = nullptr) {
+  static_assert(!PRE || sizeof(S) == 4, "the first row tile is passed in with FP32 storage");
   double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(s_phi);
   double(*s_h)[kTileAo] = reinterpret_cast<double(*)[kTileAo]>(s_phi + 4 * kMaxLeafPts);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -1567,6 +1602,15 @@ __device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *_
 #pragma unroll
       for (int jq = 0; jq < NJ; ++jq) {
         if (jj + jq >= CT) break;
+        if (PRE && i == 0) {
+          f[jq] = pre->f[jq];
+          if constexpr (GGA) {
+            fx[jq] = pre->fx[jq];
+            fy[jq] = pre->fy[jq];
+            fz[jq] = pre->fz[jq];
+          }
+          continue;
+        }
         const long long off = (long long)(it.c0 + wc * hc + 16 * (jj + jq) + l16) * L.np + it.p0 + pl;
         f[jq] = *reinterpret_cast<const V *>(phi + off);
         if constexpr (GGA) {
@@ -1832,6 +1876,7 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   const int nch = L.mp / kTKc;
+  DenEpiFirst<S> pre;
   for (int c = 0; c < P::NS - 1 && c < nch; ++c) dma(c * kTKc, s_stage + c * P::STAGE);
   tile_shape_ext(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
     constexpr int NR = decltype(R)::value, NC = decltype(C)::value;
@@ -1844,6 +1889,8 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
       else vmcnt_barrier<0>();
       const int nxt = cur == 0 ? P::NS - 1 : cur - 1;  // (c + NS - 1) % NS
       if (c + P::NS - 1 < nch) dma((c + P::NS - 1) * kTKc, s_stage + nxt * P::STAGE);
+      // the epilogue's first row tile loads under the last chunk's MFMAs
+      if (sizeof(S) == 4 && c == nch - 1) den_epi_first_loads<S, GGA>(ao, it, L, pre);
       const char *buf = s_stage + cur * P::STAGE;
       const S *sp = reinterpret_cast<const S *>(buf);
       const double *sd = reinterpret_cast<const double *>(buf + P::PHI);
@@ -1864,7 +1911,8 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
       cur = cur + 1 == P::NS ? 0 : cur + 1;
     }
   });
-  den_tile_epilogue<S, GGA>(a, ao, it, L, acc, part, reinterpret_cast<double *>(s_stage), s_red);
+  den_tile_epilogue<S, GGA, sizeof(S) == 4>(a, ao, it, L, acc, part,
+                                            reinterpret_cast<double *>(s_stage), s_red, &pre);
 }
 
 template <typename S, bool GGA>
