@@ -1532,12 +1532,14 @@ __device__ __forceinline__ void mfma_f32_k4(f4 (&acc)[4][4], const float *x, con
 // grad rho (Phi and grad Phi of its columns read from HBM), then the
 // functional (the leaf's only item) or the stored partials.  s_phi (at least
 // 4 kMaxLeafPts + 4 kTileAo doubles of LDS) holds the sums.
-// The first row tile's loads (FP32 storage: all four column tiles) can be
-// issued before the epilogue (den_epi_first_loads, during the last k-chunk's
-// MFMAs) and passed in; PRE says they were.
+// The epilogue's first loads (row tile 0's first NJ column tiles: all four
+// with FP32 storage, two with FP64) can be issued before it
+// (den_epi_first_loads, during the last k-chunk's MFMAs) and passed in; PRE
+// says they were.
 template <typename S> struct DenEpiFirst {
   using V = typename Vec4Of<S>::type;
-  V f[4], fx[4], fy[4], fz[4];
+  static constexpr int NJ = sizeof(S) == 4 ? 4 : 2;  // column tiles per load round
+  V f[NJ], fx[NJ], fy[NJ], fz[NJ];
 };
 
 template <typename S, bool GGA>
@@ -1553,7 +1555,7 @@ __device__ __forceinline__ void den_epi_first_loads(const S *__restrict__ ao, co
   const long long cs = (long long)L.mp * L.np;
   const int pl = wp * hp + 4 * g;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < DenEpiFirst<S>::NJ; ++j) {
     if (j >= CT) break;
     const long long off = (long long)(it.c0 + wc * hc + 16 * j + l16) * L.np + it.p0 + pl;
     pre.f[j] = *reinterpret_cast<const V *>(phi + off);
@@ -1571,7 +1573,6 @@ __device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *_
                                                   d4 (&acc)[4][4], double *__restrict__ part,
                                                   double *s_phi, double *s_red,
                                                   const DenEpiFirst<S> *pre = nullptr) {
-  static_assert(!PRE || sizeof(S) == 4, "the first row tile is passed in with FP32 storage");
   double(*s_q)[kMaxLeafPts] = reinterpret_cast<double(*)[kMaxLeafPts]>(s_phi);
   double(*s_h)[kTileAo] = reinterpret_cast<double(*)[kTileAo]>(s_phi + 4 * kMaxLeafPts);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -1602,7 +1603,7 @@ __device__ __forceinline__ void den_tile_epilogue(const BuildArgs &a, const S *_
 #pragma unroll
       for (int jq = 0; jq < NJ; ++jq) {
         if (jj + jq >= CT) break;
-        if (PRE && i == 0) {
+        if (PRE && i == 0 && jj == 0) {
           f[jq] = pre->f[jq];
           if constexpr (GGA) {
             fx[jq] = pre->fx[jq];
@@ -1889,8 +1890,8 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
       else vmcnt_barrier<0>();
       const int nxt = cur == 0 ? P::NS - 1 : cur - 1;  // (c + NS - 1) % NS
       if (c + P::NS - 1 < nch) dma((c + P::NS - 1) * kTKc, s_stage + nxt * P::STAGE);
-      // the epilogue's first row tile loads under the last chunk's MFMAs
-      if (sizeof(S) == 4 && c == nch - 1) den_epi_first_loads<S, GGA>(ao, it, L, pre);
+      // the epilogue's first loads under the last chunk's MFMAs
+      if (c == nch - 1) den_epi_first_loads<S, GGA>(ao, it, L, pre);
       const char *buf = s_stage + cur * P::STAGE;
       const S *sp = reinterpret_cast<const S *>(buf);
       const double *sd = reinterpret_cast<const double *>(buf + P::PHI);
@@ -1911,8 +1912,8 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
       cur = cur + 1 == P::NS ? 0 : cur + 1;
     }
   });
-  den_tile_epilogue<S, GGA, sizeof(S) == 4>(a, ao, it, L, acc, part,
-                                            reinterpret_cast<double *>(s_stage), s_red, &pre);
+  den_tile_epilogue<S, GGA, true>(a, ao, it, L, acc, part, reinterpret_cast<double *>(s_stage),
+                                  s_red, &pre);
 }
 
 template <typename S, bool GGA>
