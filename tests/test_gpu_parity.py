@@ -237,7 +237,13 @@ def test_cpp_driver_drop_in(tmp_path):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_batch_matches_single_and_oracle(precision):
     """Molecules built together in one batch context give their solo results
-    (same kernels, one launch) and, in FP64, the oracle's."""
+    (same kernels, one launch) and the oracle's within the stated tolerances
+    (in FP32 mode through the A pass and the folded scatter, per molecule).
+    FP32 mode's Vxc carries FP32 rounding of A and of the stored AO values,
+    relative to the entry: on this coarse grid (21 x 50) a few points near the
+    nucleus carry the core entries, so V[0, 0] of (H2O)2 (-1.94) is off by
+    1.2e-7 (6e-8 relative).  Its bound is 1e-7 times the largest |V| (at
+    least 1); Exc and N_elec keep the absolute tolerances."""
     from occ_amd.dft import XCBatch
     mols = [molecules.water(), molecules.alkane(2), molecules.water_cluster(2, seed=9)]
     bases = [AOBasis.load(m, "6-31g", pure=False) for m in mols]
@@ -251,11 +257,11 @@ def test_batch_matches_single_and_oracle(precision):
         assert abs(excs[i] - dft.exchange_correlation_energy()) <= 1e-11
         assert abs(nels[i] - dft.total_density()) <= 1e-11
         assert np.abs(Vs[i] - V).max() <= 1e-11
-        if precision == "fp64":
-            K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(b.flat()), pts, w,
-                                          [101, 130], Ds[i])
-            assert abs(excs[i] - exc) <= TOL_E and abs(nels[i] - nel) <= TOL_N
-            assert np.abs(Vs[i] - K).max() <= TOL_V
+        K, exc, nel = oracle.xc_build(oracle.OracleBasis.from_flat(b.flat()), pts, w,
+                                      [101, 130], Ds[i])
+        assert abs(excs[i] - exc) <= TOL_E and abs(nels[i] - nel) <= TOL_N
+        vscale = 1.0 if precision == "fp64" else max(1.0, float(np.abs(K).max()))
+        assert np.abs(Vs[i] - K).max() <= TOL_V * vscale
 
 
 def test_density_variants_agree(monkeypatch):

@@ -120,3 +120,24 @@ def test_fp32_vxc_forms_agree(monkeypatch, deriv_ids):
         _, _, V0, e0, n0 = out[0 if tile == 64 else 1]
         assert np.abs(V - V0).max() <= 5e-8, tile
         assert abs(e - e0) <= 1e-12 and abs(n - n0) <= 1e-12, tile
+
+
+def test_fp32_a_plane_is_accounted(monkeypatch):
+    """The FP32 mode's A plane (one FP32 value per leaf point and AO, beside
+    the four AO planes of a GGA) is counted in the context's device bytes:
+    with it, device_bytes grows by exactly ao_bytes / 4 over the one-kernel
+    form (OCCXC_VXC_APASS=0); FP64 and FP32-cache contexts hold no A plane."""
+    atoms = molecules.water_cluster(3, seed=2)
+    basis = AOBasis.load(atoms, "def2-svp", pure=True)
+    pts, wts = _grid(atoms)
+    info = {}
+    for precision in ("fp32", "fp32cache", "fp64"):
+        for apass in ("1", "0"):
+            monkeypatch.setenv("OCCXC_VXC_APASS", apass)
+            ctx = XCContext(basis, pts, wts, IDS, precision=precision)
+            info[precision, apass] = ctx.info()
+            ctx.close()
+    with_a, without = info["fp32", "1"], info["fp32", "0"]
+    assert with_a["device_bytes"] - without["device_bytes"] == with_a["ao_bytes"] / 4
+    for precision in ("fp32cache", "fp64"):
+        assert info[precision, "1"]["device_bytes"] == info[precision, "0"]["device_bytes"]
