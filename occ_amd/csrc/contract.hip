@@ -2376,6 +2376,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_gemm_f32(BuildArgs a, const floa
                                                          const VxcItem *__restrict__ items) {
   constexpr int NS = 2;  // chunk buffers (three, one workgroup per CU, measured slower)
   __shared__ __align__(16) char s_stage[NS * kVgStage];
+  static_assert(NS * kVgStage >= 2 * 64 * 64 * (int)sizeof(double), "scatter transposes fit");
   const VxcItem it = items[blockIdx.x];
   if (it.leaf < 0 || !a.leaf_active[it.leaf]) return;
   const LeafDev L = a.leaves[it.leaf];
