@@ -2307,8 +2307,10 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
 
 // FP32 mode, first step of the Vxc stage: A = sum_q c_q Phi_q of every active
 // leaf (the arithmetic of k_vxc_tile_f32's staging: FP64 sums of the FP32
-// values, rounded to FP32) into a.ao_a at the leaf's res_off, for all mp rows
-// and np points.  One workgroup per leaf; np / 4 lanes per AO row (4 points
+// values, rounded to FP32) into a.ao_a at the leaf's res_off, for the leaf's
+// m AOs and np points.  Rows m..mp-1 are left as they are: the contraction
+// reads them, but each of its outputs pairs one row of Phi with one row of
+// A, and the outputs of rows past m are never added into K.  One workgroup per leaf; np / 4 lanes per AO row (4 points
 // each), so a lane's coefficients are fixed.  HBM-bound: 16 bytes read and 4
 // written per element (GGA), both streamed past the caches (non-temporal:
 // 0.4 ms of 6 on (H2O)128 / def2-SVP).
@@ -2331,7 +2333,7 @@ __global__ __launch_bounds__(256) void k_vxc_apass(BuildArgs a, const float *__r
   const long long cs = (long long)L.mp * L.np;
   float *out = a.ao_a + L.res_off;
 #pragma unroll 2
-  for (int mu = row0; mu < L.mp; mu += rpp) {
+  for (int mu = row0; mu < L.m; mu += rpp) {
     const long long o = (long long)mu * L.np + p;
     const f4 f = __builtin_nontemporal_load(reinterpret_cast<const f4 *>(phi + o));
     f4 fx, fy, fz;
