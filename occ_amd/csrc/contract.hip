@@ -1478,9 +1478,17 @@ __device__ __forceinline__ void tile_shape(int rt, int ct, F &&f) {
 // The same with the 2-tile shapes too (blocks of 64 AOs or points), for the
 // kernels of the default path: without them a 64-wide block runs as 4 tiles,
 // 2.6 % (density) and 8.5 % (Vxc) of the MFMA work on (H2O)128 / def2-SVP.
+// Those kernels pass each wavefront's live tiles (live_tiles), and 0 runs
+// no MFMA.
+
+// 16-row tiles of a wavefront's half block (half rows) that hold any of the
+// `valid` rows counted from its first row: the tiles past a leaf's real AOs
+// or points hold padding only (zero AO rows, zero-weight points) and are not
+// computed.  One tile runs as the 2-tile shape (the shapes start at 2).
 template <typename F>
 __device__ __forceinline__ void tile_shape_ext(int rt, int ct, F &&f) {
-  if (rt == 4 && ct == 3) f(IntC<4>{}, IntC<3>{});
+  if (rt == 0 || ct == 0) f(IntC<0>{}, IntC<0>{});  // no tile: loads and barriers only
+  else if (rt == 4 && ct == 3) f(IntC<4>{}, IntC<3>{});
   else if (rt == 3 && ct == 4) f(IntC<3>{}, IntC<4>{});
   else if (rt == 3 && ct == 3) f(IntC<3>{}, IntC<3>{});
   else if (rt == 4 && ct == 2) f(IntC<4>{}, IntC<2>{});
@@ -1489,6 +1497,11 @@ __device__ __forceinline__ void tile_shape_ext(int rt, int ct, F &&f) {
   else if (rt == 2 && ct == 3) f(IntC<2>{}, IntC<3>{});
   else if (rt == 2 && ct == 2) f(IntC<2>{}, IntC<2>{});
   else f(IntC<4>{}, IntC<4>{});
+}
+
+__device__ __forceinline__ int live_tiles(int valid, int half) {
+  const int t = (min(max(valid, 0), half) + 15) >> 4;
+  return t == 1 ? 2 : t;
 }
 
 // Two FP64 k-steps of acc[i][j] += X_i^T Y_j.  x and y point at this lane's
@@ -1820,7 +1833,6 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
   const int l16 = lane & 15, g = lane >> 4;
   const int wp = wave & 1, wc = wave >> 1;
   const int hp = it.np >> 1, hc = it.wc >> 1;
-  const int RT = hp >> 4, CT = hc >> 4;
   const int pa = 4 * (l16 & 3) + (l16 >> 2);
   const S *phi = ao + L.ao_off;
   const double *Dm = a.D2 + MD.moff;
@@ -1876,10 +1888,13 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-  const int nch = L.mp / kTKc;
+  // AO chunks up to the leaf's real AOs (the padding rows of Phi are zero);
+  // the wavefront's tiles that hold real points and real AO columns
+  const int nch = (L.m + kTKc - 1) / kTKc;
+  const int RTl = live_tiles(L.n - (it.p0 + wp * hp), hp), CTl = live_tiles(L.m - (it.c0 + wc * hc), hc);
   DenEpiFirst<S> pre;
   for (int c = 0; c < P::NS - 1 && c < nch; ++c) dma(c * kTKc, s_stage + c * P::STAGE);
-  tile_shape_ext(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+  tile_shape_ext(RTl, CTl, [&](auto R, auto C) __attribute__((always_inline)) {
     constexpr int NR = decltype(R)::value, NC = decltype(C)::value;
     int cur = 0;  // buffer of chunk c
     for (int c = 0; c < nch; ++c) {
@@ -1895,6 +1910,7 @@ __device__ __forceinline__ void den_tile_item_dma(const BuildArgs &a, const S *_
       const char *buf = s_stage + cur * P::STAGE;
       const S *sp = reinterpret_cast<const S *>(buf);
       const double *sd = reinterpret_cast<const double *>(buf + P::PHI);
+      if constexpr (NR > 0)
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
         const int kk = 4 * g + ks;
@@ -2391,7 +2407,6 @@ __global__ __launch_bounds__(256, 2) void k_vxc_gemm_f32(BuildArgs a, const floa
   const int wr = wave & 1, wcw = wave >> 1;
   const bool diag = it.r0 == it.c0;
   const int hr = it.wr >> 1, hc = it.wc >> 1;
-  const int RT = hr >> 4, CT = hc >> 4;
   const float *phi = ao + L.ao_off;
   const float *am = a.ao_a + L.res_off;
   // loads: wavefront w fills 16-row groups w and w + 4 of each block, lane l
@@ -2418,7 +2433,10 @@ __global__ __launch_bounds__(256, 2) void k_vxc_gemm_f32(BuildArgs a, const floa
       }
     }
   };
-  const int nch = L.np / kVgP;  // np is a multiple of 64: an even count
+  // chunks up to the leaf's real points (A is zero at the padding points);
+  // the wavefront's tiles that hold real AO rows and columns
+  const int nch = (L.n + kVgP - 1) / kVgP;
+  const int RTl = live_tiles(L.m - (it.r0 + wr * hr), hr), CTl = live_tiles(L.m - (it.c0 + wcw * hc), hc);
   if (nch > 0) dma(0, s_stage);
   d4 acc[4][4];
 #pragma unroll
@@ -2427,7 +2445,7 @@ __global__ __launch_bounds__(256, 2) void k_vxc_gemm_f32(BuildArgs a, const floa
     for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   const int sw = 4 * (g ^ vxc_swz(l16));
   const int xo = (wr * hr + l16) * kVgP + sw, yo = (wcw * hc + l16) * kVgP + sw;
-  tile_shape_ext(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
+  tile_shape_ext(RTl, CTl, [&](auto R, auto C) __attribute__((always_inline)) {
     constexpr int NR = decltype(R)::value, NC = decltype(C)::value;
     f4 acf[4][4];
 #pragma unroll
@@ -2442,9 +2460,11 @@ __global__ __launch_bounds__(256, 2) void k_vxc_gemm_f32(BuildArgs a, const floa
       if (c + 1 < nch) dma((c + 1) * kVgP, s_stage + (cur ^ 1) * kVgStage);
       const float *pr = reinterpret_cast<const float *>(s_stage + cur * kVgStage);
       const float *ar = pr + kTileAo * kVgP, *pc = ar + kTileAo * kVgP, *ac = pc + kTileAo * kVgP;
-      mfma_f32_k4<NR, NC, kVgP>(acf, pr + xo, (diag ? ar : ac) + yo);
-      if (!diag) mfma_f32_k4<NR, NC, kVgP>(acf, ar + xo, pc + yo);
-      if (c & 1) {
+      if constexpr (NR > 0) {
+        mfma_f32_k4<NR, NC, kVgP>(acf, pr + xo, (diag ? ar : ac) + yo);
+        if (!diag) mfma_f32_k4<NR, NC, kVgP>(acf, ar + xo, pc + yo);
+      }
+      if ((c & 1) || c + 1 == nch) {
 #pragma unroll
         for (int i = 0; i < NR; ++i)
 #pragma unroll
