@@ -2172,10 +2172,12 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile(BuildArgs a, const S *__res
   // one point loop per tile shape
   tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
   int buf = 0;
-  for (int p0 = 0; p0 < L.np; p0 += kTVp, buf ^= 1) {
+  // chunks up to the leaf's real points: A is zero at the zero-weight
+  // padding points
+  for (int p0 = 0; p0 < L.n; p0 += kTVp, buf ^= 1) {
     __syncthreads();
     // the next chunk's coefficients (read after the barrier below)
-    if (p0 + kTVp < L.np && t < NQ * kTVp)
+    if (p0 + kTVp < L.n && t < NQ * kTVp)
       s_c[buf ^ 1][t / kTVp][t % kTVp] = cf[(t / kTVp) * a.ptot + p0 + kTVp + t % kTVp];
     for (int gi = t; gi < nrow * (kTVp / 4); gi += 256) {
       const int row = gi >> 2, p = 4 * (gi & 3);
@@ -2267,9 +2269,10 @@ __global__ __launch_bounds__(256, 2) void k_vxc_tile_f32(BuildArgs a, const floa
   // one point loop per tile shape
   tile_shape(RT, CT, [&](auto R, auto C) __attribute__((always_inline)) {
   int buf = 0;
-  for (int p0 = 0; p0 < L.np; p0 += kTVpf, buf ^= 1) {
+  // chunks up to the leaf's real points (A is zero at the padding points)
+  for (int p0 = 0; p0 < L.n; p0 += kTVpf, buf ^= 1) {
     __syncthreads();
-    if (p0 + kTVpf < L.np && t < NQ * kTVpf)
+    if (p0 + kTVpf < L.n && t < NQ * kTVpf)
       s_c[buf ^ 1][t / kTVpf][t % kTVpf] = cf[(t / kTVpf) * a.ptot + p0 + kTVpf + t % kTVpf];
     for (int gi = t; gi < nrow * (kTVpf / 4); gi += 256) {
       const int row = gi >> 3, p = 4 * (gi & 7);
