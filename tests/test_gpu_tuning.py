@@ -109,15 +109,15 @@ def test_fp32_vxc_forms_agree(monkeypatch, deriv_ids):
     out = []
     for apass in ("1", "0"):
         monkeypatch.setenv("OCCXC_VXC_APASS", apass)
-        for tile in (64, 128):
+        for tile in (32, 64, 128):
             ctx = XCContext(basis, pts, wts, ids, precision="fp32", tile_ao=tile)
             V, e, n = ctx.build(D)
             ctx.close()
             assert abs(e - exc) <= 1e-6 and abs(n - nel) <= 1e-8 and np.abs(V - K).max() <= 1e-7
             assert np.abs(V - V.T).max() == 0.0
             out.append((apass, tile, V, e, n))
-    for apass, tile, V, e, n in out[2:]:
-        _, _, V0, e0, n0 = out[0 if tile == 64 else 1]
+    for apass, tile, V, e, n in out[3:]:
+        _, _, V0, e0, n0 = out[(32, 64, 128).index(tile)]
         assert np.abs(V - V0).max() <= 5e-8, tile
         assert abs(e - e0) <= 1e-12 and abs(n - n0) <= 1e-12, tile
 
